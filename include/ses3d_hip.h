@@ -1,0 +1,185 @@
+/*
+ * ses3d_hip.h -- C ABI of the MI355X (gfx950) time-stepping library that
+ * replaces Ses3d-NT's forward hot path.
+ *
+ * The reference exposes this path as a Fortran module API, not a plugin
+ * (reference src/evolution.f90:41, src/elastic_vars.F90:89-94,
+ * src/config/channel_mod.f90:433-446). Each entry point below names the
+ * reference procedure it stands in for. Everything is plain C: opaque
+ * handles, plain pointers and sizes, int status codes (0 = success, nonzero
+ * = error; s3d_last_error() gives the message). The library owns device
+ * memory; the caller owns host buffers. Calls for one context come from one
+ * host thread and are ordered on the context's HIP stream.
+ *
+ * 6-D arrays exchanged with the host are float64 in the reference layout:
+ * column-major f(0:nx-1, 0:ny-1, 0:nz-1, 0:p, 0:p, 0:p), element index ex
+ * fastest, GLL index k slowest (reference src/elastic_vars.F90:112). Here
+ * nx, ny, nz are element COUNTS (the reference's nx_loc is count-1,
+ * src/config/grid.F90:233).
+ */
+#ifndef SES3D_HIP_H
+#define SES3D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mesh and time axis: the &grid, &model and &time NAMELIST groups
+ * (reference src/parser/parser.f90:148-260). */
+typedef struct s3d_grid {
+    int nx, ny, nz;        /* elements along theta, phi, r            */
+    int lpd;               /* Lagrange polynomial degree, 2..7        */
+    double lat_min, lat_max, lon_min, lon_max;  /* degrees            */
+    double rad_min, rad_max;                    /* metres             */
+    int taper_width;       /* absorbing strip, elements               */
+    double taper_slope;
+    double dt;             /* seconds                                 */
+    int nt;                /* number of time steps                    */
+} s3d_grid;
+
+/* Arithmetic of the time step. These have no counterpart in the reference,
+ * whose precision is the compile-time real_kind (src/parameters.F90:49). */
+enum { S3D_FP64 = 0, S3D_FP32 = 1 };
+typedef struct s3d_options {
+    int precision;         /* S3D_FP64 or S3D_FP32                        */
+    int kahan_v;           /* FP32: compensated velocity integration      */
+    int kahan_sigma;       /* FP32: compensated stress integration        */
+    int device;            /* HIP device ordinal                          */
+    int xcd_remap;         /* 1: give each XCD a contiguous element range */
+} s3d_options;
+
+/* Model parameter ids (reference src/model_paras.F90:39-46). */
+enum { S3D_RHOINV = 0, S3D_LAMBDA, S3D_MU, S3D_A, S3D_B, S3D_C, S3D_Q };
+
+/* Field ids for s3d_get_field / s3d_set_field (reference
+ * src/elastic_vars.F90:89-94; channel field_* views,
+ * src/config/channel_mod.f90:505-1189). */
+enum {
+    S3D_VX = 0, S3D_VY, S3D_VZ,
+    S3D_SXX, S3D_SXY, S3D_SXZ, S3D_SYY, S3D_SYZ, S3D_SZZ,  /* *_pml stresses */
+    S3D_MM = 18, S3D_TAPER = 19,
+    S3D_MXX = 32   /* memory variable: S3D_MXX + 6*mechanism + comp,
+                      comp order xx, xy, xz, yy, yz, zz; mechanism 0..2 */
+};
+
+/* Receiver channels (reference src/config/channel_mod.f90:270-336). */
+enum {
+    S3D_CH_VX = 0, S3D_CH_VY, S3D_CH_VZ, S3D_CH_N, S3D_CH_E, S3D_CH_Z,
+    S3D_CH_PRESSURE
+};
+
+/* Wavelets (reference src/misc/source_signature_mod.f90:50-147). */
+enum { S3D_RICKER = 0, S3D_GAUSS, S3D_SIN3, S3D_BRUNE, S3D_DELTA };
+
+typedef struct s3d_ctx s3d_ctx;
+
+/* Last error message of the calling thread. */
+const char *s3d_last_error(void);
+
+/* Number of HIP devices visible (0 when none). */
+int s3d_device_count(int *count);
+
+/* Allocate the device state of one simulation.
+ * Replaces allocate_geometric_parameters, allocate_model_parameters and
+ * allocate_elastic_variables (reference src/ses3d-nt.F90:128-146). */
+int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out);
+
+/* Release everything the context owns. */
+int s3d_destroy(s3d_ctx *ctx);
+
+/* Model input: element layers ez0 .. ez0+nez-1 of one parameter, float64 in
+ * the reference layout with nz replaced by nez. Parameters never set keep the
+ * reference's defaults (0; Q = HUGE, i.e. elastic), src/model_paras.F90:70-76.
+ * Replaces read_model_parameters / mpi_read_parallel
+ * (reference src/model_paras.F90:97-137). */
+int s3d_set_model_slab(s3d_ctx *ctx, int param, int ez0, int nez,
+                       const double *data);
+int s3d_set_model(s3d_ctx *ctx, int param, const double *data);
+
+/* Geometry, taper, auxiliary model parameters, assembled mass matrix and the
+ * derivative matrix, all in FP64, then rounded once to the step precision.
+ * Replaces init_geometric_parameters (src/geometric_paras.f90:108),
+ * init_auxiliary_model_parameters (src/model_paras.F90:389) and init_temp
+ * (src/evolution.f90:48). Must follow the model upload. */
+int s3d_init(s3d_ctx *ctx);
+
+/* Courant number of the model for grid->dt (reference cfl_loc,
+ * src/model_paras.F90:529-561). Requires s3d_init. */
+int s3d_cfl(s3d_ctx *ctx, double *courant);
+
+/* GLL points, weights and derivative matrix dl[i + (p+1)*j] = l_i'(xi_j)
+ * (reference get_knots / get_weights / get_dlgll, src/gll_mod.f90:178-275). */
+int s3d_gll(int lpd, double *knots, double *weights, double *dl);
+
+/* Point location: element (0-based ex, ey, ez) and the (p+1)^3 Lagrange
+ * weights vm (column-major i, j, k) of a point given by lat, lon (degrees) and
+ * depth (metres). Returns 0 when the point is inside the domain, 1 when not.
+ * Replaces construct_point_cls (src/config/point.f90:150-254). */
+int s3d_locate(s3d_ctx *ctx, double lat, double lon, double depth,
+               int *exyz, double *vm);
+
+/* Source-time function sampled at t = (i-1)*dt, i = 1..nt (reference
+ * make_source_signature_, src/config/source.f90:564-652). */
+int s3d_wavelet(int wavelet, double onset, double width, double dt, int nt,
+                double *out);
+
+/* Moment-tensor point source: stf6[c*nt + it-1] is channel c (xx, xy, xz,
+ * yy, yz, zz) at step it. All-zero channels are dropped, and a later source
+ * overwrites an earlier one's component in the same element, like the
+ * reference's assignment in add_m** (src/config/source.f90:274-327,
+ * src/config/channel_mod.f90:859-999). */
+int s3d_add_source_mt(s3d_ctx *ctx, const int *exyz, const double *vm,
+                      const double *stf6);
+/* Single-force point source: stf3[c*nt + it-1] for x, y, z
+ * (src/config/channel_mod.f90:1023-1095). */
+int s3d_add_source_sf(s3d_ctx *ctx, const int *exyz, const double *vm,
+                      const double *stf3);
+
+/* Receiver with nch channel codes (S3D_CH_*); returns its index in *id.
+ * Replaces parse_receiver_file's channel setup (src/config/receiver.f90:435-492). */
+int s3d_add_receiver(s3d_ctx *ctx, const int *exyz, const double *vm,
+                     int nch, const int *channels, int *id);
+
+/* Advance nsteps iterations of the forward loop body
+ * (reference src/ses3d-nt.F90:240-270): ses3d_evolution, every source's
+ * add(it) and every receiver's record(it). */
+int s3d_step(s3d_ctx *ctx, int nsteps);
+
+/* Iterations completed so far. */
+int s3d_current_step(s3d_ctx *ctx, int *it);
+
+/* nt samples of one receiver channel (reference channel data(:),
+ * src/config/channel_mod.f90:420-426). Samples not yet recorded are 0. */
+int s3d_get_trace(s3d_ctx *ctx, int receiver, int channel, double *out);
+
+/* Full-field copy to / from the host in the reference layout
+ * (reference field_* views and volume_snapshot get_field,
+ * src/volume_snapshot.f90:192-298). Setting a velocity component takes the
+ * value of each unique node from the highest-index element that holds it. */
+int s3d_get_field(s3d_ctx *ctx, int field, double *out6d);
+int s3d_set_field(s3d_ctx *ctx, int field, const double *in6d);
+
+/* min and max of vx, vy, vz (reference v_ptp_loc, src/elastic_vars.F90:179-185). */
+int s3d_vrange(s3d_ctx *ctx, double *out2);
+
+/* Kinetic energy (reference kinetic_energy, src/elastic_vars.F90:219-246). */
+int s3d_kinetic_energy(s3d_ctx *ctx, double *out);
+
+/* Block until all work queued on the context's stream is done. */
+int s3d_synchronize(s3d_ctx *ctx);
+
+/* Kernel timing with HIP events on the context's stream: while enabled,
+ * every launch of the element and node kernels is bracketed by events.
+ * s3d_kernel_times returns the summed milliseconds and launch counts of
+ * (0) the element kernel, (1) the node kernel, (2) the receiver kernel since
+ * the last reset. No counterpart in the reference (which prints no timing). */
+int s3d_timing_enable(s3d_ctx *ctx, int enable);
+int s3d_kernel_times(s3d_ctx *ctx, double *ms3, int *count3);
+
+/* Device bytes held by the context. */
+int s3d_device_bytes(s3d_ctx *ctx, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,128 @@
+/*
+ * s3d_oracle.h -- FP64 CPU restatement of the Ses3d-NT forward time step.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * may load it. The product (ses3d-nt_amd/) never links or calls it.
+ *
+ * Data layout follows the reference exactly: every field is a 6-D column-major
+ * array f(0:nx-1, 0:ny-1, 0:nz-1, 0:p, 0:p, 0:p) of doubles, element index ex
+ * fastest, GLL index k slowest (reference src/elastic_vars.F90:112).
+ * Here nx, ny, nz are element COUNTS (the reference stores count-1 as nx_loc,
+ * src/config/grid.F90:233).
+ */
+#ifndef S3D_ORACLE_H
+#define S3D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_config {
+    int nx, ny, nz, lpd;               /* &grid nx, ny, nz, lpd            */
+    double lat_min, lat_max;           /* &model, degrees                  */
+    double lon_min, lon_max;           /* &model, degrees                  */
+    double rad_min, rad_max;           /* &model, metres                   */
+    int taper_width;                   /* &grid taper_width (elements)     */
+    double taper_slope;                /* &grid taper_slope                */
+    double dt;                         /* &time dt                         */
+    int nt;                            /* &time nt                         */
+} ora_config;
+
+/* Field identifiers for ora_get_field / ora_set_field. */
+enum {
+    ORA_VX = 0, ORA_VY, ORA_VZ,
+    ORA_SXX, ORA_SXY, ORA_SXZ, ORA_SYY, ORA_SYZ, ORA_SZZ,  /* *_pml stress */
+    ORA_EXX, ORA_EYY, ORA_EZZ, ORA_EXY, ORA_EXZ, ORA_EYZ,  /* strain rate  */
+    ORA_EYX, ORA_EZX, ORA_EZY,
+    ORA_MM, ORA_TAPER, ORA_THETA, ORA_PHI, ORA_R,
+    ORA_SX, ORA_SY, ORA_SZ,           /* assembled acceleration of last step */
+    ORA_MXX = 32, /* memory variables: ORA_MXX + 6*(mech) + comp, comp order
+                     xx, xy, xz, yy, yz, zz, mech 0..2 */
+    ORA_NFIELDS = 64
+};
+
+/* Model parameter identifiers (reference src/model_paras.F90:39-46). */
+enum { ORA_RHOINV = 0, ORA_LAMBDA, ORA_MU, ORA_A, ORA_B, ORA_C, ORA_Q };
+
+/* Receiver channel codes (reference src/config/channel_mod.f90:270-336). */
+enum {
+    ORA_CH_VX = 0, ORA_CH_VY, ORA_CH_VZ, ORA_CH_N, ORA_CH_E, ORA_CH_Z,
+    ORA_CH_PRESSURE, ORA_CH_ROT_VX, ORA_CH_ROT_VY, ORA_CH_ROT_VZ
+};
+
+typedef struct ora_ctx ora_ctx;
+
+/* GLL tables (reference src/gll_mod.f90:61-275). knots/weights hold p+1
+ * values, dl holds (p+1)^2 values, dl[i + (p+1)*j] = l_i'(xi_j). */
+int ora_gll(int lpd, double *knots, double *weights, double *dl);
+
+ora_ctx *ora_create(const ora_config *cfg);
+void ora_destroy(ora_ctx *c);
+int ora_set_threads(int n);
+
+/* Upload one 6-D model parameter. Unset parameters keep the reference's
+ * allocation defaults: 0, and Q = HUGE (src/model_paras.F90:70-76). */
+int ora_set_model(ora_ctx *c, int param, const double *field6d);
+
+/* Geometry, taper, auxiliary model parameters, assembled mass matrix and the
+ * derivative matrix (reference ses3d-nt.F90:128-150). Must follow the model. */
+int ora_init(ora_ctx *c);
+
+/* Courant number for cfg->dt (reference src/model_paras.F90:529-561). */
+double ora_cfl(ora_ctx *c);
+
+/* Point location (reference src/config/point.f90:150-254).
+ * Returns 0 and fills exyz[3] (0-based) and vm[(p+1)^3] (column-major i,j,k)
+ * when the point lies in the domain, 1 otherwise. */
+int ora_locate(ora_ctx *c, double lat, double lon, double depth,
+               int *exyz, double *vm);
+
+/* Source-time function sample vector t = (i-1)*dt, i = 1..nt
+ * (reference src/config/source.f90:564-652). wavelet: 0 RICKER, 1 GAUSS,
+ * 2 SIN**3, 3 BRUNE, 4 DELTA. */
+int ora_stf(int wavelet, double onset, double width, double dt, int nt,
+            double *out);
+
+/* Register a moment-tensor source: stf6[comp*nt + it-1] is the channel data of
+ * component comp (xx, xy, xz, yy, yz, zz). All-zero channels are dropped like
+ * the reference (src/config/source.f90:320-323). */
+int ora_add_source_mt(ora_ctx *c, const int *exyz, const double *vm,
+                      const double *stf6);
+/* Single-force source: stf3[comp*nt + it-1] for x, y, z. */
+int ora_add_source_sf(ora_ctx *c, const int *exyz, const double *vm,
+                      const double *stf3);
+
+/* Register a receiver with nch channels; returns receiver index or -1. */
+int ora_add_receiver(ora_ctx *c, const int *exyz, const double *vm,
+                     int nch, const int *channels);
+
+/* Advance n iterations of the forward loop body (reference
+ * src/ses3d-nt.F90:240-270): evolution, source add(it), record(it). */
+int ora_step(ora_ctx *c, int n);
+
+/* Run only ses3d_evolution once (no source add, no record). */
+int ora_evolution(ora_ctx *c);
+
+int ora_get_field(ora_ctx *c, int field, double *out6d);
+int ora_set_field(ora_ctx *c, int field, const double *in6d);
+/* Trace of receiver rec, channel ch: nt samples. */
+int ora_get_trace(ora_ctx *c, int rec, int ch, double *out);
+int ora_current_it(ora_ctx *c);
+/* min/max of v over all element-local nodes (src/elastic_vars.F90:179-185). */
+int ora_vrange(ora_ctx *c, double *out2);
+/* kinetic energy (src/elastic_vars.F90:219-246). */
+double ora_kinetic_energy(ora_ctx *c);
+/* derived scalars: dx_elm, dy_elm, dz_elm, jac, x_loc_min, x_loc_max,
+ * y_loc_min, y_loc_max, z_loc_min, z_loc_max */
+int ora_scalars(ora_ctx *c, double *out10);
+/* 1-D coordinate tables theta(ex,i), phi(ey,j), r(ez,k), ex fastest. */
+int ora_coords_1d(ora_ctx *c, double *theta, double *phi, double *r);
+/* 1-D taper profiles before corner normalisation, same shape as coords. */
+int ora_profiles_1d(ora_ctx *c, double *px, double *py, double *pz);
+int ora_visco_enabled(ora_ctx *c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

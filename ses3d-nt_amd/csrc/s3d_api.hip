@@ -1,0 +1,1182 @@
+// s3d_api.hip -- C ABI (include/ses3d_hip.h) over the gfx950 kernels.
+//
+// Host-side setup runs in FP64 with the reference's formulas (cited per
+// function); the step precision (FP64 or FP32) only applies to the state and
+// to the rounded static tables the kernels read.
+#include "../../include/ses3d_hip.h"
+#include "s3d_internal.h"
+#include "s3d_kernels.hip"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace s3d;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &m)
+{
+    g_err = m;
+    return 1;
+}
+
+#define HIPCHK(x)                                                                  \
+    do {                                                                           \
+        hipError_t e_ = (x);                                                       \
+        if (e_ != hipSuccess)                                                      \
+            return fail(std::string(#x) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+constexpr double PI = 3.141592653589793238462643383279502884;
+constexpr double EARTH_RADIUS = 6371000.0;   // src/parameters.F90:64
+constexpr double TAU_P[NRDISS] = {1000.0, 70.0, 5.25};  // src/geometric_paras.f90:42
+constexpr double HUGE64 = 1.79769313486231570815e+308;
+
+// -------------------------------------------------------------------------
+// GLL basis (src/gll_mod.f90:61-275): points are -1, 1 and the roots of
+// P_N' found by Newton iteration in long double; weights 2/(N(N+1)P_N^2).
+// -------------------------------------------------------------------------
+void legendre(int n, long double x, long double *pn, long double *dpn)
+{
+    long double p0 = 1.0L, p1 = x;
+    for (int k = 2; k <= n; ++k) {
+        long double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    *pn = p1;
+    *dpn = (long double)n * (p0 - x * p1) / (1.0L - x * x);
+}
+
+int gll_tables(int N, double *knots, double *weights, double *dl)
+{
+    if (N < 1 || N > 7) return 1;
+    long double xs[MAXP1];
+    xs[0] = -1.0L;
+    xs[N] = 1.0L;
+    for (int i = 1; i < N; ++i) {
+        long double x = -cosl((long double)PI * i / N);
+        for (int it = 0; it < 100; ++it) {
+            long double pn, dpn;
+            legendre(N, x, &pn, &dpn);
+            long double d2 = (2.0L * x * dpn - (long double)N * (N + 1) * pn) / (1.0L - x * x);
+            long double step = dpn / d2;
+            x -= step;
+            if (fabsl(step) < 1e-30L) break;
+        }
+        xs[i] = x;
+    }
+    for (int i = 0; i <= N / 2; ++i) {
+        long double m = 0.5L * (xs[N - i] - xs[i]);
+        xs[i] = -m;
+        xs[N - i] = m;
+    }
+    if (N % 2 == 0) xs[N / 2] = 0.0L;
+    for (int i = 0; i <= N; ++i) {
+        knots[i] = (double)xs[i];
+        long double pn, dpn;
+        legendre(N, xs[i], &pn, &dpn);
+        weights[i] = (double)(2.0L / ((long double)N * (N + 1) * pn * pn));
+    }
+    if (N == 1) weights[0] = weights[1] = 0.5;   // reference placeholder values
+    if (dl) {
+        for (int i = 0; i <= N; ++i)
+            for (int j = 0; j <= N; ++j) {
+                double y;
+                if (i == j) {
+                    y = 0.0;
+                    for (int k = 0; k <= N; ++k)
+                        if (k != i) y += 1.0 / (knots[i] - knots[k]);
+                } else {
+                    double num = 1.0, den = 1.0;
+                    for (int k = 0; k <= N; ++k) {
+                        if (k == i) continue;
+                        if (k != j) num *= knots[j] - knots[k];
+                        den *= knots[i] - knots[k];
+                    }
+                    y = num / den;
+                }
+                dl[i + (N + 1) * j] = y;
+            }
+    }
+    return 0;
+}
+
+// coordinates_1d (src/coordinates_mod.f90:114-141); out[e*P1 + j]
+void coords_1d(const double *knots, int p, int elms, double mn, double mx, double *out)
+{
+    const int P1 = p + 1;
+    double delta = (mx - mn) / elms;
+    for (int i = 0; i < elms; ++i)
+        for (int j = 0; j < P1; ++j) out[i * P1 + j] = mn + ((double)i + 0.5 * (1.0 + knots[j])) * delta;
+    out[(elms - 1) * P1 + p] = mx;
+}
+
+double deg2rad(double d) { return d * PI / 180.0; }   // coordinate_utilities_mod.f90:74-84
+
+void lagrange_1d(const double *knots, int P1, double x, double *l)
+{
+    // src/interpolation_mod.f90:83-171
+    double w[MAXP1];
+    for (int j = 0; j < P1; ++j) {
+        double a = 1.0, b = 1.0;
+        for (int k = 0; k < j; ++k) a *= knots[j] - knots[k];
+        for (int k = j + 1; k < P1; ++k) b *= knots[j] - knots[k];
+        w[j] = 1.0 / (a * b);
+    }
+    for (int j = 0; j < P1; ++j)
+        if (x == knots[j]) {
+            for (int q = 0; q < P1; ++q) l[q] = q == j ? 1.0 : 0.0;
+            return;
+        }
+    double prod = 1.0;
+    for (int k = 0; k < P1; ++k) prod *= x - knots[k];
+    for (int j = 0; j < P1; ++j) l[j] = prod / (x - knots[j]) * w[j];
+}
+
+}  // namespace
+
+struct Slot {
+    int e;
+    int chan[9];
+    std::vector<double> coef;   // 9 * NP
+};
+
+struct s3d_ctx {
+    s3d_grid g;
+    s3d_options o;
+    int p, P1, NP, PB;
+    long nel, nn, nblk, nuniq;
+    bool fp32 = false, visco = false, aniso = false, inited = false;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    double *stage[7] = {};
+    bool q_finite = false, abc_nonzero = false;
+    // geometry (host, FP64)
+    double knots[MAXP1], weights[MAXP1], dl[MAXP1 * MAXP1];
+    double x_glb_min, x_glb_max, y_glb_min, y_glb_max, z_glb_min, z_glb_max;
+    double dx, dy, dz, jac, x_loc_min, x_loc_max, y_loc_min, y_loc_max, z_loc_min, z_loc_max;
+    std::vector<double> x1, y1, z1, px1, py1, pz1;
+    double courant = 0.0;
+    // device state
+    void *v[3] = {}, *lo[3] = {}, *inv_mm = nullptr, *sig = nullptr, *sig_lo = nullptr,
+         *mem = nullptr, *lam = nullptr, *mu = nullptr, *tau = nullptr, *rho = nullptr,
+         *ca = nullptr, *cb = nullptr, *cc = nullptr, *force = nullptr, *tth = nullptr,
+         *tr = nullptr, *wdev = nullptr;
+    double *mm64 = nullptr, *dpx = nullptr, *dpy = nullptr, *dpz = nullptr;
+    double bytes = 0;
+    // sources
+    std::vector<int> slot_of;
+    std::vector<Slot> slots;
+    std::vector<double> stf;   // nchan * nt
+    int nchan = 0;
+    bool src_dirty = false;
+    int *d_slot = nullptr, *d_chan = nullptr;
+    double *d_coef = nullptr, *d_stf = nullptr;
+    // receivers
+    std::vector<int> rec_e, rec_nch, rec_ch;
+    std::vector<double> rec_vm;
+    bool rec_dirty = false;
+    int *d_rec_e = nullptr, *d_rec_nch = nullptr, *d_rec_ch = nullptr;
+    double *d_rec_vm = nullptr, *d_traces = nullptr;
+    int nrec_dev = 0;
+    int it = 0;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> ev[3];   // start/stop pairs per kernel class
+    double ms[3] = {0, 0, 0};
+    int cnt[3] = {0, 0, 0};
+};
+
+namespace {
+
+int dalloc(s3d_ctx *c, void **p, size_t bytes, bool zero = true)
+{
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail("hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+    }
+    c->bytes += (double)bytes;
+    if (zero) {
+        e = hipMemsetAsync(*p, 0, bytes ? bytes : 16, c->stream);
+        if (e != hipSuccess) return fail(std::string("hipMemsetAsync: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+void dfree(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+size_t rsize(const s3d_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double); }
+
+inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---- template dispatch helpers ----
+template <int P, typename R>
+ElemArgs<R> elem_args(s3d_ctx *c, int it)
+{
+    ElemArgs<R> a;
+    std::memset(&a, 0, sizeof(a));
+    const int P1 = P + 1;
+    a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz; a.nel = (int)c->nel;
+    a.remap = c->o.xcd_remap;
+    a.it = it;
+    a.nt = c->g.nt;
+    a.kahan_sigma = c->o.kahan_sigma && c->fp32;
+    a.vx = (const R *)c->v[0]; a.vy = (const R *)c->v[1]; a.vz = (const R *)c->v[2];
+    a.sig = (R *)c->sig;
+    a.sig_lo = (R *)c->sig_lo;
+    a.mem = (R *)c->mem;
+    a.lam = (const R *)c->lam; a.mu = (const R *)c->mu; a.tau = (const R *)c->tau;
+    a.ca = (const R *)c->ca; a.cb = (const R *)c->cb; a.cc = (const R *)c->cc;
+    a.force = (R *)c->force;
+    a.tth = (const ThTab<R> *)c->tth;
+    a.tr = (const RTab<R> *)c->tr;
+    a.px = c->dpx; a.py = c->dpy; a.pz = c->dpz;
+    a.alpha = c->g.taper_slope;
+    a.taper_on = c->g.taper_width > 0;
+    a.src_slot = c->d_slot;
+    a.src_chan = c->d_chan;
+    a.src_coef = c->d_coef;
+    a.stf = c->d_stf;
+    a.dt = (R)c->g.dt;
+    a.d1 = (R)((2.0 / c->dx) * c->jac);
+    a.d2 = (R)((2.0 / c->dy) * c->jac);
+    a.d3 = (R)(-((2.0 / c->dz) * c->jac));
+    for (int m = 0; m < NRDISS; ++m) {
+        a.mem_a[m] = (R)(1.0 - c->g.dt / TAU_P[m]);
+        a.mem_c[m] = (R)(c->g.dt / (NRDISS * TAU_P[m]));
+    }
+    for (int q = 0; q < P1; ++q) a.w[q] = (R)c->weights[q];
+    for (int i = 0; i < P1; ++i)
+        for (int q = 0; q < P1; ++q) {
+            // dl(q,i) = l_q'(xi_i) is stored at c->dl[q + P1*i]
+            a.Dx[i * P1 + q] = (R)(2.0 * c->dl[q + P1 * i] / c->dx);
+            a.Dy[i * P1 + q] = (R)(2.0 * c->dl[q + P1 * i] / c->dy);
+            a.Dz[i * P1 + q] = (R)(2.0 * c->dl[q + P1 * i] / c->dz);
+            a.dl[i * P1 + q] = (R)c->dl[i + P1 * q];
+        }
+    return a;
+}
+
+template <int P, typename R>
+NodeArgs<R> node_args(s3d_ctx *c)
+{
+    NodeArgs<R> a;
+    std::memset(&a, 0, sizeof(a));
+    a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz; a.nel = (int)c->nel;
+    a.nblk = c->nblk;
+    a.vx = (R *)c->v[0]; a.vy = (R *)c->v[1]; a.vz = (R *)c->v[2];
+    a.lx = (R *)c->lo[0]; a.ly = (R *)c->lo[1]; a.lz = (R *)c->lo[2];
+    a.force = (const R *)c->force;
+    a.inv_mm = (const R *)c->inv_mm;
+    a.px = c->dpx; a.py = c->dpy; a.pz = c->dpz;
+    a.alpha = c->g.taper_slope;
+    a.taper_on = c->g.taper_width > 0;
+    a.dt = (R)c->g.dt;
+    return a;
+}
+
+void ev_begin(s3d_ctx *c, int k)
+{
+    if (!c->timing) return;
+    hipEvent_t e;
+    if (hipEventCreate(&e) == hipSuccess) {
+        (void)hipEventRecord(e, c->stream);
+        c->ev[k].push_back(e);
+    }
+}
+void ev_end(s3d_ctx *c, int k) { ev_begin(c, k); }
+
+template <int P, typename R>
+int launch_step(s3d_ctx *c, int it)
+{
+    using S = Shape<P>;
+    const unsigned ge = nblocks(c->nel, S::EPB);
+    ElemArgs<R> ea = elem_args<P, R>(c, it);
+    ev_begin(c, 0);
+    if (c->visco)
+        hipLaunchKernelGGL((elem_kernel<P, R, true>), dim3(ge), dim3(S::NT), 0, c->stream, ea);
+    else
+        hipLaunchKernelGGL((elem_kernel<P, R, false>), dim3(ge), dim3(S::NT), 0, c->stream, ea);
+    ev_end(c, 0);
+    HIPCHK(hipGetLastError());
+    NodeArgs<R> na = node_args<P, R>(c);
+    const unsigned gn = nblocks(c->nuniq, 256);
+    ev_begin(c, 1);
+    if (c->lo[0])
+        hipLaunchKernelGGL((node_kernel<P, R, true>), dim3(gn), dim3(256), 0, c->stream, na);
+    else
+        hipLaunchKernelGGL((node_kernel<P, R, false>), dim3(gn), dim3(256), 0, c->stream, na);
+    ev_end(c, 1);
+    HIPCHK(hipGetLastError());
+    if (c->nrec_dev > 0) {
+        ev_begin(c, 2);
+        hipLaunchKernelGGL((record_kernel<P, R>), dim3(c->nrec_dev), dim3(64), 0, c->stream,
+                           c->g.nx, c->g.ny, (int)c->nel, (const R *)c->v[0], (const R *)c->v[1],
+                           (const R *)c->v[2], (const R *)c->lo[0], (const R *)c->lo[1],
+                           (const R *)c->lo[2], (const R *)c->sig, c->d_rec_e, c->d_rec_vm,
+                           c->d_rec_nch, c->d_rec_ch, c->d_traces, c->g.nt, it);
+        ev_end(c, 2);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+template <typename R>
+int dispatch_step(s3d_ctx *c, int it)
+{
+    switch (c->p) {
+    case 2: return launch_step<2, R>(c, it);
+    case 3: return launch_step<3, R>(c, it);
+    case 4: return launch_step<4, R>(c, it);
+    case 5: return launch_step<5, R>(c, it);
+    case 6: return launch_step<6, R>(c, it);
+    case 7: return launch_step<7, R>(c, it);
+    default: return fail("unsupported lpd");
+    }
+}
+
+// Generic P-dispatch for setup / IO kernels.
+template <template <int, typename> class F, typename R, typename... A>
+int with_p(int p, A &&...args)
+{
+    switch (p) {
+    case 2: return F<2, R>::run(args...);
+    case 3: return F<3, R>::run(args...);
+    case 4: return F<4, R>::run(args...);
+    case 5: return F<5, R>::run(args...);
+    case 6: return F<6, R>::run(args...);
+    case 7: return F<7, R>::run(args...);
+    default: return fail("unsupported lpd");
+    }
+}
+
+template <int P, typename R>
+struct MassRun {
+    static int run(s3d_ctx *c, const double *sin_t, const double *rad, const double *w)
+    {
+        hipLaunchKernelGGL((mass_kernel<P, R>), dim3(nblocks(c->nuniq, 256)), dim3(256), 0, c->stream,
+                           c->stage[S3D_RHOINV], sin_t, rad, w, c->jac, c->mm64, (R *)c->inv_mm,
+                           c->g.nx, c->g.ny, c->g.nz, c->nblk);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+};
+
+template <int P, typename R>
+struct CflRun {
+    static int run(s3d_ctx *c, const double *th, const double *ph, const double *rad, double *out)
+    {
+        unsigned nb = nblocks(c->nn, 256);
+        double *part = nullptr;
+        HIPCHK(hipMalloc(&part, sizeof(double) * 3 * nb));
+        hipLaunchKernelGGL((cfl_kernel<P>), dim3(nb), dim3(256), 0, c->stream, c->stage[S3D_RHOINV],
+                           c->stage[S3D_LAMBDA], c->stage[S3D_MU], th, ph, rad, c->g.nx, c->g.ny,
+                           c->g.nz, part);
+        std::vector<double> h(3 * (size_t)nb);
+        hipError_t e = hipMemcpyAsync(h.data(), part, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(part);
+        if (e != hipSuccess) return fail(std::string("cfl: ") + hipGetErrorString(e));
+        double m[3] = {0, 0, 0};
+        for (unsigned b = 0; b < nb; ++b)
+            for (int q = 0; q < 3; ++q) m[q] = std::max(m[q], h[3 * (size_t)b + q]);
+        double dt = c->g.dt;
+        *out = std::max(m[0] * dt, std::max(m[1] * dt, m[2] * dt));
+        return 0;
+    }
+};
+
+template <int P, typename R>
+struct ExpandRun {
+    static int run(s3d_ctx *c, const void *src, const void *lo, double *dst)
+    {
+        hipLaunchKernelGGL((expand_unique<P, R>), dim3(nblocks(c->nn, 256)), dim3(256), 0, c->stream,
+                           (const R *)src, (const R *)lo, dst, c->g.nx, c->g.ny, c->g.nz);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+};
+
+template <int P, typename R>
+struct GatherRun {
+    static int run(s3d_ctx *c, const double *src, void *dst, void *lo)
+    {
+        hipLaunchKernelGGL((gather_unique<P, R>), dim3(nblocks(c->nuniq, 256)), dim3(256), 0, c->stream,
+                           src, (R *)dst, (R *)lo, c->g.nx, c->g.ny, c->g.nz, c->nblk);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+};
+
+template <int P, typename R>
+struct VrangeRun {
+    static int run(s3d_ctx *c, double *out)
+    {
+        unsigned nb = nblocks(c->nuniq, 256);
+        double *part = nullptr;
+        HIPCHK(hipMalloc(&part, sizeof(double) * 2 * nb));
+        hipLaunchKernelGGL((vrange_kernel<P, R>), dim3(nb), dim3(256), 0, c->stream, (const R *)c->v[0],
+                           (const R *)c->v[1], (const R *)c->v[2], c->g.nx, c->g.ny, c->g.nz, c->nblk, part);
+        std::vector<double> h(2 * (size_t)nb);
+        hipError_t e = hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(part);
+        if (e != hipSuccess) return fail(std::string("vrange: ") + hipGetErrorString(e));
+        double mn = 1e308, mx = -1e308;
+        for (unsigned b = 0; b < nb; ++b) {
+            mn = std::min(mn, h[2 * (size_t)b]);
+            mx = std::max(mx, h[2 * (size_t)b + 1]);
+        }
+        out[0] = mn;
+        out[1] = mx;
+        return 0;
+    }
+};
+
+template <int P, typename R>
+struct KineticRun {
+    static int run(s3d_ctx *c, double *out)
+    {
+        unsigned nb = nblocks(c->nn, 256);
+        double *part = nullptr;
+        HIPCHK(hipMalloc(&part, sizeof(double) * nb));
+        hipLaunchKernelGGL((kinetic_kernel<P, R>), dim3(nb), dim3(256), 0, c->stream, (const R *)c->v[0],
+                           (const R *)c->v[1], (const R *)c->v[2], (const R *)c->rho,
+                           (const ThTab<R> *)c->tth, (const RTab<R> *)c->tr, (const R *)c->wdev,
+                           c->g.nx, c->g.ny, c->g.nz, part);
+        std::vector<double> h(nb);
+        hipError_t e = hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(part);
+        if (e != hipSuccess) return fail(std::string("kinetic: ") + hipGetErrorString(e));
+        double s = 0.0;
+        for (unsigned b = 0; b < nb; ++b) s += h[b];
+        *out = 0.5 * c->jac * s;
+        return 0;
+    }
+};
+
+int upload_sources(s3d_ctx *c)
+{
+    if (!c->src_dirty) return 0;
+    dfree(c->d_slot); dfree(c->d_chan); dfree(c->d_coef); dfree(c->d_stf);
+    c->d_slot = nullptr; c->d_chan = nullptr; c->d_coef = nullptr; c->d_stf = nullptr;
+    if (!c->slots.empty()) {
+        std::vector<int> chan(9 * c->slots.size());
+        std::vector<double> coef(9 * (size_t)c->NP * c->slots.size());
+        for (size_t s = 0; s < c->slots.size(); ++s) {
+            for (int q = 0; q < 9; ++q) chan[9 * s + q] = c->slots[s].chan[q];
+            std::memcpy(&coef[9 * (size_t)c->NP * s], c->slots[s].coef.data(), sizeof(double) * 9 * c->NP);
+        }
+        HIPCHK(hipMalloc(&c->d_slot, sizeof(int) * c->nel));
+        HIPCHK(hipMalloc(&c->d_chan, sizeof(int) * chan.size()));
+        HIPCHK(hipMalloc(&c->d_coef, sizeof(double) * coef.size()));
+        HIPCHK(hipMalloc(&c->d_stf, sizeof(double) * std::max<size_t>(1, c->stf.size())));
+        HIPCHK(hipMemcpy(c->d_slot, c->slot_of.data(), sizeof(int) * c->nel, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_chan, chan.data(), sizeof(int) * chan.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice));
+        if (!c->stf.empty())
+            HIPCHK(hipMemcpy(c->d_stf, c->stf.data(), sizeof(double) * c->stf.size(), hipMemcpyHostToDevice));
+    }
+    c->src_dirty = false;
+    return 0;
+}
+
+int upload_receivers(s3d_ctx *c)
+{
+    if (!c->rec_dirty) return 0;
+    const int n = (int)c->rec_e.size();
+    const size_t tr = (size_t)n * 8 * c->g.nt;
+    double *traces = nullptr;
+    HIPCHK(hipMalloc(&traces, sizeof(double) * std::max<size_t>(1, tr)));
+    HIPCHK(hipMemsetAsync(traces, 0, sizeof(double) * std::max<size_t>(1, tr), c->stream));
+    if (c->d_traces && c->nrec_dev > 0)
+        HIPCHK(hipMemcpyAsync(traces, c->d_traces, sizeof(double) * (size_t)c->nrec_dev * 8 * c->g.nt,
+                              hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    dfree(c->d_traces); dfree(c->d_rec_e); dfree(c->d_rec_nch); dfree(c->d_rec_ch); dfree(c->d_rec_vm);
+    c->d_traces = traces;
+    HIPCHK(hipMalloc(&c->d_rec_e, sizeof(int) * n));
+    HIPCHK(hipMalloc(&c->d_rec_nch, sizeof(int) * n));
+    HIPCHK(hipMalloc(&c->d_rec_ch, sizeof(int) * 8 * n));
+    HIPCHK(hipMalloc(&c->d_rec_vm, sizeof(double) * c->NP * (size_t)n));
+    HIPCHK(hipMemcpy(c->d_rec_e, c->rec_e.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rec_nch, c->rec_nch.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rec_ch, c->rec_ch.data(), sizeof(int) * 8 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rec_vm, c->rec_vm.data(), sizeof(double) * c->NP * (size_t)n, hipMemcpyHostToDevice));
+    c->nrec_dev = n;
+    c->rec_dirty = false;
+    return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *s3d_last_error(void) { return g_err.c_str(); }
+
+int s3d_device_count(int *count)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return 0;
+}
+
+int s3d_gll(int lpd, double *knots, double *weights, double *dl)
+{
+    if (gll_tables(lpd, knots, weights, dl)) return fail("lpd must be 1..7");
+    return 0;
+}
+
+int s3d_wavelet(int wavelet, double onset, double width, double dt, int nt, double *out)
+{
+    // src/misc/source_signature_mod.f90:50-147, t = (i-1) dt
+    for (int i = 0; i < nt; ++i) {
+        double t = (double)i * dt, s = 0.0;
+        switch (wavelet) {
+        case S3D_RICKER: {
+            double q = PI / width, a2 = -(q * q), u = t - onset;
+            s = 2.0 * (0.5 + a2 * (u * u)) * std::exp(a2 * (u * u));
+        } break;
+        case S3D_GAUSS: {
+            double u = (t - onset) / width;
+            s = std::exp(-(u * u));
+        } break;
+        case S3D_SIN3:
+            if (onset < t && t < onset + width) {
+                double sv = std::sin(PI * (t - onset) / width);
+                s = sv * sv * sv;
+            }
+            break;
+        case S3D_BRUNE:
+            if (t >= onset) s = (t - onset) / (width * width) * std::exp(-(t - onset) / width);
+            break;
+        case S3D_DELTA: s = (t == onset) ? 1.0 : 0.0; break;
+        default: return fail("unknown wavelet");
+        }
+        out[i] = s;
+    }
+    return 0;
+}
+
+int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out)
+{
+    if (!grid || !out) return fail("null argument");
+    if (grid->lpd < 2 || grid->lpd > 7) return fail("lpd must be 2..7 on the GPU path");
+    if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return fail("element counts must be >= 1");
+    if (grid->nt < 1) return fail("nt must be >= 1");
+    if (grid->taper_width > 0 && grid->taper_slope == 0.0) return fail("taper_slope must be nonzero");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
+    s3d_ctx *c = new s3d_ctx();
+    c->g = *grid;
+    s3d_options d{};
+    d.precision = S3D_FP64;
+    c->o = opt ? *opt : d;
+    c->fp32 = c->o.precision == S3D_FP32;
+    if (!c->fp32) { c->o.kahan_v = 0; c->o.kahan_sigma = 0; }
+    c->dev = c->o.device;
+    if (c->dev < 0 || c->dev >= ndev) { delete c; return fail("bad device ordinal"); }
+    hipError_t e = hipSetDevice(c->dev);
+    if (e == hipSuccess) e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) { delete c; return fail(std::string("stream: ") + hipGetErrorString(e)); }
+    c->p = grid->lpd;
+    c->P1 = c->p + 1;
+    c->NP = c->P1 * c->P1 * c->P1;
+    c->PB = c->p * c->p * c->p;
+    c->nel = (long)grid->nx * grid->ny * grid->nz;
+    c->nn = c->nel * c->NP;
+    c->nblk = (long)(grid->nx + 1) * (grid->ny + 1) * (grid->nz + 1);
+    c->nuniq = c->nblk * c->PB;
+    gll_tables(c->p, c->knots, c->weights, c->dl);
+    *out = c;
+    return 0;
+}
+
+int s3d_destroy(s3d_ctx *c)
+{
+    if (!c) return 0;
+    (void)hipSetDevice(c->dev);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
+    void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->inv_mm, c->sig,
+                    c->sig_lo, c->mem, c->lam, c->mu, c->tau, c->rho, c->ca, c->cb, c->cc,
+                    c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
+                    c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
+                    c->d_rec_ch, c->d_rec_vm, c->d_traces};
+    for (void *b : bufs) dfree(b);
+    for (int k = 0; k < 3; ++k)
+        for (hipEvent_t e : c->ev[k]) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+int s3d_set_model_slab(s3d_ctx *c, int param, int ez0, int nez, const double *data)
+{
+    if (!c || !data) return fail("null argument");
+    if (param < 0 || param > 6) return fail("unknown model parameter");
+    if (c->inited) return fail("model must be set before s3d_init");
+    if (ez0 < 0 || nez < 1 || ez0 + nez > c->g.nz) return fail("slab outside the mesh");
+    HIPCHK(hipSetDevice(c->dev));
+    if (!c->stage[param]) {
+        if (dalloc(c, (void **)&c->stage[param], sizeof(double) * c->nn, false)) return 1;
+        c->bytes -= sizeof(double) * (double)c->nn;   // staging is transient
+        if (param == S3D_Q) {
+            // default HUGE everywhere (src/model_paras.F90:76)
+            std::vector<double> h(c->nel, HUGE64);
+            for (int n = 0; n < c->NP; ++n)
+                HIPCHK(hipMemcpy(c->stage[param] + (size_t)n * c->nel, h.data(), sizeof(double) * c->nel,
+                                 hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(hipMemset(c->stage[param], 0, sizeof(double) * c->nn));
+        }
+    }
+    const size_t layer = (size_t)c->g.nx * c->g.ny;
+    const size_t width = layer * nez * sizeof(double);
+    HIPCHK(hipMemcpy2D(c->stage[param] + layer * ez0, sizeof(double) * c->nel, data, width, width,
+                       c->NP, hipMemcpyHostToDevice));
+    const size_t n = layer * nez * c->NP;
+    if (param == S3D_Q) {
+        for (size_t q = 0; q < n; ++q)
+            if (data[q] != HUGE64) { c->q_finite = true; break; }
+    } else if (param == S3D_A || param == S3D_B || param == S3D_C) {
+        for (size_t q = 0; q < n; ++q)
+            if (data[q] != 0.0) { c->abc_nonzero = true; break; }
+    }
+    return 0;
+}
+
+int s3d_set_model(s3d_ctx *c, int param, const double *data)
+{
+    if (!c) return fail("null argument");
+    return s3d_set_model_slab(c, param, 0, c->g.nz, data);
+}
+
+int s3d_init(s3d_ctx *c)
+{
+    if (!c) return fail("null argument");
+    if (c->inited) return fail("s3d_init called twice");
+    if (!c->stage[S3D_RHOINV] || !c->stage[S3D_LAMBDA] || !c->stage[S3D_MU])
+        return fail("rhoinv, lambda and mu must be set before s3d_init");
+    HIPCHK(hipSetDevice(c->dev));
+    const s3d_grid &g = c->g;
+    const int P1 = c->P1;
+    // src/config/model.f90:89-239 with one rank
+    c->x_glb_min = deg2rad(90.0 - g.lat_max);
+    c->x_glb_max = deg2rad(90.0 - g.lat_min);
+    c->y_glb_min = deg2rad(g.lon_min);
+    c->y_glb_max = deg2rad(g.lon_max);
+    c->z_glb_min = g.rad_min;
+    c->z_glb_max = g.rad_max;
+    double dxg = c->x_glb_max - c->x_glb_min, dyg = c->y_glb_max - c->y_glb_min,
+           dzg = c->z_glb_max - c->z_glb_min;
+    c->dx = dxg / g.nx;
+    c->dy = dyg / g.ny;
+    c->dz = dzg / g.nz;
+    c->x_loc_min = c->x_glb_min;
+    c->x_loc_max = c->x_glb_min + dxg;
+    c->y_loc_min = c->y_glb_min;
+    c->y_loc_max = c->y_glb_min + dyg;
+    c->z_loc_min = g.rad_min;
+    c->z_loc_max = g.rad_min + dzg;
+    c->jac = c->dx * c->dy * c->dz / 8.0;
+    c->x1.assign((size_t)g.nx * P1, 0.0);
+    c->y1.assign((size_t)g.ny * P1, 0.0);
+    c->z1.assign((size_t)g.nz * P1, 0.0);
+    coords_1d(c->knots, c->p, g.nx, c->x_loc_min, c->x_loc_max, c->x1.data());
+    coords_1d(c->knots, c->p, g.ny, c->y_loc_min, c->y_loc_max, c->y1.data());
+    coords_1d(c->knots, c->p, g.nz, c->z_loc_max, c->z_loc_min, c->z1.data());
+    // damping profiles (src/geometric_paras.f90:190-279)
+    const int pml = g.taper_width;
+    const double alpha = g.taper_slope;
+    auto prof = [&](const std::vector<double> &x1, int n, double d, double lo, double hi, bool do_lo,
+                    std::vector<double> &out) {
+        out.assign((size_t)n * P1, 0.0);
+        if (pml <= 0) return;
+        for (int e = 0; e < n; ++e)
+            for (int m = 0; m < P1; ++m) {
+                double t;
+                if (do_lo && e < pml) {
+                    t = ((lo + pml * d) - x1[e * P1 + m]) / (pml * d);
+                    out[e * P1 + m] = alpha * (t * t);
+                }
+                if (e >= n - pml) {
+                    t = ((hi - pml * d) - x1[e * P1 + m]) / (pml * d);
+                    out[e * P1 + m] = alpha * (t * t);
+                }
+            }
+    };
+    prof(c->x1, g.nx, c->dx, c->x_loc_min, c->x_loc_max, true, c->px1);
+    prof(c->y1, g.ny, c->dy, c->y_loc_min, c->y_loc_max, true, c->py1);
+    // r: only the bottom strip; the surface is free (src/geometric_paras.f90:256-279)
+    c->pz1.assign((size_t)g.nz * P1, 0.0);
+    if (pml > 0)
+        for (int e = std::max(0, g.nz - pml); e < g.nz; ++e)
+            for (int m = 0; m < P1; ++m) {
+                double t = ((c->z_loc_min + pml * c->dz) - c->z1[e * P1 + m]) / (pml * c->dz);
+                c->pz1[e * P1 + m] = alpha * (t * t);
+            }
+
+    // ---- 1-D tables ----
+    const size_t rs = rsize(c);
+    std::vector<double> sin1(g.nx * P1), th1 = c->x1, ph1 = c->y1, r1 = c->z1;
+    for (size_t q = 0; q < sin1.size(); ++q) sin1[q] = std::sin(c->x1[q]);
+    if (c->fp32) {
+        std::vector<ThTab<float>> tt(g.nx * P1);
+        std::vector<RTab<float>> tr(g.nz * P1);
+        for (size_t q = 0; q < tt.size(); ++q) {
+            double s = sin1[q];
+            tt[q] = {(float)s, (float)(1.0 / s), (float)(std::cos(c->x1[q]) / s), 0.f};
+        }
+        for (size_t q = 0; q < tr.size(); ++q) {
+            double r = c->z1[q];
+            tr[q] = {(float)r, (float)(1.0 / r), (float)(r * r), 0.f};
+        }
+        if (dalloc(c, &c->tth, sizeof(tt[0]) * tt.size())) return 1;
+        if (dalloc(c, &c->tr, sizeof(tr[0]) * tr.size())) return 1;
+        HIPCHK(hipMemcpy(c->tth, tt.data(), sizeof(tt[0]) * tt.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->tr, tr.data(), sizeof(tr[0]) * tr.size(), hipMemcpyHostToDevice));
+        std::vector<float> w(P1);
+        for (int q = 0; q < P1; ++q) w[q] = (float)c->weights[q];
+        if (dalloc(c, &c->wdev, sizeof(float) * P1)) return 1;
+        HIPCHK(hipMemcpy(c->wdev, w.data(), sizeof(float) * P1, hipMemcpyHostToDevice));
+    } else {
+        std::vector<ThTab<double>> tt(g.nx * P1);
+        std::vector<RTab<double>> tr(g.nz * P1);
+        for (size_t q = 0; q < tt.size(); ++q) {
+            double s = sin1[q];
+            tt[q] = {s, 1.0 / s, std::cos(c->x1[q]) / s, 0.0};
+        }
+        for (size_t q = 0; q < tr.size(); ++q) {
+            double r = c->z1[q];
+            tr[q] = {r, 1.0 / r, r * r, 0.0};
+        }
+        if (dalloc(c, &c->tth, sizeof(tt[0]) * tt.size())) return 1;
+        if (dalloc(c, &c->tr, sizeof(tr[0]) * tr.size())) return 1;
+        HIPCHK(hipMemcpy(c->tth, tt.data(), sizeof(tt[0]) * tt.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->tr, tr.data(), sizeof(tr[0]) * tr.size(), hipMemcpyHostToDevice));
+        if (dalloc(c, &c->wdev, sizeof(double) * P1)) return 1;
+        HIPCHK(hipMemcpy(c->wdev, c->weights, sizeof(double) * P1, hipMemcpyHostToDevice));
+    }
+    auto up = [&](double **dst, const std::vector<double> &h) -> int {
+        if (dalloc(c, (void **)dst, sizeof(double) * h.size(), false)) return 1;
+        HIPCHK(hipMemcpy(*dst, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        return 0;
+    };
+    if (up(&c->dpx, c->px1) || up(&c->dpy, c->py1) || up(&c->dpz, c->pz1)) return 1;
+    double *d_sin = nullptr, *d_r = nullptr, *d_w = nullptr, *d_th = nullptr, *d_ph = nullptr;
+    HIPCHK(hipMalloc(&d_sin, sizeof(double) * sin1.size()));
+    HIPCHK(hipMalloc(&d_r, sizeof(double) * r1.size()));
+    HIPCHK(hipMalloc(&d_w, sizeof(double) * P1));
+    HIPCHK(hipMalloc(&d_th, sizeof(double) * th1.size()));
+    HIPCHK(hipMalloc(&d_ph, sizeof(double) * ph1.size()));
+    HIPCHK(hipMemcpy(d_sin, sin1.data(), sizeof(double) * sin1.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_r, r1.data(), sizeof(double) * r1.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_w, c->weights, sizeof(double) * P1, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_th, th1.data(), sizeof(double) * th1.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d_ph, ph1.data(), sizeof(double) * ph1.size(), hipMemcpyHostToDevice));
+
+    // ---- state ----
+    c->visco = c->q_finite;
+    c->aniso = c->abc_nonzero;
+    for (int q = 0; q < 3; ++q) {
+        if (dalloc(c, &c->v[q], rs * c->nuniq)) return 1;
+        if (c->fp32 && c->o.kahan_v && dalloc(c, &c->lo[q], rs * c->nuniq)) return 1;
+    }
+    if (dalloc(c, &c->inv_mm, rs * c->nuniq)) return 1;
+    if (dalloc(c, (void **)&c->mm64, sizeof(double) * c->nuniq)) return 1;
+    if (dalloc(c, &c->sig, rs * 6 * c->nn)) return 1;
+    if (c->fp32 && c->o.kahan_sigma && dalloc(c, &c->sig_lo, rs * 6 * c->nn)) return 1;
+    if (c->visco && dalloc(c, &c->mem, rs * 18 * c->nn)) return 1;
+    if (dalloc(c, &c->force, rs * 3 * c->nn, false)) return 1;
+    if (dalloc(c, &c->lam, rs * c->nn, false)) return 1;
+    if (dalloc(c, &c->mu, rs * c->nn, false)) return 1;
+    if (c->visco && dalloc(c, &c->tau, rs * c->nn, false)) return 1;
+    if (dalloc(c, &c->rho, rs * c->nn, false)) return 1;
+    if (c->aniso) {
+        if (dalloc(c, &c->ca, rs * c->nn, false) || dalloc(c, &c->cb, rs * c->nn, false) ||
+            dalloc(c, &c->cc, rs * c->nn, false))
+            return 1;
+    }
+    const unsigned gm = nblocks(c->nn, 256);
+    if (c->fp32)
+        hipLaunchKernelGGL(material_kernel<float>, dim3(gm), dim3(256), 0, c->stream, c->stage[S3D_RHOINV],
+                           c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q], c->stage[S3D_A],
+                           c->stage[S3D_B], c->stage[S3D_C], (float *)c->lam, (float *)c->mu,
+                           (float *)c->tau, (float *)c->rho, (float *)c->ca, (float *)c->cb,
+                           (float *)c->cc, c->nel, c->NP);
+    else
+        hipLaunchKernelGGL(material_kernel<double>, dim3(gm), dim3(256), 0, c->stream, c->stage[S3D_RHOINV],
+                           c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q], c->stage[S3D_A],
+                           c->stage[S3D_B], c->stage[S3D_C], (double *)c->lam, (double *)c->mu,
+                           (double *)c->tau, (double *)c->rho, (double *)c->ca, (double *)c->cb,
+                           (double *)c->cc, c->nel, c->NP);
+    HIPCHK(hipGetLastError());
+    int rc = c->fp32 ? with_p<MassRun, float>(c->p, c, (const double *)d_sin, (const double *)d_r, (const double *)d_w)
+                     : with_p<MassRun, double>(c->p, c, (const double *)d_sin, (const double *)d_r, (const double *)d_w);
+    if (rc) return rc;
+    rc = with_p<CflRun, double>(c->p, c, (const double *)d_th, (const double *)d_ph, (const double *)d_r,
+                                &c->courant);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(d_sin); (void)hipFree(d_r); (void)hipFree(d_w); (void)hipFree(d_th); (void)hipFree(d_ph);
+    for (int q = 0; q < 7; ++q) { dfree(c->stage[q]); c->stage[q] = nullptr; }
+    c->slot_of.assign(c->nel, -1);
+    c->inited = true;
+    c->it = 0;
+    return 0;
+}
+
+int s3d_cfl(s3d_ctx *c, double *courant)
+{
+    if (!c || !courant) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    *courant = c->courant;
+    return 0;
+}
+
+int s3d_locate(s3d_ctx *c, double lat, double lon, double depth, int *exyz, double *vm)
+{
+    // construct_point_cls (src/config/point.f90:150-254)
+    if (!c || !exyz || !vm) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    double x = deg2rad(90.0 - lat), y = deg2rad(lon), z = EARTH_RADIUS - depth;
+    if (!(c->x_glb_min < x && x < c->x_glb_max && c->y_glb_min < y && y < c->y_glb_max &&
+          c->z_glb_min < z && z <= c->z_glb_max))
+        return 1;
+    int ex = (c->g.nx - 1) - (int)std::floor((c->x_loc_max - x) / c->dx);
+    int ey = (c->g.ny - 1) - (int)std::floor((c->y_loc_max - y) / c->dy);
+    int ez = (int)std::floor((c->z_loc_max - z) / c->dz);
+    if (ex < 0 || ex >= c->g.nx || ey < 0 || ey >= c->g.ny || ez < 0 || ez >= c->g.nz) return 1;
+    double sx = 2.0 * ((x - c->x_loc_min) / c->dx - ex) - 1.0;
+    double sy = 2.0 * ((y - c->y_loc_min) / c->dy - ey) - 1.0;
+    double sz = 2.0 * ((c->z_loc_max - z) / c->dz - ez) - 1.0;
+    double lx[MAXP1], ly[MAXP1], lz[MAXP1];
+    lagrange_1d(c->knots, c->P1, sx, lx);
+    lagrange_1d(c->knots, c->P1, sy, ly);
+    lagrange_1d(c->knots, c->P1, sz, lz);
+    const int P1 = c->P1;
+    for (int k = 0; k < P1; ++k)
+        for (int j = 0; j < P1; ++j)
+            for (int i = 0; i < P1; ++i) vm[i + P1 * (j + P1 * k)] = lx[i] * ly[j] * lz[k];
+    exyz[0] = ex;
+    exyz[1] = ey;
+    exyz[2] = ez;
+    return 0;
+}
+
+static int add_source(s3d_ctx *c, const int *exyz, const double *vm, const double *stf, int ncomp, int first)
+{
+    if (!c || !exyz || !vm || !stf) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    int ex = exyz[0], ey = exyz[1], ez = exyz[2];
+    if (ex < 0 || ex >= c->g.nx || ey < 0 || ey >= c->g.ny || ez < 0 || ez >= c->g.nz)
+        return fail("source element outside the mesh");
+    const long e = ex + (long)c->g.nx * (ey + (long)c->g.ny * ez);
+    int sl = c->slot_of[e];
+    if (sl < 0) {
+        Slot s;
+        s.e = (int)e;
+        for (int q = 0; q < 9; ++q) s.chan[q] = -1;
+        s.coef.assign(9 * (size_t)c->NP, 0.0);
+        c->slots.push_back(s);
+        sl = (int)c->slots.size() - 1;
+        c->slot_of[e] = sl;
+    }
+    Slot &S = c->slots[sl];
+    const int P1 = c->P1, nt = c->g.nt;
+    for (int q = 0; q < ncomp; ++q) {
+        const double *row = stf + (size_t)q * nt;
+        bool nz = false;
+        for (int i = 0; i < nt; ++i)
+            if (row[i] != 0.0) { nz = true; break; }
+        if (!nz) continue;   // dropped channel (src/config/source.f90:320-323)
+        const int comp = first + q;
+        c->stf.insert(c->stf.end(), row, row + nt);
+        S.chan[comp] = c->nchan++;
+        for (int k = 0; k < P1; ++k)
+            for (int j = 0; j < P1; ++j)
+                for (int i = 0; i < P1; ++i) {
+                    const int n = i + P1 * (j + P1 * k);
+                    double cf;
+                    if (comp < 6) {
+                        // vm / (jac W r^2 sin(theta)) (src/config/channel_mod.f90:866)
+                        double r = c->z1[ez * P1 + k], st = std::sin(c->x1[ex * P1 + i]);
+                        double rr = r * (r * st);
+                        double W = c->weights[i] * c->weights[j] * c->weights[k];
+                        cf = vm[n] / (c->jac * W * rr);
+                    } else {
+                        cf = vm[n];   // src/config/channel_mod.f90:1028
+                    }
+                    S.coef[(size_t)comp * c->NP + n] = cf;
+                }
+    }
+    c->src_dirty = true;
+    return 0;
+}
+
+int s3d_add_source_mt(s3d_ctx *c, const int *exyz, const double *vm, const double *stf6)
+{
+    return add_source(c, exyz, vm, stf6, 6, 0);
+}
+
+int s3d_add_source_sf(s3d_ctx *c, const int *exyz, const double *vm, const double *stf3)
+{
+    return add_source(c, exyz, vm, stf3, 3, 6);
+}
+
+int s3d_add_receiver(s3d_ctx *c, const int *exyz, const double *vm, int nch, const int *channels, int *id)
+{
+    if (!c || !exyz || !vm || !channels) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    if (nch < 1 || nch > 8) return fail("a receiver records 1..8 channels");
+    int ex = exyz[0], ey = exyz[1], ez = exyz[2];
+    if (ex < 0 || ex >= c->g.nx || ey < 0 || ey >= c->g.ny || ez < 0 || ez >= c->g.nz)
+        return fail("receiver element outside the mesh");
+    for (int q = 0; q < nch; ++q)
+        if (channels[q] < S3D_CH_VX || channels[q] > S3D_CH_PRESSURE)
+            return fail("unsupported receiver channel " + std::to_string(channels[q]));
+    c->rec_e.push_back((int)(ex + (long)c->g.nx * (ey + (long)c->g.ny * ez)));
+    c->rec_nch.push_back(nch);
+    for (int q = 0; q < 8; ++q) c->rec_ch.push_back(q < nch ? channels[q] : -1);
+    c->rec_vm.insert(c->rec_vm.end(), vm, vm + c->NP);
+    c->rec_dirty = true;
+    if (id) *id = (int)c->rec_e.size() - 1;
+    return 0;
+}
+
+int s3d_step(s3d_ctx *c, int nsteps)
+{
+    if (!c) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    if (nsteps < 0) return fail("negative step count");
+    if (c->it + nsteps > c->g.nt) return fail("stepping past nt");
+    HIPCHK(hipSetDevice(c->dev));
+    if (upload_sources(c) || upload_receivers(c)) return 1;
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = c->fp32 ? dispatch_step<float>(c, c->it + 1) : dispatch_step<double>(c, c->it + 1);
+        if (rc) return rc;
+        c->it += 1;
+    }
+    return 0;
+}
+
+int s3d_current_step(s3d_ctx *c, int *it)
+{
+    if (!c || !it) return fail("null argument");
+    *it = c->it;
+    return 0;
+}
+
+int s3d_get_trace(s3d_ctx *c, int rec, int ch, double *out)
+{
+    if (!c || !out) return fail("null argument");
+    if (rec < 0 || rec >= (int)c->rec_e.size()) return fail("no such receiver");
+    if (ch < 0 || ch >= c->rec_nch[rec]) return fail("no such channel");
+    HIPCHK(hipSetDevice(c->dev));
+    if (upload_receivers(c)) return 1;
+    HIPCHK(hipMemcpyAsync(out, c->d_traces + ((size_t)rec * 8 + ch) * c->g.nt, sizeof(double) * c->g.nt,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int s3d_get_field(s3d_ctx *c, int field, double *out)
+{
+    if (!c || !out) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    HIPCHK(hipSetDevice(c->dev));
+    const long nn = c->nn;
+    if (field == S3D_TAPER) {
+        const int P1 = c->P1;
+        for (int k = 0; k < P1; ++k)
+            for (int j = 0; j < P1; ++j)
+                for (int i = 0; i < P1; ++i)
+                    for (int ez = 0; ez < c->g.nz; ++ez)
+                        for (int ey = 0; ey < c->g.ny; ++ey)
+                            for (int ex = 0; ex < c->g.nx; ++ex) {
+                                long n = ex + (long)c->g.nx * (ey + (long)c->g.ny * (ez + (long)c->g.nz *
+                                                                                              (i + (long)P1 * (j + (long)P1 * k))));
+                                out[n] = c->g.taper_width > 0
+                                             ? taper_value(c->px1[ex * P1 + i], c->py1[ey * P1 + j],
+                                                           c->pz1[ez * P1 + k], c->g.taper_slope)
+                                             : 1.0;
+                            }
+        return 0;
+    }
+    double *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, sizeof(double) * nn));
+    int rc = 0;
+    if (field >= S3D_VX && field <= S3D_VZ) {
+        rc = c->fp32 ? with_p<ExpandRun, float>(c->p, c, (const void *)c->v[field], (const void *)c->lo[field], buf)
+                     : with_p<ExpandRun, double>(c->p, c, (const void *)c->v[field], (const void *)nullptr, buf);
+    } else if (field == S3D_MM) {
+        rc = with_p<ExpandRun, double>(c->p, c, (const void *)c->mm64, (const void *)nullptr, buf);
+    } else if ((field >= S3D_SXX && field <= S3D_SZZ) || (field >= S3D_MXX && field < S3D_MXX + 18)) {
+        const void *src = nullptr;
+        const void *lo = nullptr;
+        if (field <= S3D_SZZ) {
+            src = (const char *)c->sig + rsize(c) * nn * (field - S3D_SXX);
+            if (c->sig_lo) lo = (const char *)c->sig_lo + rsize(c) * nn * (field - S3D_SXX);
+        } else if (c->mem) {
+            src = (const char *)c->mem + rsize(c) * nn * (field - S3D_MXX);
+        }
+        if (!src) {
+            HIPCHK(hipMemsetAsync(buf, 0, sizeof(double) * nn, c->stream));
+        } else if (c->fp32) {
+            hipLaunchKernelGGL(from_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                               (const float *)src, buf, c->nel, c->NP);
+            if (lo) {
+                // value = hi - lo
+                double *tmp = nullptr;
+                HIPCHK(hipMalloc(&tmp, sizeof(double) * nn));
+                hipLaunchKernelGGL(from_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                                   (const float *)lo, tmp, c->nel, c->NP);
+                std::vector<double> a(nn), b(nn);
+                HIPCHK(hipMemcpyAsync(a.data(), buf, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipMemcpyAsync(b.data(), tmp, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                (void)hipFree(tmp);
+                for (long q = 0; q < nn; ++q) out[q] = a[q] - b[q];
+                (void)hipFree(buf);
+                return 0;
+            }
+        } else {
+            hipLaunchKernelGGL(from_device_layout<double>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                               (const double *)src, buf, c->nel, c->NP);
+        }
+    } else {
+        (void)hipFree(buf);
+        return fail("unknown field id " + std::to_string(field));
+    }
+    if (rc) { (void)hipFree(buf); return rc; }
+    HIPCHK(hipMemcpyAsync(out, buf, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(buf);
+    return 0;
+}
+
+int s3d_set_field(s3d_ctx *c, int field, const double *in)
+{
+    if (!c || !in) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    HIPCHK(hipSetDevice(c->dev));
+    const long nn = c->nn;
+    double *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, sizeof(double) * nn));
+    HIPCHK(hipMemcpyAsync(buf, in, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    int rc = 0;
+    if (field >= S3D_VX && field <= S3D_VZ) {
+        rc = c->fp32 ? with_p<GatherRun, float>(c->p, c, (const double *)buf, c->v[field], c->lo[field])
+                     : with_p<GatherRun, double>(c->p, c, (const double *)buf, c->v[field], (void *)nullptr);
+    } else if ((field >= S3D_SXX && field <= S3D_SZZ) || (field >= S3D_MXX && field < S3D_MXX + 18)) {
+        void *dst = nullptr;
+        if (field <= S3D_SZZ) {
+            dst = (char *)c->sig + rsize(c) * nn * (field - S3D_SXX);
+            if (c->sig_lo)
+                HIPCHK(hipMemsetAsync((char *)c->sig_lo + rsize(c) * nn * (field - S3D_SXX), 0, rsize(c) * nn,
+                                      c->stream));
+        } else if (c->mem) {
+            dst = (char *)c->mem + rsize(c) * nn * (field - S3D_MXX);
+        } else {
+            rc = fail("memory variables exist only for a visco-elastic model");
+        }
+        if (dst) {
+            if (c->fp32)
+                hipLaunchKernelGGL(to_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                                   (const double *)buf, (float *)dst, c->nel, c->NP);
+            else
+                hipLaunchKernelGGL(to_device_layout<double>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                                   (const double *)buf, (double *)dst, c->nel, c->NP);
+        }
+    } else {
+        rc = fail("field " + std::to_string(field) + " cannot be set");
+    }
+    hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(buf);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(std::string("set_field: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int s3d_vrange(s3d_ctx *c, double *out2)
+{
+    if (!c || !out2) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    HIPCHK(hipSetDevice(c->dev));
+    return c->fp32 ? with_p<VrangeRun, float>(c->p, c, out2) : with_p<VrangeRun, double>(c->p, c, out2);
+}
+
+int s3d_kinetic_energy(s3d_ctx *c, double *out)
+{
+    if (!c || !out) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    HIPCHK(hipSetDevice(c->dev));
+    return c->fp32 ? with_p<KineticRun, float>(c->p, c, out) : with_p<KineticRun, double>(c->p, c, out);
+}
+
+int s3d_synchronize(s3d_ctx *c)
+{
+    if (!c) return fail("null argument");
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int s3d_timing_enable(s3d_ctx *c, int enable)
+{
+    if (!c) return fail("null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) {
+        for (hipEvent_t e : c->ev[k]) (void)hipEventDestroy(e);
+        c->ev[k].clear();
+        c->ms[k] = 0;
+        c->cnt[k] = 0;
+    }
+    c->timing = enable != 0;
+    return 0;
+}
+
+int s3d_kernel_times(s3d_ctx *c, double *ms3, int *count3)
+{
+    if (!c || !ms3 || !count3) return fail("null argument");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; ++k) {
+        auto &v = c->ev[k];
+        for (size_t q = 0; q + 1 < v.size(); q += 2) {
+            float m = 0.f;
+            HIPCHK(hipEventElapsedTime(&m, v[q], v[q + 1]));
+            c->ms[k] += m;
+            c->cnt[k] += 1;
+        }
+        for (hipEvent_t e : v) (void)hipEventDestroy(e);
+        v.clear();
+        ms3[k] = c->ms[k];
+        count3[k] = c->cnt[k];
+    }
+    return 0;
+}
+
+int s3d_device_bytes(s3d_ctx *c, double *bytes)
+{
+    if (!c || !bytes) return fail("null argument");
+    *bytes = c->bytes;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,101 @@
+// s3d_internal.h -- device data layout and kernel argument blocks.
+//
+// Layout in HBM (one simulation):
+//   element-local fields (stress, memory variables, material, element force):
+//       f[c][e][n], e = ex + nx*(ey + ny*ez), n = i + P1*(j + P1*k)
+//     so the (p+1)^3 nodes of one element are contiguous and a workgroup's
+//     loads/stores of an element are fully coalesced.
+//   unique-node fields (velocity, its compensation, 1/M):
+//       g[b][o], b = bx + (nx+1)*(by + (ny+1)*bz), o = li + p*(lj + p*lk)
+//     element block b "owns" the p^3 nodes with local indices 0..p-1; the
+//     padding blocks bx = nx (by = ny, bz = nz) hold the last node plane.
+//     Global node (gx, gy, gz) = (bx*p + li, by*p + lj, bz*p + lk).
+//   1-D tables: per (element, GLL index) along each axis, FP64 computed,
+//     rounded once to the step precision.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace s3d {
+
+constexpr int MAXP1 = 8;       // p <= 7
+constexpr int NRDISS = 3;      // relaxation mechanisms
+
+// strain-rate / stress component order used everywhere: xx xy xz yy yz zz
+enum { XX = 0, XY, XZ, YY, YZ, ZZ };
+
+template <typename R>
+struct ThTab {                 // per (ex, i): colatitude dependent factors
+    R sin_t, inv_sin, cot, pad;
+};
+template <typename R>
+struct RTab {                  // per (ez, k): radius dependent factors
+    R r, inv_r, r2, pad;
+};
+
+template <typename R>
+struct ElemArgs {
+    int nx, ny, nz, nel;
+    int remap;                 // XCD-aware block order
+    int it;                    // 1-based iteration being computed
+    int nt;
+    int kahan_sigma;
+    const R *vx, *vy, *vz;     // unique-node velocity
+    R *sig;                    // [6][nel][NP]
+    R *sig_lo;                 // [6][nel][NP] compensation or null
+    R *mem;                    // [18][nel][NP] (visco) or null
+    const R *lam, *mu, *tau;   // [nel][NP]
+    const R *ca, *cb, *cc;     // anisotropy A, B, C or null
+    R *force;                  // [3][nel][NP]
+    const ThTab<R> *tth;       // [nx*P1]
+    const RTab<R> *tr;         // [nz*P1]
+    const double *px, *py, *pz;  // 1-D taper profiles [n*P1]
+    double alpha;              // taper slope
+    int taper_on;
+    const int *src_slot;       // [nel], -1 = none, or null
+    const int *src_chan;       // [nslot][9] channel row in stf, -1 = none
+    const double *src_coef;    // [nslot][9][NP]
+    const double *stf;         // [nchan][nt]
+    R dt;
+    R d1, d2, d3;              // (2/dx)J, (2/dy)J, -(2/dz)J
+    R mem_a[NRDISS];           // 1 - dt/tau_p
+    R mem_c[NRDISS];           // dt/(3 tau_p)
+    R w[MAXP1];                // GLL weights
+    R Dx[MAXP1 * MAXP1];       // Dx[i*P1+q] = 2 l_q'(xi_i) / dx
+    R Dy[MAXP1 * MAXP1];
+    R Dz[MAXP1 * MAXP1];
+    R dl[MAXP1 * MAXP1];       // dl[i*P1+q] = l_i'(xi_q)
+};
+
+template <typename R>
+struct NodeArgs {
+    int nx, ny, nz, nel;
+    long nblk;                 // (nx+1)(ny+1)(nz+1)
+    R *vx, *vy, *vz;
+    R *lx, *ly, *lz;           // compensation terms or null
+    const R *force;            // [3][nel][NP]
+    const R *inv_mm;           // unique
+    const double *px, *py, *pz;
+    double alpha;
+    int taper_on;
+    R dt;
+};
+
+// taper = exp(-0.1 * prof^2) after corner normalisation
+// (reference src/geometric_paras.f90:303-316). 0.1 is the reference's
+// single-precision literal promoted to double.
+__host__ __device__ inline double taper_value(double px, double py, double pz,
+                                              double alpha)
+{
+    double prof = px + py + pz;
+    if (prof == 0.0) return 1.0;
+    double den = 1.0 + prof / alpha;
+    double pz2 = 2.0 * pz / den;
+    double py2 = 2.0 * py / den;
+    double px2 = 2.0 * px / den;
+    double p2 = px2 + py2 + pz2;
+    return exp(-((double)0.1f * (p2 * p2)));
+}
+
+}  // namespace s3d

@@ -1,0 +1,695 @@
+// s3d_kernels.hip -- gfx950 kernels of the forward time step.
+//
+// One time step (reference src/ses3d-nt.F90:240-270 around
+// src/evolution.f90:87-444) is two kernels plus a tiny receiver kernel:
+//
+//   elem_kernel  one workgroup per EPB elements, one lane per element-local
+//                node. Gathers the element's velocity into LDS, forms the
+//                strain rate (K1-K4), the stress rate and the stress / memory
+//                variable update (K5-K7, SLS), adds the moment-tensor glut,
+//                stages the effective stress in LDS and contracts the weak
+//                divergence (K8). Writes the element force, the tapered
+//                stress and the memory variables; all HBM traffic coalesced.
+//   node_kernel  one lane per unique node. Sums the (up to 8) element copies
+//                of the force in the reference's theta -> phi -> r order
+//                (K10, src/communicate_fields.F90:48-79; deterministic, no
+//                atomics), multiplies by 1/M (K9), integrates the velocity
+//                (K12, optional compensated summation) and tapers it (K13).
+//   record_kernel one wave per receiver: sum vm * field over the element.
+//
+// HBM bytes per element-local node and step (FP32, Kahan on v, visco):
+//   stress 24 r + 24 w, memory variables 72 r + 72 w, lambda/mu/tau 12 r,
+//   force 12 w + 12 r, velocity ~(3 r gather) + unique-node v, lo, 1/M
+//   (~24*u r + 24*u w with u = unique/element-local ~ 0.52).
+#include "s3d_internal.h"
+
+namespace s3d {
+
+template <int P>
+struct Shape {
+    static constexpr int P1 = P + 1;
+    static constexpr int NP = P1 * P1 * P1;
+    static constexpr int PB = P * P * P;
+    static constexpr int EPB = (256 / NP) > 0 ? (256 / NP) : 1;
+    static constexpr int NT = ((EPB * NP + 63) / 64) * 64;
+};
+
+__device__ inline int remap_block(int b, int G, int remap)
+{
+    // Dispatch sends consecutive workgroups to consecutive XCDs (8 on MI355X).
+    // Remapping gives each XCD a contiguous range of elements, so element
+    // neighbours (shared velocity / force nodes) meet in the same L2.
+    if (!remap || (G & 7)) return b;
+    return (b & 7) * (G >> 3) + (b >> 3);
+}
+
+template <int P>
+__device__ inline long unique_index(int ex, int ey, int ez, int i, int j, int k,
+                                    int nx, int ny)
+{
+    int bx = ex + (i == P), by = ey + (j == P), bz = ez + (k == P);
+    int li = (i == P) ? 0 : i, lj = (j == P) ? 0 : j, lk = (k == P) ? 0 : k;
+    long b = (long)bx + (long)(nx + 1) * ((long)by + (long)(ny + 1) * bz);
+    return b * (P * P * P) + li + P * (lj + P * lk);
+}
+
+template <typename R>
+__device__ inline void kahan_add(R &s, R &c, R inc)
+{
+    R y = inc - c;
+    R t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// ---------------------------------------------------------------------------
+// element kernel
+// ---------------------------------------------------------------------------
+template <int P, typename R, bool VISCO>
+__global__ void __launch_bounds__(Shape<P>::NT)
+elem_kernel(const ElemArgs<R> a)
+{
+    using S = Shape<P>;
+    constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
+    __shared__ R lds[6][EPB * NP];
+
+    const int t = threadIdx.x;
+    const int slot = t / NP;
+    const int node = t - slot * NP;
+    const int blk = remap_block(blockIdx.x, gridDim.x, a.remap);
+    const int e = blk * EPB + slot;
+    const bool inside = slot < EPB;
+    const bool valid = inside && e < a.nel;
+    const int i = node % P1, j = (node / P1) % P1, k = node / (P1 * P1);
+    const int ee = valid ? e : 0;
+    const int ex = ee % a.nx, ey = (ee / a.nx) % a.ny, ez = ee / (a.nx * a.ny);
+    const int L = (inside ? slot : 0) * NP;
+    const long fs = (long)a.nel * NP;      // field stride
+    const long eb = (long)ee * NP + node;  // element-local address
+
+    // ---- velocity gather (unique nodes -> LDS) ----
+    R vx = 0, vy = 0, vz = 0;
+    if (valid) {
+        long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
+        vx = a.vx[u];
+        vy = a.vy[u];
+        vz = a.vz[u];
+    }
+    if (inside) {
+        lds[0][L + node] = vx;
+        lds[1][L + node] = vy;
+        lds[2][L + node] = vz;
+    }
+    // ---- state loads (issued before the barrier to hide latency) ----
+    R sg[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sg[c] = valid ? a.sig[c * fs + eb] : R(0);
+    R lam = valid ? a.lam[eb] : R(0);
+    R mu = valid ? a.mu[eb] : R(0);
+    R tau = R(0);
+    R M[VISCO ? 18 : 1];
+    if constexpr (VISCO) {
+        tau = valid ? a.tau[eb] : R(0);
+#pragma unroll
+        for (int c = 0; c < 18; ++c) M[c] = valid ? a.mem[c * fs + eb] : R(0);
+    }
+    __syncthreads();
+
+    // ---- K1: tensor-product derivatives along theta, phi, r ----
+    R dxv[3] = {0, 0, 0}, dyv[3] = {0, 0, 0}, dzv[3] = {0, 0, 0};
+    {
+        const R *Dxi = a.Dx + i * P1;
+        const R *Dyj = a.Dy + j * P1;
+        const R *Dzk = a.Dz + k * P1;
+#pragma unroll
+        for (int q = 0; q < P1; ++q) {
+            const int nxq = L + q + P1 * (j + P1 * k);
+            const int nyq = L + i + P1 * (q + P1 * k);
+            const int nzq = L + i + P1 * (j + P1 * q);
+            const R cx = Dxi[q], cy = Dyj[q], cz = Dzk[q];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                dxv[c] += lds[c][nxq] * cx;
+                dyv[c] += lds[c][nyq] * cy;
+                dzv[c] += lds[c][nzq] * cz;
+            }
+        }
+    }
+
+    // ---- K2: spherical completion, K4: symmetrisation ----
+    const ThTab<R> th = a.tth[ex * P1 + i];
+    const RTab<R> rr = a.tr[ez * P1 + k];
+    const R inv_r = rr.inv_r;
+    const R inv_rs = inv_r * th.inv_sin;
+    const R exx = (vz + dxv[0]) * inv_r;
+    const R eyy = dyv[1] * inv_rs + (vz + vx * th.cot) * inv_r;
+    const R ezz = -dzv[2];
+    const R ezx = -dzv[0];
+    const R exz_c = (dxv[2] - vx) * inv_r;
+    const R ezy = -dzv[1];
+    const R eyz_c = -vy * inv_r + dyv[2] * inv_rs;
+    const R exy_c = dxv[1] * inv_r;
+    const R eyx = -vy * th.cot * inv_r + dyv[0] * inv_rs;
+    const R exy = (exy_c + eyx) * R(0.5);
+    const R exz = (exz_c + ezx) * R(0.5);
+    const R eyz = (eyz_c + ezy) * R(0.5);
+
+    // ---- K5: stress rate ----
+    R rt[6];
+    const R tmu = R(2) * mu;
+    if constexpr (!VISCO) {
+        const R Lr = lam * (exx + eyy + ezz);
+        rt[XX] = Lr + tmu * exx;
+        rt[YY] = Lr + tmu * eyy;
+        rt[ZZ] = Lr + tmu * ezz;
+        rt[XY] = tmu * exy;
+        rt[XZ] = tmu * exz;
+        rt[YZ] = tmu * eyz;
+    } else {
+        // build-defined SLS (loop form of src/evolution.f90:216-284)
+        R Md[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) Md[c] = M[c] + M[6 + c] + M[12 + c];
+        const R Msum = Md[XX] + Md[YY] + Md[ZZ];
+        const R tmut = tmu * (R(1) + tau);                     // 2 mu_tau
+        const R lk = lam - R(2.0 / 3.0) * tau * mu;            // kappa - 2 mu_tau / 3
+        const R Lr = lk * (exx + eyy + ezz) - tmu * Msum * R(1.0 / 3.0);
+        rt[XX] = Lr + tmut * exx + tmu * Md[XX];
+        rt[YY] = Lr + tmut * eyy + tmu * Md[YY];
+        rt[ZZ] = Lr + tmut * ezz + tmu * Md[ZZ];
+        rt[XY] = tmut * exy + tmu * Md[XY];
+        rt[XZ] = tmut * exz + tmu * Md[XZ];
+        rt[YZ] = tmut * eyz + tmu * Md[YZ];
+    }
+    if (a.ca) {   // anisotropy (src/evolution.f90:290-296)
+        const R A = valid ? a.ca[eb] : R(0);
+        const R B = valid ? a.cb[eb] : R(0);
+        const R C = valid ? a.cc[eb] : R(0);
+        rt[XX] += C * ezz + A * (eyy + exx);
+        rt[YY] += C * ezz + A * (eyy + exx);
+        rt[ZZ] += C * (eyy + exx);
+        rt[XZ] += R(2) * B * exz;
+        rt[YZ] += R(2) * B * eyz;
+    }
+
+    // ---- K6: integrate the stress ----
+    R lo[6];
+    if (a.kahan_sigma) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            lo[c] = valid ? a.sig_lo[c * fs + eb] : R(0);
+            kahan_add(sg[c], lo[c], a.dt * rt[c]);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sg[c] = sg[c] + a.dt * rt[c];
+    }
+
+    // ---- K7: moment-tensor glut and single force of the previous add() ----
+    R src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (a.src_slot && valid && a.it > 1) {
+        const int sl = a.src_slot[ee];
+        if (sl >= 0) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                const int ch = a.src_chan[sl * 9 + c];
+                if (ch >= 0) {
+                    const double d = a.stf[(long)ch * a.nt + (a.it - 2)];
+                    src[c] = (R)(-(d * a.src_coef[((long)sl * 9 + c) * NP + node]));
+                }
+            }
+        }
+    }
+
+    __syncthreads();   // LDS velocity no longer needed
+    if (inside) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) lds[c][L + node] = sg[c] + src[c];
+    }
+
+    // ---- memory variables (loop form of src/evolution.f90:415-442) ----
+    if constexpr (VISCO) {
+        const R ev[6] = {exx, exy, exz, eyy, eyz, ezz};
+#pragma unroll
+        for (int m = 0; m < NRDISS; ++m) {
+            const R cm = a.mem_c[m] * tau;
+#pragma unroll
+            for (int c = 0; c < 6; ++c) M[6 * m + c] = M[6 * m + c] * a.mem_a[m] - cm * ev[c];
+        }
+    }
+
+    // ---- taper factor of this node (K13) ----
+    R tp = R(1);
+    if (a.taper_on && valid) {
+        tp = (R)taper_value(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
+    }
+
+    __syncthreads();
+
+    // ---- K8: weak divergence ----
+    R ax[P1], ay[P1], az[P1];
+    {
+        const R *dli = a.dl + i * P1;
+        const R *dlj = a.dl + j * P1;
+        const R *dlk = a.dl + k * P1;
+#pragma unroll
+        for (int q = 0; q < P1; ++q) {
+            ax[q] = dli[q] * a.w[q] * a.tth[ex * P1 + q].sin_t;
+            ay[q] = dlj[q] * a.w[q];
+            az[q] = dlk[q] * a.w[q] * a.tr[ez * P1 + q].r2;
+        }
+    }
+    R Sx[3] = {0, 0, 0}, Sy[3] = {0, 0, 0}, Sz[3] = {0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < P1; ++q) {
+        const int nxq = L + q + P1 * (j + P1 * k);
+        const int nyq = L + i + P1 * (q + P1 * k);
+        const int nzq = L + i + P1 * (j + P1 * q);
+        // x contraction: sxx, sxy, sxz feed sx, sy, sz
+        Sx[0] += ax[q] * lds[XX][nxq];
+        Sx[1] += ax[q] * lds[XY][nxq];
+        Sx[2] += ax[q] * lds[XZ][nxq];
+        // y contraction: sxy, syy, syz
+        Sy[0] += ay[q] * lds[XY][nyq];
+        Sy[1] += ay[q] * lds[YY][nyq];
+        Sy[2] += ay[q] * lds[YZ][nyq];
+        // z contraction: sxz, syz, szz
+        Sz[0] += az[q] * lds[XZ][nzq];
+        Sz[1] += az[q] * lds[YZ][nzq];
+        Sz[2] += az[q] * lds[ZZ][nzq];
+    }
+    const R Cx = a.w[j] * a.w[k] * rr.r * a.d1;
+    const R Cy = a.w[i] * a.w[k] * rr.r * a.d2;
+    const R Cz = a.w[i] * a.w[j] * th.sin_t * a.d3;
+
+    if (!valid) return;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        a.force[c * fs + eb] = src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c];
+
+    // ---- store tapered stress (K13) and memory variables ----
+    if (a.kahan_sigma) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            a.sig[c * fs + eb] = sg[c] * tp;
+            a.sig_lo[c * fs + eb] = lo[c] * tp;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] * tp;
+    }
+    if constexpr (VISCO) {
+#pragma unroll
+        for (int c = 0; c < 18; ++c) a.mem[c * fs + eb] = M[c];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// node kernel: assembly, 1/M, velocity integration, taper
+// ---------------------------------------------------------------------------
+template <int P, typename R, bool KV>
+__global__ void __launch_bounds__(256)
+node_kernel(const NodeArgs<R> a)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1, PB = P * P * P;
+    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long b = tid / PB;
+    const int o = (int)(tid - b * PB);
+    if (b >= a.nblk) return;
+    const int li = o % P, lj = (o / P) % P, lk = o / (P * P);
+    const int nbx = a.nx + 1, nby = a.ny + 1;
+    const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
+    if ((bx == a.nx && li) || (by == a.ny && lj) || (bz == a.nz && lk)) return;
+
+    const long fs = (long)a.nel * NP;
+    // copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P, lk+dz*P)
+    bool px[2] = {bx < a.nx, li == 0 && bx > 0};
+    bool py[2] = {by < a.ny, lj == 0 && by > 0};
+    bool pz[2] = {bz < a.nz, lk == 0 && bz > 0};
+    R acc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const R *f = a.force + c * fs;
+        R zsum = 0;
+        bool zhas = false;
+#pragma unroll
+        for (int dz = 0; dz < 2; ++dz) {
+            if (!pz[dz]) continue;
+            R ysum = 0;
+            bool yhas = false;
+#pragma unroll
+            for (int dy = 0; dy < 2; ++dy) {
+                if (!py[dy]) continue;
+                R xsum = 0;
+                bool xhas = false;
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx) {
+                    if (!px[dx]) continue;
+                    const long e = (long)(bx - dx) + (long)a.nx * ((long)(by - dy) + (long)a.ny * (bz - dz));
+                    const int n = (li + dx * P) + P1 * ((lj + dy * P) + P1 * (lk + dz * P));
+                    const R val = f[e * NP + n];
+                    xsum = xhas ? xsum + val : val;
+                    xhas = true;
+                }
+                ysum = yhas ? ysum + xsum : xsum;
+                yhas = true;
+            }
+            zsum = zhas ? zsum + ysum : ysum;
+            zhas = true;
+        }
+        acc[c] = zsum;
+    }
+    const long u = tid;
+    const R im = a.inv_mm[u];
+    R tp = R(1);
+    if (a.taper_on) {
+        const int ex = bx < a.nx ? bx : a.nx - 1, i = bx < a.nx ? li : P;
+        const int ey = by < a.ny ? by : a.ny - 1, j = by < a.ny ? lj : P;
+        const int ez = bz < a.nz ? bz : a.nz - 1, k = bz < a.nz ? lk : P;
+        tp = (R)taper_value(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
+    }
+    R *v[3] = {a.vx, a.vy, a.vz};
+    if constexpr (KV) {
+        R *l[3] = {a.lx, a.ly, a.lz};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R s = v[c][u], cc = l[c][u];
+            kahan_add(s, cc, a.dt * (acc[c] * im));
+            v[c][u] = s * tp;
+            l[c][u] = cc * tp;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c][u] = (v[c][u] + a.dt * (acc[c] * im)) * tp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// receivers: one wave per receiver, all its channels
+// ---------------------------------------------------------------------------
+template <int P, typename R>
+__global__ void __launch_bounds__(64)
+record_kernel(int nx, int ny, int nel, const R *vx, const R *vy, const R *vz,
+              const R *lx, const R *ly, const R *lz,
+              const R *sig, const int *rec_e, const double *rec_vm,
+              const int *rec_nch, const int *rec_ch, double *traces, int nt, int it)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1;
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = rec_e[r];
+    const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
+    const long fs = (long)nel * NP;
+    double acc[4] = {0, 0, 0, 0};   // vx, vy, vz, pressure
+    for (int n = lane; n < NP; n += 64) {
+        const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
+        const long u = unique_index<P>(ex, ey, ez, i, j, k, nx, ny);
+        const double w = rec_vm[(long)r * NP + n];
+        // compensated velocity: value = hi - lo
+        acc[0] += w * ((double)vx[u] - (lx ? (double)lx[u] : 0.0));
+        acc[1] += w * ((double)vy[u] - (ly ? (double)ly[u] : 0.0));
+        acc[2] += w * ((double)vz[u] - (lz ? (double)lz[u] : 0.0));
+        const long eb = (long)e * NP + n;
+        const double tr = (double)sig[XX * fs + eb] + (double)sig[YY * fs + eb] + (double)sig[ZZ * fs + eb];
+        acc[3] += -((double)(1.0f / 3.0f) * tr) * w;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_down(acc[c], off, 64);
+    if (lane == 0) {
+        const int nch = rec_nch[r];
+        for (int q = 0; q < nch; ++q) {
+            const int ch = rec_ch[r * 8 + q];
+            double val = 0.0;
+            switch (ch) {
+            case 0: val = acc[0]; break;          // vx
+            case 1: val = acc[1]; break;          // vy
+            case 2: val = acc[2]; break;          // vz
+            case 3: val = -acc[0]; break;         // n = -vx
+            case 4: val = acc[1]; break;          // e = vy
+            case 5: val = acc[2]; break;          // z = vz
+            case 6: val = acc[3]; break;          // pressure
+            default: break;
+            }
+            traces[((long)r * 8 + q) * nt + (it - 1)] = val;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// setup and I/O kernels
+// ---------------------------------------------------------------------------
+
+// reference layout [n][e] (FP64) -> device layout [e][n] (R)
+template <typename R>
+__global__ void to_device_layout(const double *src, R *dst, long nel, int NP)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nel * NP) return;
+    long e = t / NP;
+    int n = (int)(t - e * NP);
+    dst[t] = (R)src[(long)n * nel + e];
+}
+
+template <typename R>
+__global__ void from_device_layout(const R *src, double *dst, long nel, int NP)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nel * NP) return;
+    long n = t / nel;
+    long e = t - n * nel;
+    dst[t] = (double)src[e * NP + n];
+}
+
+// unique-node field -> reference layout (every copy)
+template <int P, typename R>
+__global__ void expand_unique(const R *src, const R *lo, double *dst, int nx, int ny, int nz)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1;
+    const long nel = (long)nx * ny * nz;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nel * NP) return;
+    long n = t / nel;
+    long e = t - n * nel;
+    int i = (int)(n % P1), j = (int)((n / P1) % P1), k = (int)(n / (P1 * P1));
+    int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((long)nx * ny));
+    long u = unique_index<P>(ex, ey, ez, i, j, k, nx, ny);
+    double v = (double)src[u];
+    if (lo) v -= (double)lo[u];
+    dst[t] = v;
+}
+
+// reference layout -> unique-node field, taking the owner element's copy
+template <int P, typename R>
+__global__ void gather_unique(const double *src, R *dst, R *lo, int nx, int ny, int nz, long nblk)
+{
+    constexpr int P1 = P + 1, PB = P * P * P;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long b = t / PB;
+    int o = (int)(t - b * PB);
+    if (b >= nblk) return;
+    int li = o % P, lj = (o / P) % P, lk = o / (P * P);
+    int nbx = nx + 1, nby = ny + 1;
+    int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
+    if ((bx == nx && li) || (by == ny && lj) || (bz == nz && lk)) return;
+    int ex = bx < nx ? bx : nx - 1, i = bx < nx ? li : P;
+    int ey = by < ny ? by : ny - 1, j = by < ny ? lj : P;
+    int ez = bz < nz ? bz : nz - 1, k = bz < nz ? lk : P;
+    long nel = (long)nx * ny * nz;
+    long e = ex + (long)nx * (ey + (long)ny * ez);
+    int n = i + P1 * (j + P1 * k);
+    double v = src[(long)n * nel + e];
+    dst[t] = (R)v;
+    if (lo) lo[t] = (R)(v - (double)(R)v) * R(-1);
+}
+
+// Material: lambda, mu, tau (= 6.0814 Q^-1.015, src/model_paras.F90:407),
+// rho from the FP64 staging arrays in reference layout.
+template <typename R>
+__global__ void material_kernel(const double *rhoinv, const double *lam, const double *mu,
+                                const double *q, const double *A, const double *B, const double *C,
+                                R *dlam, R *dmu, R *dtau, R *drho, R *dA, R *dB, R *dC,
+                                long nel, int NP)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nel * NP) return;
+    long e = t / NP;
+    int n = (int)(t - e * NP);
+    long s = (long)n * nel + e;
+    dlam[t] = lam ? (R)lam[s] : R(0);
+    dmu[t] = mu ? (R)mu[s] : R(0);
+    if (dtau) {
+        double qq = q ? q[s] : 1.79769313486231570815e+308;
+        dtau[t] = (R)((double)6.0814f * pow(qq, (double)-1.015f));
+    }
+    double ri = rhoinv ? rhoinv[s] : 0.0;
+    drho[t] = (R)(ri > 0.0 ? 1.0 / ri : 0.0);
+    if (dA) {
+        dA[t] = A ? (R)A[s] : R(0);
+        dB[t] = B ? (R)B[s] : R(0);
+        dC[t] = C ? (R)C[s] : R(0);
+    }
+}
+
+// Mass matrix -J r^2 sin(theta) rho W per copy (src/model_paras.F90:411),
+// assembled with the same theta -> phi -> r summation tree as the force.
+template <int P, typename R>
+__global__ void mass_kernel(const double *rhoinv, const double *sin_t, const double *rad,
+                            const double *w, double jac, double *mm64, R *inv_mm,
+                            int nx, int ny, int nz, long nblk)
+{
+    constexpr int P1 = P + 1, PB = P * P * P;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long b = t / PB;
+    int o = (int)(t - b * PB);
+    if (b >= nblk) return;
+    int li = o % P, lj = (o / P) % P, lk = o / (P * P);
+    int nbx = nx + 1, nby = ny + 1;
+    int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
+    if ((bx == nx && li) || (by == ny && lj) || (bz == nz && lk)) {
+        inv_mm[t] = R(0);
+        mm64[t] = 0.0;
+        return;
+    }
+    const long nel = (long)nx * ny * nz;
+    bool px[2] = {bx < nx, li == 0 && bx > 0};
+    bool py[2] = {by < ny, lj == 0 && by > 0};
+    bool pz[2] = {bz < nz, lk == 0 && bz > 0};
+    double zsum = 0;
+    bool zhas = false;
+    for (int dz = 0; dz < 2; ++dz) {
+        if (!pz[dz]) continue;
+        double ysum = 0;
+        bool yhas = false;
+        for (int dy = 0; dy < 2; ++dy) {
+            if (!py[dy]) continue;
+            double xsum = 0;
+            bool xhas = false;
+            for (int dx = 0; dx < 2; ++dx) {
+                if (!px[dx]) continue;
+                int ex = bx - dx, ey = by - dy, ez = bz - dz;
+                int i = li + dx * P, j = lj + dy * P, k = lk + dz * P;
+                long e = ex + (long)nx * (ey + (long)ny * ez);
+                int n = i + P1 * (j + P1 * k);
+                double ri = rhoinv ? rhoinv[(long)n * nel + e] : 0.0;
+                double rho = ri > 0.0 ? 1.0 / ri : 0.0;
+                double r = rad[ez * P1 + k];
+                double ww = w[i] * w[j] * w[k];
+                double val = -(jac * (r * r) * sin_t[ex * P1 + i] * rho * ww);
+                xsum = xhas ? xsum + val : val;
+                xhas = true;
+            }
+            ysum = yhas ? ysum + xsum : xsum;
+            yhas = true;
+        }
+        zsum = zhas ? zsum + ysum : ysum;
+        zhas = true;
+    }
+    mm64[t] = zsum;
+    inv_mm[t] = (R)(zsum != 0.0 ? 1.0 / zsum : 0.0);
+}
+
+// Courant number partial maxima (src/model_paras.F90:529-561), FP64.
+template <int P>
+__global__ void cfl_kernel(const double *rhoinv, const double *lam, const double *mu,
+                           const double *theta, const double *phi, const double *rad,
+                           int nx, int ny, int nz, double *partial)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1;
+    const long nel = (long)nx * ny * nz;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double m[3] = {0, 0, 0};
+    if (t < nel * NP) {
+        long n = t / nel, e = t - n * nel;
+        int i = (int)(n % P1), j = (int)((n / P1) % P1), k = (int)(n / (P1 * P1));
+        int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((long)nx * ny));
+        double ri = rhoinv ? rhoinv[t] : 0.0;
+        double l = lam ? lam[t] : 0.0, u = mu ? mu[t] : 0.0;
+        double cp = sqrt((l + 2.0 * u) * ri), cs = sqrt(u * ri);
+        double vmax = cs < cp ? cp : cs;
+        double r = rad[ez * P1 + k];
+        if (i >= 1) m[0] = vmax / (r * (theta[ex * P1 + i] - theta[ex * P1 + i - 1]));
+        if (j >= 1) m[1] = vmax / (r * sin(theta[ex * P1 + i]) * (phi[ey * P1 + j] - phi[ey * P1 + j - 1]));
+        if (k >= 1) m[2] = vmax / (rad[ez * P1 + k - 1] - r);
+    }
+    __shared__ double sm[3][256];
+    for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] = m[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 3; ++c)
+                sm[c][threadIdx.x] = fmax(sm[c][threadIdx.x], sm[c][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) partial[blockIdx.x * 3 + c] = sm[c][0];
+}
+
+// v range and kinetic energy partials over unique / element-local nodes
+template <int P, typename R>
+__global__ void vrange_kernel(const R *vx, const R *vy, const R *vz, int nx, int ny, int nz,
+                              long nblk, double *partial)
+{
+    constexpr int PB = P * P * P;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long b = t / PB;
+    int o = (int)(t - b * PB);
+    double mn = 1e308, mx = -1e308;
+    if (b < nblk) {
+        int li = o % P, lj = (o / P) % P, lk = o / (P * P);
+        int nbx = nx + 1, nby = ny + 1;
+        int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
+        if (!((bx == nx && li) || (by == ny && lj) || (bz == nz && lk))) {
+            double a0 = vx[t], a1 = vy[t], a2 = vz[t];
+            mn = fmin(a0, fmin(a1, a2));
+            mx = fmax(a0, fmax(a1, a2));
+        }
+    }
+    __shared__ double s0[256], s1[256];
+    s0[threadIdx.x] = mn;
+    s1[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s0[threadIdx.x] = fmin(s0[threadIdx.x], s0[threadIdx.x + s]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = s0[0];
+        partial[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+template <int P, typename R>
+__global__ void kinetic_kernel(const R *vx, const R *vy, const R *vz, const R *rho,
+                               const ThTab<R> *tth, const RTab<R> *tr, const R *w,
+                               int nx, int ny, int nz, double *partial)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1;
+    const long nel = (long)nx * ny * nz;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double acc = 0.0;
+    if (t < nel * NP) {
+        long e = t / NP;
+        int n = (int)(t - e * NP);
+        int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
+        int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((long)nx * ny));
+        long u = unique_index<P>(ex, ey, ez, i, j, k, nx, ny);
+        double a0 = vx[u], a1 = vy[u], a2 = vz[u];
+        double r = tr[ez * P1 + k].r;
+        acc = (a0 * a0 + a1 * a1 + a2 * a2) * (r * r * (double)tth[ex * P1 + i].sin_t) *
+              (double)rho[t] * ((double)w[i] * w[j] * w[k]);
+    }
+    __shared__ double s0[256];
+    s0[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) s0[threadIdx.x] += s0[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = s0[0];
+}
+
+}  // namespace s3d

@@ -1,0 +1,289 @@
+"""Python host interface over the C ABI in include/ses3d_hip.h.
+
+This is the Python mirror of the reference's Fortran module API for the hot
+path (evolution_mod, source/receiver channels, src/ses3d-nt.F90:240-270):
+one `Simulation` per mesh, loaded with a model, sources and receivers, then
+stepped. It loads the in-tree ses3d-nt_amd/libses3d_hip.so and fails loudly
+when that library is missing: there is no CPU fallback.
+
+6-D arrays are float64 numpy arrays of shape (p+1, p+1, p+1, nz, ny, nx) in C
+order, byte-identical to the reference's column-major
+f(0:nx-1, 0:ny-1, 0:nz-1, 0:p, 0:p, 0:p).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libses3d_hip.so")
+_LIB = None
+
+FP64, FP32 = 0, 1
+RHOINV, LAMBDA, MU, A, B, C_, Q = range(7)
+VX, VY, VZ = 0, 1, 2
+SXX, SXY, SXZ, SYY, SYZ, SZZ = 3, 4, 5, 6, 7, 8
+MM, TAPER = 18, 19
+MXX = 32
+CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6}
+WAVELET = {"RICKER": 0, "GAUSS": 1, "SIN**3": 2, "BRUNE": 3, "DELTA": 4}
+
+EXPORTS = [
+    "s3d_last_error", "s3d_device_count", "s3d_create", "s3d_destroy",
+    "s3d_set_model_slab", "s3d_set_model", "s3d_init", "s3d_cfl", "s3d_gll",
+    "s3d_locate", "s3d_wavelet", "s3d_add_source_mt", "s3d_add_source_sf",
+    "s3d_add_receiver", "s3d_step", "s3d_current_step", "s3d_get_trace",
+    "s3d_get_field", "s3d_set_field", "s3d_vrange", "s3d_kinetic_energy",
+    "s3d_synchronize", "s3d_timing_enable", "s3d_kernel_times",
+    "s3d_device_bytes",
+]
+
+
+class S3dGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("lpd", C.c_int),
+                ("lat_min", C.c_double), ("lat_max", C.c_double),
+                ("lon_min", C.c_double), ("lon_max", C.c_double),
+                ("rad_min", C.c_double), ("rad_max", C.c_double),
+                ("taper_width", C.c_int), ("taper_slope", C.c_double),
+                ("dt", C.c_double), ("nt", C.c_int)]
+
+
+class S3dOptions(C.Structure):
+    _fields_ = [("precision", C.c_int), ("kahan_v", C.c_int),
+                ("kahan_sigma", C.c_int), ("device", C.c_int),
+                ("xcd_remap", C.c_int)]
+
+
+class S3dError(RuntimeError):
+    pass
+
+
+def lib():
+    """The HIP library; raises when it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise S3dError("libses3d_hip.so not built: run python -c "
+                           "'import __graft_entry__ as g; g.build()'")
+        L = C.CDLL(LIB_PATH)
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        vp = C.c_void_p
+        L.s3d_last_error.restype = C.c_char_p
+        L.s3d_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.s3d_create.argtypes = [C.POINTER(S3dGrid), C.POINTER(S3dOptions), C.POINTER(vp)]
+        L.s3d_destroy.argtypes = [vp]
+        L.s3d_set_model_slab.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
+        L.s3d_set_model.argtypes = [vp, C.c_int, dp]
+        L.s3d_init.argtypes = [vp]
+        L.s3d_cfl.argtypes = [vp, C.POINTER(C.c_double)]
+        L.s3d_gll.argtypes = [C.c_int, dp, dp, dp]
+        L.s3d_locate.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, dp]
+        L.s3d_wavelet.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp]
+        L.s3d_add_source_mt.argtypes = [vp, ip, dp, dp]
+        L.s3d_add_source_sf.argtypes = [vp, ip, dp, dp]
+        L.s3d_add_receiver.argtypes = [vp, ip, dp, C.c_int, ip, C.POINTER(C.c_int)]
+        L.s3d_step.argtypes = [vp, C.c_int]
+        L.s3d_current_step.argtypes = [vp, C.POINTER(C.c_int)]
+        L.s3d_get_trace.argtypes = [vp, C.c_int, C.c_int, dp]
+        L.s3d_get_field.argtypes = [vp, C.c_int, dp]
+        L.s3d_set_field.argtypes = [vp, C.c_int, dp]
+        L.s3d_vrange.argtypes = [vp, dp]
+        L.s3d_kinetic_energy.argtypes = [vp, C.POINTER(C.c_double)]
+        L.s3d_synchronize.argtypes = [vp]
+        L.s3d_timing_enable.argtypes = [vp, C.c_int]
+        L.s3d_kernel_times.argtypes = [vp, dp, ip]
+        L.s3d_device_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        _LIB = L
+    return _LIB
+
+
+def _chk(rc):
+    if rc != 0:
+        raise S3dError(lib().s3d_last_error().decode())
+    return rc
+
+
+def device_count():
+    n = C.c_int(0)
+    lib().s3d_device_count(C.byref(n))
+    return n.value
+
+
+def gll(p):
+    k, w, d = np.zeros(p + 1), np.zeros(p + 1), np.zeros((p + 1) ** 2)
+    _chk(lib().s3d_gll(p, k, w, d))
+    return k, w, d.reshape(p + 1, p + 1)
+
+
+def wavelet(name, onset, width, dt, nt):
+    out = np.zeros(nt)
+    _chk(lib().s3d_wavelet(WAVELET[name], onset, width, dt, nt, out))
+    return out
+
+
+class Simulation:
+    """Device state of one simulation (one mesh, one model)."""
+
+    def __init__(self, grid, precision="fp32", kahan_v=True, kahan_sigma=False,
+                 device=0, xcd_remap=True):
+        g = S3dGrid(grid.nx, grid.ny, grid.nz, grid.lpd, grid.lat_min,
+                    grid.lat_max, grid.lon_min, grid.lon_max, grid.rad_min,
+                    grid.rad_max, grid.taper_width, grid.taper_slope, grid.dt,
+                    grid.nt)
+        o = S3dOptions(FP32 if precision == "fp32" else FP64, int(kahan_v),
+                       int(kahan_sigma), int(device), int(xcd_remap))
+        h = C.c_void_p()
+        _chk(lib().s3d_create(C.byref(g), C.byref(o), C.byref(h)))
+        self._h = h
+        self.grid = grid
+        self.p = grid.lpd
+        self.precision = precision
+        self.shape = (grid.lpd + 1,) * 3 + (grid.nz, grid.ny, grid.nx)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().s3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- setup -------------------------------------------------------------
+    def set_model(self, param, field):
+        _chk(lib().s3d_set_model(self._h, param, np.ascontiguousarray(field, np.float64).reshape(-1)))
+
+    def set_model_slab(self, param, ez0, field):
+        f = np.ascontiguousarray(field, np.float64)
+        nez = f.shape[3]
+        _chk(lib().s3d_set_model_slab(self._h, param, ez0, nez, f.reshape(-1)))
+
+    def init(self):
+        _chk(lib().s3d_init(self._h))
+
+    def cfl(self):
+        x = C.c_double()
+        _chk(lib().s3d_cfl(self._h, C.byref(x)))
+        return x.value
+
+    def locate(self, lat, lon, depth):
+        e = np.zeros(3, np.int32)
+        vm = np.zeros((self.p + 1) ** 3)
+        rc = lib().s3d_locate(self._h, lat, lon, depth, e, vm)
+        if rc == 1:
+            return None
+        _chk(rc)
+        return e, vm
+
+    def add_source_mt(self, exyz, vm, stf6):
+        _chk(lib().s3d_add_source_mt(self._h, np.asarray(exyz, np.int32),
+                                     np.ascontiguousarray(vm, np.float64),
+                                     np.ascontiguousarray(stf6, np.float64).reshape(-1)))
+
+    def add_source_sf(self, exyz, vm, stf3):
+        _chk(lib().s3d_add_source_sf(self._h, np.asarray(exyz, np.int32),
+                                     np.ascontiguousarray(vm, np.float64),
+                                     np.ascontiguousarray(stf3, np.float64).reshape(-1)))
+
+    def add_receiver(self, exyz, vm, channels):
+        ch = np.array([CH[c] for c in channels], np.int32)
+        rid = C.c_int()
+        _chk(lib().s3d_add_receiver(self._h, np.asarray(exyz, np.int32),
+                                    np.ascontiguousarray(vm, np.float64), len(ch), ch,
+                                    C.byref(rid)))
+        return rid.value
+
+    # -- stepping and outputs ------------------------------------------------
+    def step(self, n=1):
+        _chk(lib().s3d_step(self._h, n))
+
+    @property
+    def it(self):
+        x = C.c_int()
+        _chk(lib().s3d_current_step(self._h, C.byref(x)))
+        return x.value
+
+    def trace(self, rec, ch):
+        out = np.zeros(self.grid.nt)
+        _chk(lib().s3d_get_trace(self._h, rec, ch, out))
+        return out
+
+    def get(self, field):
+        out = np.zeros(self.shape)
+        _chk(lib().s3d_get_field(self._h, field, out.reshape(-1)))
+        return out
+
+    def set(self, field, arr):
+        _chk(lib().s3d_set_field(self._h, field, np.ascontiguousarray(arr, np.float64).reshape(-1)))
+
+    def vrange(self):
+        out = np.zeros(2)
+        _chk(lib().s3d_vrange(self._h, out))
+        return out
+
+    def kinetic_energy(self):
+        x = C.c_double()
+        _chk(lib().s3d_kinetic_energy(self._h, C.byref(x)))
+        return x.value
+
+    def synchronize(self):
+        _chk(lib().s3d_synchronize(self._h))
+
+    def timing(self, enable=True):
+        _chk(lib().s3d_timing_enable(self._h, int(enable)))
+
+    def kernel_times(self):
+        ms = np.zeros(3)
+        cnt = np.zeros(3, np.int32)
+        _chk(lib().s3d_kernel_times(self._h, ms, cnt))
+        return ms, cnt
+
+    def device_bytes(self):
+        x = C.c_double()
+        _chk(lib().s3d_device_bytes(self._h, C.byref(x)))
+        return x.value
+
+
+def build_case(case, precision="fp32", kahan_v=True, kahan_sigma=False,
+               slab=None, xcd_remap=True):
+    """Simulation loaded with a synth.Case (model, sources, receivers).
+    The model is uploaded `slab` element layers at a time (all at once when
+    None). Returns (simulation, list of receivers that were placed)."""
+    g = case.grid
+    sim = Simulation(g, precision, kahan_v, kahan_sigma, xcd_remap=xcd_remap)
+    nz = g.nz
+    step = slab or nz
+    for ez0 in range(0, nz, step):
+        nez = min(step, nz - ez0)
+        m = case.model_slab(ez0, nez)
+        sim.set_model_slab(RHOINV, ez0, m["rhoinv"])
+        sim.set_model_slab(LAMBDA, ez0, m["lambda"])
+        sim.set_model_slab(MU, ez0, m["mu"])
+        if "q" in m:
+            sim.set_model_slab(Q, ez0, m["q"])
+        del m
+    sim.init()
+    for s in case.sources:
+        loc = sim.locate(s.lat, s.lon, s.depth)
+        if loc is None:
+            continue
+        f = wavelet(s.wavelet, s.onset, s.width, g.dt, g.nt)
+        sim.add_source_mt(loc[0], loc[1], np.outer(np.asarray(s.moment_tensor), f))
+    placed = []
+    for r in case.receivers:
+        loc = sim.locate(r.lat, r.lon, r.depth)
+        if loc is None:
+            continue
+        sim.add_receiver(loc[0], loc[1], r.channels)
+        placed.append(r)
+    return sim, placed

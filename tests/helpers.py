@@ -1,0 +1,53 @@
+"""Shared test helpers: seeded random states and error norms."""
+import numpy as np
+
+from ses3d_amd import synth
+
+
+def small_grid(nx=3, ny=4, nz=2, lpd=4, nt=50, taper_width=1, dt=0.2):
+    return synth.Grid(nx, ny, nz, lpd, -3.0, 4.0, 10.0, 18.0, 6071000.0,
+                      6371000.0, taper_width, 1.2, dt, nt)
+
+
+def unique_to_local(u, grid):
+    """Global unique-node array u[gz, gy, gx] -> 6-D element-local copies."""
+    p = grid.lpd
+    P1 = p + 1
+    i = np.arange(P1)
+    gx = (np.arange(grid.nx)[None, :] * p + i[:, None])      # (P1, nx)
+    gy = (np.arange(grid.ny)[None, :] * p + i[:, None])
+    gz = (np.arange(grid.nz)[None, :] * p + i[:, None])
+    # out[k, j, i, ez, ey, ex] = u[gz[k,ez], gy[j,ey], gx[i,ex]]
+    return u[gz[:, None, None, :, None, None], gy[None, :, None, None, :, None],
+             gx[None, None, :, None, None, :]]
+
+
+def random_unique(grid, rng, scale=1.0):
+    p = grid.lpd
+    shp = (grid.nz * p + 1, grid.ny * p + 1, grid.nx * p + 1)
+    return scale * rng.standard_normal(shp)
+
+
+def random_model(grid, rng, visco=False, aniso=False):
+    P1 = grid.lpd + 1
+    shp = (P1, P1, P1, grid.nz, grid.ny, grid.nx)
+    vp = rng.uniform(5500.0, 8000.0, shp)
+    vs = vp / rng.uniform(1.65, 1.85, shp)
+    rho = rng.uniform(2600.0, 3400.0, shp)
+    mu = rho * vs * vs
+    lam = rho * vp * vp - 2 * mu
+    m = {"rhoinv": 1.0 / rho, "lambda": lam, "mu": mu}
+    if visco:
+        m["q"] = rng.uniform(80.0, 600.0, shp)
+    if aniso:
+        m["a"] = 0.02 * lam * rng.standard_normal(shp)
+        m["b"] = 0.02 * mu * rng.standard_normal(shp)
+        m["c"] = 0.02 * lam * rng.standard_normal(shp)
+    return m
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    d = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (d if d > 0 else 1.0)

@@ -1,0 +1,101 @@
+"""HIP path vs the FP64 oracle, through the C ABI (libses3d_hip.so)."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import random_model, random_unique, rel_l2, small_grid, unique_to_local
+from ses3d_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+PARAMS = {"rhoinv": (oracle.RHOINV, api.RHOINV), "lambda": (oracle.LAMBDA, api.LAMBDA),
+          "mu": (oracle.MU, api.MU), "a": (oracle.A, api.A), "b": (oracle.B, api.B),
+          "c": (oracle.C_, api.C_), "q": (oracle.Q, api.Q)}
+
+
+def _pair(grid, model, precision="fp64", kahan_v=True, kahan_sigma=False):
+    o = oracle.Oracle(grid)
+    s = api.Simulation(grid, precision, kahan_v=kahan_v, kahan_sigma=kahan_sigma)
+    for k, arr in model.items():
+        o.set_model(PARAMS[k][0], arr)
+        s.set_model(PARAMS[k][1], arr)
+    o.init()
+    s.init()
+    return o, s
+
+
+def _random_state(o, s, grid, rng, visco=False):
+    for f in (api.VX, api.VY, api.VZ):
+        v = unique_to_local(random_unique(grid, rng, 1e-3), grid)
+        o.set(f, v)
+        s.set(f, v)
+    P1 = grid.lpd + 1
+    shp = (P1, P1, P1, grid.nz, grid.ny, grid.nx)
+    for f in range(api.SXX, api.SZZ + 1):
+        sg = 1e5 * rng.standard_normal(shp)
+        o.set(f, sg)
+        s.set(f, sg)
+    if visco:
+        for f in range(api.MXX, api.MXX + 18):
+            m = 1e-9 * rng.standard_normal(shp)
+            o.set(f, m)
+            s.set(f, m)
+
+
+FIELDS = [api.VX, api.VY, api.VZ] + list(range(api.SXX, api.SZZ + 1))
+
+
+@pytest.mark.parametrize("lpd", [4, 2, 3, 5, 6, 7])
+def test_one_step_fp64_random_state(lpd):
+    grid = small_grid(lpd=lpd, nx=3, ny=2, nz=2)
+    rng = np.random.default_rng(100 + lpd)
+    o, s = _pair(grid, random_model(grid, rng))
+    _random_state(o, s, grid, rng)
+    o.step(1)
+    s.step(1)
+    for f in FIELDS:
+        assert rel_l2(s.get(f), o.get(f)) < 1e-12, f
+
+
+@pytest.mark.parametrize("aniso", [False, True])
+def test_one_step_fp64_visco(aniso):
+    grid = small_grid(nx=3, ny=3, nz=2)
+    rng = np.random.default_rng(7)
+    o, s = _pair(grid, random_model(grid, rng, visco=True, aniso=aniso))
+    assert o.visco
+    _random_state(o, s, grid, rng, visco=True)
+    for _ in range(3):
+        o.step(1)
+        s.step(1)
+    for f in FIELDS + list(range(api.MXX, api.MXX + 18)):
+        assert rel_l2(s.get(f), o.get(f)) < 1e-12, f
+
+
+def test_mass_matrix_and_taper_fp64():
+    grid = small_grid(nx=4, ny=3, nz=3, taper_width=1)
+    rng = np.random.default_rng(3)
+    o, s = _pair(grid, random_model(grid, rng))
+    assert rel_l2(s.get(api.MM), o.get(oracle.MM)) < 1e-14
+    assert rel_l2(s.get(api.TAPER), o.get(oracle.TAPER)) < 1e-15
+    assert abs(s.cfl() - o.cfl()) < 1e-14 * o.cfl()
+
+
+def _run_case(case, precision, steps, **kw):
+    sim, recs = api.build_case(case, precision=precision, **kw)
+    sim.step(steps)
+    return sim, recs
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 1e-4)])
+def test_homogeneous_traces(precision, tol):
+    case = synth.homogeneous(nt=300, nx=8, ny=8, nz=5, n_rec=6)
+    o, orecs = oracle.build_case(case)
+    o.step(300)
+    sim, recs = _run_case(case, precision, 300)
+    assert len(recs) == len(orecs) > 0
+    errs = []
+    for r in range(len(recs)):
+        for ch in range(3):
+            errs.append(rel_l2(sim.trace(r, ch), o.trace(r, ch)))
+    print(precision, "max relL2", max(errs), "median", np.median(errs))
+    assert max(errs) < tol
