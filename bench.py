@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X forward time step: GLL-node updates per second.
+
+Workload (default): the layered visco-elastic precision-study mesh,
+192 x 192 x 96 elements, GLL order 4 (442 368 000 element-local nodes), three
+SLS relaxation mechanisms, one moment-tensor source, 256 surface receivers,
+synthetic seeded model (synth.layered). One "step" is one iteration of the
+reference's forward loop (src/ses3d-nt.F90:240-270): evolution, source add,
+receiver record.
+
+Prints ONE JSON line. `value` is element-local GLL-node updates per second of
+the FP32 solver (compensated velocity integration) with every input resident
+in HBM. The same mesh is also stepped in FP64 on the GPU for the FP32 / FP64
+speedup, and the FP64 oracle (oracle/liboracle.so, C + OpenMP) is timed on a
+bounded sample of the same workload on the host cores as `cpu_baseline`.
+Kernel times come from HIP events recorded on the library's own stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ses3d-nt_amd"))
+
+import numpy as np  # noqa: E402
+
+from ses3d_amd import api, synth  # noqa: E402
+
+HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+CPU_SAMPLE = dict(nx=24, ny=24, nz=12)   # oracle sample of the same workload
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def step_bytes(s_state, s_static, u, visco, kahan_v, kahan_sigma, L=3, S=1):
+    """Algorithmic HBM bytes per element-local node per step:
+    s_state*[12 + 12 L visco + (6 + 6 Kv) u + 12 Ks] + s_static*[(2+visco) + u]/S"""
+    return (s_state * (12 + 12 * L * visco + (6 + 6 * kahan_v) * u + 12 * kahan_sigma)
+            + s_static * ((2 + visco) + u) / S)
+
+
+def elem_kernel_bytes(s, u, visco, kahan_sigma):
+    """Algorithmic bytes per element-local node of one element-kernel launch:
+    stress r+w (12 words, +12 compensation), memory variables r+w (36 words),
+    lambda, mu (+tau) reads, the velocity read once per unique node (3u words)
+    and the element force write (3 words)."""
+    return s * (12 + 12 * kahan_sigma + 36 * visco + (2 + visco) + 3 * u + 3)
+
+
+def load_sims(case, variants, slab):
+    """Build one Simulation per (precision, kahan_v) variant from a single
+    pass over the model slabs; returns a list of (name, sim)."""
+    g = case.grid
+    sims = [(name, api.Simulation(g, prec, kahan_v=kv)) for name, prec, kv in variants]
+    t0 = time.time()
+    for ez0 in range(0, g.nz, slab):
+        nez = min(slab, g.nz - ez0)
+        m = case.model_slab(ez0, nez)
+        for _, sim in sims:
+            sim.set_model_slab(api.RHOINV, ez0, m["rhoinv"])
+            sim.set_model_slab(api.LAMBDA, ez0, m["lambda"])
+            sim.set_model_slab(api.MU, ez0, m["mu"])
+            if "q" in m:
+                sim.set_model_slab(api.Q, ez0, m["q"])
+        if ez0 % (8 * slab) == 0:
+            log("[bench] model layers %d..%d uploaded (%.0f s)" % (ez0, ez0 + nez - 1, time.time() - t0))
+    for name, sim in sims:
+        sim.init()
+        nrec = 0
+        for s_ in case.sources:
+            loc = sim.locate(s_.lat, s_.lon, s_.depth)
+            if loc is not None:
+                f = api.wavelet(s_.wavelet, s_.onset, s_.width, g.dt, g.nt)
+                sim.add_source_mt(loc[0], loc[1], np.outer(np.asarray(s_.moment_tensor), f))
+        for r in case.receivers:
+            loc = sim.locate(r.lat, r.lon, r.depth)
+            if loc is not None:
+                sim.add_receiver(loc[0], loc[1], r.channels)
+                nrec += 1
+        sim.synchronize()
+        log("[bench] %s ready: device %.1f GB, courant %.3f, %d receivers (%.0f s)"
+            % (name, sim.device_bytes() / 1e9, sim.cfl(), nrec, time.time() - t0))
+    return sims
+
+
+def time_gpu(sim, steps, warmup):
+    sim.step(warmup)
+    sim.synchronize()
+    _torch_sync()
+    sim.timing(True)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    _torch_sync()
+    wall = time.perf_counter() - t0
+    ms, cnt = sim.kernel_times()
+    vr = sim.vrange()
+    sim.close()
+    return wall, ms, cnt, vr
+
+
+def _torch_sync():
+    t = sys.modules.get("torch")
+    if t is not None and t.cuda.is_available():
+        t.cuda.synchronize()
+
+
+def cpu_baseline(case_kw, target_s=15.0):
+    """FP64 oracle on a bounded sub-mesh of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    case = synth.layered(seed=case_kw["seed"], nt=100000, nx=CPU_SAMPLE["nx"],
+                         ny=CPU_SAMPLE["ny"], nz=CPU_SAMPLE["nz"], n_rec=16)
+    threads = oracle.lib().ora_set_threads(0)
+    o, _ = oracle.build_case(case)
+    o.step(2)
+    t0 = time.perf_counter()
+    o.step(3)
+    per = (time.perf_counter() - t0) / 3
+    n = max(3, int(target_s / max(per, 1e-6)))
+    t0 = time.perf_counter()
+    o.step(n)
+    wall = time.perf_counter() - t0
+    o.close()
+    g = case.grid
+    return {"value": g.nodes * n / wall, "unit": "GLL-node updates/s",
+            "cores": int(threads), "kind": "port",
+            "sample": "FP64 oracle (C, OpenMP) on a %dx%dx%d-element, p=4 sub-mesh of the "
+                      "same layered visco-elastic workload, %d steps, %.1f s"
+                      % (g.nx, g.ny, g.nz, n, wall)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=192)
+    ap.add_argument("--ny", type=int, default=192)
+    ap.add_argument("--nz", type=int, default=96)
+    ap.add_argument("--slab", type=int, default=4)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the whole mesh on a single "
+              "MI355X and has no multi-GPU path (--gpus %d requested)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if api.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+
+    nt = args.warmup + args.steps
+    case = synth.layered(seed=1, nt=nt, nx=args.nx, ny=args.ny, nz=args.nz, n_rec=256)
+    g = case.grid
+    N = g.nodes
+    u = g.unique_nodes / N
+    log("[bench] mesh %dx%dx%d p=%d: %d element-local nodes, u=%.4f, dt=%.4f s"
+        % (g.nx, g.ny, g.nz, g.lpd, N, u, g.dt))
+
+    variants = [("fp32", "fp32", True)] + ([] if args.no_fp64 else [("fp64", "fp64", False)])
+    sims = dict(load_sims(case, variants, args.slab))
+    wall, ms, cnt, vr = time_gpu(sims["fp32"], args.steps, args.warmup)
+    value = N * args.steps / wall
+    log("[bench] fp32 %.3f ms/step, elem %.3f ms, node %.3f ms, v range %s"
+        % (1e3 * wall / args.steps, ms[0] / cnt[0], ms[1] / cnt[1], vr))
+
+    elem_ms = ms[0] / max(cnt[0], 1)
+    eb = elem_kernel_bytes(4, u, 1, 0) * N
+    achieved = eb / (elem_ms * 1e-3)
+    Bstep = step_bytes(4, 4, u, 1, 1, 0)
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": None,
+            "kernel": "elem_kernel<4,float,visco>",
+            "algorithmic_bytes_per_launch": eb}
+    roof["achieved"] = achieved / 1e9
+    roof["peak"] = HBM_PEAK / 1e9
+    step_frac = Bstep * value / HBM_PEAK
+
+    out = {
+        "metric": "GLL-node updates/s, FP32 vs FP64 GPU at <=1e-5 rel L2 seismogram misfit",
+        "value": value, "unit": "GLL-node updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded layered visco-elastic model, GRF vs perturbation)",
+        "config": {"workload": "layered visco-elastic section 40x40 deg x 1400 km, "
+                               "%dx%dx%d elements, GLL order 4, 3 SLS mechanisms, FP32 with "
+                               "compensated velocity integration" % (g.nx, g.ny, g.nz),
+                   "elements": g.n_elem, "gll_nodes": N, "unique_nodes": g.unique_nodes,
+                   "precision": "fp32+kahan_v", "parallelism": "single GPU"},
+        "roofline": roof,
+        "step_roofline": {"bytes_per_node": Bstep, "achieved_GBps": Bstep * value / 1e9,
+                          "frac": step_frac,
+                          "elem_ms": elem_ms, "node_ms": ms[1] / max(cnt[1], 1),
+                          "record_ms": ms[2] / max(cnt[2], 1)},
+    }
+    if not args.no_fp64:
+        wall64, ms64, cnt64, _ = time_gpu(sims["fp64"], args.steps, args.warmup)
+        v64 = N * args.steps / wall64
+        out["fp64_gpu"] = {"value": v64, "ms_per_step": 1e3 * wall64 / args.steps,
+                           "elem_ms": ms64[0] / max(cnt64[0], 1),
+                           "node_ms": ms64[1] / max(cnt64[1], 1),
+                           "speedup_fp32_over_fp64": value / v64}
+        log("[bench] fp64 %.3f ms/step" % (1e3 * wall64 / args.steps))
+    if not args.no_cpu:
+        cb = cpu_baseline({"seed": 1})
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+        log("[bench] cpu baseline %.3e node-updates/s on %d cores" % (cb["value"], cb["cores"]))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -58,6 +58,7 @@ def lib():
         L.ora_cfl.restype = C.c_double
         L.ora_cfl.argtypes = [C.c_void_p]
         L.ora_locate.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, ip, dp]
+        L.ora_lagrange.argtypes = [C.c_int, C.c_double, dp]
         L.ora_stf.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp]
         L.ora_add_source_mt.argtypes = [C.c_void_p, ip, dp, dp]
         L.ora_add_source_sf.argtypes = [C.c_void_p, ip, dp, dp]
@@ -87,6 +88,13 @@ def gll(p):
         raise ValueError("unsupported lpd %d" % p)
     # d[i + (p+1) j] = l_i'(x_j)  ->  D[j, i]
     return k, w, d.reshape(p + 1, p + 1)
+
+
+def lagrange(p, x):
+    out = np.zeros(p + 1)
+    if lib().ora_lagrange(p, x, out):
+        raise ValueError(p)
+    return out
 
 
 def stf(wavelet, onset, width, dt, nt):

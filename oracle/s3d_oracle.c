@@ -545,6 +545,14 @@ static void lagrange_1d(const double *knots, int P1, double x, double *l)
     for (int j = 0; j < P1; ++j) l[j] = prod / (x - knots[j]) * w[j];
 }
 
+int ora_lagrange(int lpd, double x, double *l)
+{
+    double k[MAXP], w[MAXP];
+    if (ora_gll(lpd, k, w, NULL)) return 1;
+    lagrange_1d(k, lpd + 1, x, l);
+    return 0;
+}
+
 int ora_locate(ora_ctx *c, double lat, double lon, double depth, int *exyz,
                double *vm)
 {

@@ -78,6 +78,10 @@ double ora_cfl(ora_ctx *c);
 int ora_locate(ora_ctx *c, double lat, double lon, double depth,
                int *exyz, double *vm);
 
+/* Lagrange polynomials l_0..l_p of the GLL points at x
+ * (reference lagrange_p_kw, src/interpolation_mod.f90:151-171). */
+int ora_lagrange(int lpd, double x, double *l);
+
 /* Source-time function sample vector t = (i-1)*dt, i = 1..nt
  * (reference src/config/source.f90:564-652). wavelet: 0 RICKER, 1 GAUSS,
  * 2 SIN**3, 3 BRUNE, 4 DELTA. */

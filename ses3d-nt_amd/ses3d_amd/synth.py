@@ -92,7 +92,9 @@ class Case:
 
     # -- model -------------------------------------------------------------
     def model_slab(self, ez0, nez):
-        """rhoinv, lambda, mu, q for element layers ez0 .. ez0+nez-1."""
+        """rhoinv, lambda, mu, q for element layers ez0 .. ez0+nez-1.
+        For the layered model the returned arrays are work buffers that the
+        next call of the same slab size overwrites: copy them to keep them."""
         return model_slab(self, ez0, nez)
 
     def model_full(self):
@@ -213,25 +215,45 @@ class _GRF:
         self.n = n
 
     @staticmethod
-    def _interp_matrix(x, lo, hi, n):
-        t = (np.asarray(x).ravel() - lo) / (hi - lo) * (n - 1)
-        t = np.clip(t, 0.0, n - 1 - 1e-12)
+    def _weights(x, lo, hi, n):
+        t = (np.asarray(x, dtype=np.float64).ravel() - lo) / (hi - lo) * (n - 1)
+        t = np.clip(t, 0.0, n - 1 - 1e-9)
         i0 = np.floor(t).astype(np.int64)
-        f = t - i0
-        M = np.zeros((t.size, n))
-        M[np.arange(t.size), i0] = 1.0 - f
-        M[np.arange(t.size), i0 + 1] = f
-        return M
+        return i0, t - i0
 
-    def sample(self, th, ph, r):
-        """field on the tensor lattice th (a), ph (b), r (c) -> (c, b, a)"""
-        It = self._interp_matrix(th, self.t0, self.t1, self.n[0])
-        Ip = self._interp_matrix(ph, self.p0, self.p1, self.n[1])
-        Ir = self._interp_matrix(r, self.r0, self.r1, self.n[2])
-        h = np.einsum("ck,ijk->ijc", Ir, self.g)
-        h = np.einsum("bj,ijc->ibc", Ip, h)
-        h = np.einsum("ai,ibc->cba", It, h)
-        return h
+    def sample(self, th, ph, r, scratch=None):
+        """field on the tensor lattice th (a), ph (b), r (c) -> (c, b, a).
+        `scratch` (a dict) keeps the work arrays between calls of one shape:
+        reusing memory that is already mapped is much faster than fresh
+        allocations for arrays of this size."""
+        it, ft = self._weights(th, self.t0, self.t1, self.n[0])
+        ip, fp = self._weights(ph, self.p0, self.p1, self.n[1])
+        ir, fr = self._weights(r, self.r0, self.r1, self.n[2])
+        sc = {} if scratch is None else scratch
+        na, nb, nc = it.size, ip.size, ir.size
+        key = (na, nb, nc)
+        if sc.get("key") != key:
+            sc.clear()
+            sc["key"] = key
+            sc["h1"] = np.empty((self.n[0], self.n[1], nc))
+            sc["h2"] = np.empty((self.n[0], nb, nc))
+            sc["h3"] = np.empty((nc, nb, na))
+            sc["t"] = np.empty((nc, nb, na))
+        g = self.g                                            # (t, p, r)
+        h1 = sc["h1"]
+        np.multiply(g[:, :, ir], (1.0 - fr), out=h1)
+        h1 += g[:, :, ir + 1] * fr                            # (t, p, c)
+        h2 = sc["h2"]
+        np.multiply(h1[:, ip, :], (1.0 - fp)[None, :, None], out=h2)
+        h2 += h1[:, ip + 1, :] * fp[None, :, None]            # (t, b, c)
+        h2t = np.ascontiguousarray(h2.transpose(2, 1, 0))    # (c, b, t)
+        h3, tmp = sc["h3"], sc["t"]
+        np.take(h2t, it, axis=2, out=h3)
+        h3 *= (1.0 - ft)[None, None, :]
+        np.take(h2t, it + 1, axis=2, out=tmp)
+        tmp *= ft[None, None, :]
+        h3 += tmp
+        return h3                                             # (c, b, a)
 
 
 def model_slab(case, ez0, nez):
@@ -254,20 +276,31 @@ def model_slab(case, ez0, nez):
     vp1, vs1, rho1, q1 = layered_1d(rs)            # (P1[k], nez)
     if case._grf is None:
         case._grf = _GRF(g, case.seed)
-    # perturbation on (k, ez, j, ey, i, ex)
-    d = case._grf.sample(th.T.ravel(), ph.T.ravel(), rs.T.ravel())  # (nez*P1, ny*P1, nx*P1)
-    d = d.reshape(nez, P1, g.ny, P1, g.nx, P1)     # [ez,k, ey,j, ex,i]
-    d = d.transpose(1, 3, 5, 0, 2, 4)              # [k, j, i, ez, ey, ex]
+    sc = case.params.setdefault("_scratch", {})
+    if sc.get("shape") != shp:
+        sc.clear()
+        sc["shape"] = shp
+        for name in ("rhoinv", "lambda", "mu", "q"):
+            sc[name] = np.zeros(shp)
+        sc["grf"] = {}
+    # lattice orders (k, ez), (j, ey), (i, ex) keep the ex axis contiguous
+    d = case._grf.sample(th.ravel(), ph.ravel(), rs.ravel(), sc["grf"])
+    d = d.reshape(P1, nez, P1, g.ny, P1, g.nx).transpose(0, 2, 4, 1, 3, 5)
     amp = case.params.get("dvs", 0.05)
-    vs = vs1.reshape(P1, 1, 1, nez, 1, 1) * (1.0 + amp * d)
-    vp = np.broadcast_to(vp1.reshape(P1, 1, 1, nez, 1, 1), shp)
-    rho = np.broadcast_to(rho1.reshape(P1, 1, 1, nez, 1, 1), shp)
-    mu = rho * vs * vs
-    lam = rho * vp * vp - 2.0 * mu
-    out = {"rhoinv": np.ascontiguousarray(1.0 / rho), "lambda": np.ascontiguousarray(lam),
-           "mu": np.ascontiguousarray(mu)}
+    bshape = (P1, 1, 1, nez, 1, 1)
+    mu, lam, rhoinv = sc["mu"], sc["lambda"], sc["rhoinv"]
+    np.multiply(d, amp, out=mu)
+    mu += 1.0
+    mu *= vs1.reshape(bshape)                      # vs
+    mu *= mu                                       # vs^2
+    mu *= rho1.reshape(bshape)                     # rho vs^2
+    np.multiply(mu, -2.0, out=lam)
+    lam += (rho1 * vp1 * vp1).reshape(bshape)      # rho vp^2 - 2 mu
+    rhoinv[...] = (1.0 / rho1).reshape(bshape)
+    out = {"rhoinv": rhoinv, "lambda": lam, "mu": mu}
     if case.visco:
-        out["q"] = np.ascontiguousarray(np.broadcast_to(q1.reshape(P1, 1, 1, nez, 1, 1), shp))
+        sc["q"][...] = q1.reshape(bshape)
+        out["q"] = sc["q"]
     return out
 
 

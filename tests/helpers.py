@@ -51,3 +51,22 @@ def rel_l2(a, b):
     b = np.asarray(b, dtype=np.float64)
     d = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (d if d > 0 else 1.0)
+
+
+def local_to_unique(f, grid):
+    """6-D element-local copies -> unique-node array u[gz, gy, gx], taking
+    for every node the copy of the highest-index element that holds it."""
+    p = grid.lpd
+    P1 = p + 1
+    out = np.zeros((grid.nz * p + 1, grid.ny * p + 1, grid.nx * p + 1))
+    for k in range(P1):
+        for j in range(P1):
+            for i in range(P1):
+                out[k:k + grid.nz * p:p, j:j + grid.ny * p:p, i:i + grid.nx * p:p] = f[k, j, i]
+    return out
+
+
+def node_coords(o):
+    """(theta, phi, r) at every element-local node of an initialised oracle."""
+    import oracle
+    return o.get(oracle.THETA), o.get(oracle.PHI), o.get(oracle.R)

@@ -1,0 +1,312 @@
+"""Pins of the FP64 oracle's time step by what the mathematics fixes.
+
+Each test states the closed form it checks. None of them re-types the
+oracle's formula: they compare against rigid-body motions, textbook
+spherical strain rates, integrals over the spherical section, the GLL
+partition of unity, superposition, wave speed and a geometric series.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import local_to_unique, node_coords, rel_l2, small_grid
+from ses3d_amd import synth
+
+
+def _oracle(grid, vp=6000.0, vs=3500.0, rho=2700.0, q=None):
+    o = oracle.Oracle(grid)
+    mu = rho * vs * vs
+    o.set_model(oracle.RHOINV, np.full(o.shape, 1.0 / rho))
+    o.set_model(oracle.LAMBDA, np.full(o.shape, rho * vp * vp - 2 * mu))
+    o.set_model(oracle.MU, np.full(o.shape, mu))
+    if q is not None:
+        o.set_model(oracle.Q, np.full(o.shape, q))
+    o.init()
+    return o
+
+
+def _set_v(o, vt, vp_, vr):
+    o.set(oracle.VX, vt)
+    o.set(oracle.VY, vp_)
+    o.set(oracle.VZ, vr)
+
+
+def _strain(o):
+    ids = [oracle.EXX, oracle.EYY, oracle.EZZ, oracle.EXY, oracle.EXZ, oracle.EYZ]
+    return [o.get(f) for f in ids]
+
+
+def _cart_to_sph(th, ph, V):
+    """spherical (theta, phi, r) components of a Cartesian vector field V"""
+    et = np.stack([np.cos(th) * np.cos(ph), np.cos(th) * np.sin(ph), -np.sin(th)])
+    ep = np.stack([-np.sin(ph), np.cos(ph), np.zeros_like(ph)])
+    er = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)])
+    return (V * et).sum(0), (V * ep).sum(0), (V * er).sum(0)
+
+
+GRID = small_grid(nx=4, ny=3, nz=3, lpd=4, taper_width=0)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_rigid_rotation_has_zero_strain_rate(axis):
+    o = _oracle(GRID)
+    th, ph, r = node_coords(o)
+    X = np.stack([r * np.sin(th) * np.cos(ph), r * np.sin(th) * np.sin(ph), r * np.cos(th)])
+    w = np.zeros((3,) + r.shape)
+    w[axis] = 1e-6                                     # rad/s
+    V = np.cross(w, X, axis=0)
+    vt, vp_, vr = _cart_to_sph(th, ph, V)
+    _set_v(o, vt, vp_, vr)
+    o.evolution()
+    scale = 1e-6
+    for e in _strain(o):
+        assert np.max(np.abs(e)) < 1e-7 * scale
+
+
+def test_translation_has_zero_strain_rate():
+    o = _oracle(GRID)
+    th, ph, r = node_coords(o)
+    V = np.ones((3,) + r.shape) * np.array([0.3, -0.7, 0.2])[:, None, None, None, None, None, None]
+    vt, vp_, vr = _cart_to_sph(th, ph, V)
+    _set_v(o, vt, vp_, vr)
+    o.evolution()
+    for e in _strain(o):
+        assert np.max(np.abs(e)) < 1e-7 * (1.0 / 6.0e6)
+
+
+def test_dilatation_is_identity_strain_rate():
+    o = _oracle(GRID)
+    th, ph, r = node_coords(o)
+    c = 2.5e-9
+    _set_v(o, np.zeros_like(r), np.zeros_like(r), c * r)
+    o.evolution()
+    exx, eyy, ezz, exy, exz, eyz = _strain(o)
+    for d in (exx, eyy, ezz):
+        assert np.max(np.abs(d - c)) < 1e-12 * c
+    for d in (exy, exz, eyz):
+        assert np.max(np.abs(d)) < 1e-12 * c
+
+
+def test_radial_shear_matches_textbook_rate():
+    # v_theta = a r^2: e_r_theta = (d v_t/dr - v_t / r) / 2 = a r / 2,
+    # e_theta_theta = e_phi_phi = e_rr = 0 except e_phi_phi = v_t cot(theta) / r
+    o = _oracle(GRID)
+    th, ph, r = node_coords(o)
+    a = 1e-16
+    _set_v(o, a * r * r, np.zeros_like(r), np.zeros_like(r))
+    o.evolution()
+    exx, eyy, ezz, exy, exz, eyz = _strain(o)
+    assert rel_l2(exz, a * r / 2) < 1e-12
+    assert rel_l2(eyy, a * r / np.tan(th)) < 1e-12
+    assert np.max(np.abs(ezz)) < 1e-12 * np.max(a * r)
+    assert np.max(np.abs(exx)) < 1e-12 * np.max(a * r)
+
+
+def _force(o):
+    """assembled force per unique node: MM * s (s = acceleration)"""
+    mm = local_to_unique(o.get(oracle.MM), o.grid)
+    return [mm * local_to_unique(o.get(f), o.grid) for f in (oracle.SX, oracle.SY, oracle.SZ)]
+
+
+def _section_integrals(g):
+    t0 = math.radians(90.0 - g.lat_max)
+    t1 = math.radians(90.0 - g.lat_min)
+    dp = math.radians(g.lon_max - g.lon_min)
+    r0, r1 = g.rad_min, g.rad_max
+    ct = math.cos(t0) - math.cos(t1)
+    return {"vol": (r1 ** 3 - r0 ** 3) / 3 * ct * dp,
+            "r_sin": (r1 ** 2 - r0 ** 2) / 2 * ct * dp,     # int r sin(theta) dr dtheta dphi
+            "r": (r1 ** 2 - r0 ** 2) / 2 * (t1 - t0) * dp}  # int r dr dtheta dphi
+
+
+def test_mass_matrix_sums_to_minus_rho_volume():
+    o = _oracle(GRID, rho=2700.0)
+    mm = local_to_unique(o.get(oracle.MM), GRID)
+    vol = _section_integrals(GRID)["vol"]
+    assert abs(mm.sum() + 2700.0 * vol) < 1e-11 * 2700.0 * vol
+
+
+@pytest.mark.parametrize("comp", ["xx", "yy", "zz", "xy"])
+def test_weak_divergence_integration_by_parts(comp):
+    """With v = 0 and a constant stress component C, the discrete weak
+    divergence contracted with a global coordinate reproduces the section
+    integral of C times the metric factor (GLL partition of unity makes the
+    plain sum vanish)."""
+    g = GRID
+    o = _oracle(g)
+    th, ph, r = node_coords(o)
+    C = 1e6
+    fid = {"xx": oracle.SXX, "yy": oracle.SYY, "zz": oracle.SZZ, "xy": oracle.SXY}[comp]
+    o.set(fid, np.full(r.shape, C))
+    o.evolution()
+    fx, fy, fz = _force(o)
+    TH, PH, R = (local_to_unique(a, g) for a in (th, ph, r))
+    I = _section_integrals(g)
+    for f in (fx, fy, fz):
+        assert abs(f.sum()) <= 1e-10 * np.abs(f).sum()
+    if comp == "xx":
+        # sum_i theta_i l_i' = dtheta/dxi: int sigma_tt / r dV
+        assert abs((fx * TH).sum() - C * I["r_sin"]) < 1e-9 * C * I["r_sin"]
+    elif comp == "yy":
+        assert abs((fy * PH).sum() - C * I["r"]) < 1e-9 * C * I["r"]
+    elif comp == "zz":
+        assert abs((fz * R).sum() - C * I["vol"]) < 1e-9 * C * I["vol"]
+    else:
+        assert abs((fx * PH).sum() - C * I["r"]) < 1e-9 * C * I["r"]
+        assert abs((fy * TH).sum() - C * I["r_sin"]) < 1e-9 * C * I["r_sin"]
+
+
+def test_assembly_is_sum_of_element_contributions():
+    g = small_grid(nx=3, ny=3, nz=2, lpd=3, taper_width=0)
+    rng = np.random.default_rng(5)
+    P1 = g.lpd + 1
+    shp = (P1, P1, P1, g.nz, g.ny, g.nx)
+
+    def force_of(mask):
+        o = _oracle(g)
+        s = 1e5 * rng_fixed * mask
+        o.set(oracle.SXX, s)
+        o.evolution()
+        return _force(o)[0], o
+
+    rng_fixed = rng.standard_normal(shp)
+    m1 = np.zeros(shp); m1[..., 1, 1, 1] = 1.0      # element (1,1,1)
+    m2 = np.zeros(shp); m2[..., 0, 1, 1] = 1.0      # its theta neighbour (0,1,1)
+    f1, _ = force_of(m1)
+    f2, _ = force_of(m2)
+    f12, o12 = force_of(m1 + m2)
+    assert rel_l2(f12, f1 + f2) < 1e-14
+    # every element-local copy of a shared node holds the assembled value
+    s = o12.get(oracle.SX)
+    u = local_to_unique(s, g)
+    from helpers import unique_to_local
+    assert np.array_equal(unique_to_local(u, g), s)
+    # the contributions really overlap on the shared face
+    assert np.count_nonzero(f1 * f2) > 0
+
+
+def test_taper_closed_form():
+    g = small_grid(nx=8, ny=8, nz=6, lpd=4, taper_width=2)
+    o = _oracle(g)
+    t = o.get(oracle.TAPER)
+    alpha = g.taper_slope
+    a01 = float(np.float32(0.1))
+    # interior nodes are untouched
+    assert t[:, :, :, 0, 3, 3].min() == 1.0
+    # outermost theta face, away from other strips: prof_x = alpha there,
+    # corner normalisation 2 alpha / (1 + 1) = alpha
+    assert abs(t[2, 2, 0, 0, 3, 0] - math.exp(-a01 * alpha ** 2)) < 1e-15
+    # outermost theta-phi edge at the free surface: prof = 2 alpha,
+    # each profile 2 alpha / 3, sum 4 alpha / 3
+    assert abs(t[0, 0, 0, 0, 0, 0] - math.exp(-a01 * (4 * alpha / 3) ** 2)) < 1e-15
+    # the surface (k = 0 of the top element layer) has no r strip
+    assert t[0, 2, 2, 0, 4, 4] == 1.0
+
+
+def test_courant_number_of_the_homogeneous_case_is_0_30():
+    # the reference prints "Courant number: 0.30" for dt = 0.5396 s on this mesh
+    case = synth.homogeneous()
+    g = case.grid
+    g2 = synth.Grid(**{**g.__dict__, "dt": 0.5396})
+    o, _ = oracle.build_case(synth.Case("h", g2, "homogeneous", 0, False, [], [], case.params))
+    assert "%5.2f" % o.cfl() == " 0.30"
+
+
+def test_zero_source_gives_zero_field():
+    case = synth.homogeneous(nt=60, n_rec=4)
+    case.sources = []
+    o, _ = oracle.build_case(case)
+    o.step(60)
+    assert np.all(o.vrange() == 0.0)
+
+
+def test_linearity_in_the_moment_tensor():
+    case = synth.homogeneous(nt=120, n_rec=4)
+    m = np.array(case.sources[0].moment_tensor)
+    traces = []
+    for mt in (m, 2.5 * m):
+        case.sources[0].moment_tensor = tuple(mt)
+        o, _ = oracle.build_case(case)
+        o.step(120)
+        traces.append(np.array([o.trace(r, c) for r in range(4) for c in range(3)]))
+    assert rel_l2(traces[1], 2.5 * traces[0]) < 1e-13
+
+
+def test_explosion_matches_the_full_space_solution():
+    """Isotropic source M(t) = M0 Ricker(t) in a homogeneous medium: before any
+    boundary reflection the radial velocity is
+        v_R = (Mdot(t - R/a) / R^2 + Mddot(t - R/a) / (a R)) / (4 pi rho a^2)
+    (the gradient of the potential -M(t - R/a) / (4 pi rho a^2 R)). The source
+    sits inside an element; the fit allows a time shift of a few steps (the
+    source acts one step after it is sampled)."""
+    nt, width = 300, 30.0
+    case = synth.homogeneous(nt=nt, n_rec=0, nx=24, ny=24, nz=16)
+    M0, rho, a = 1e18, 2700.0, 6000.0
+    slat, slon, sdep = 0.37, 0.41, 240e3
+    case.sources = [synth.Source(slat, slon, sdep, (M0, 0, 0, M0, 0, M0), "RICKER", 1.5 * width, width)]
+    dists = (1.2, 2.4)
+    case.receivers = [synth.Receiver(slat - d, slon, sdep, "R", ("vx", "vy", "vz")) for d in dists]
+    o, recs = oracle.build_case(case)
+    assert len(recs) == 2
+    o.step(nt)
+    dt = case.grid.dt
+    t = np.arange(1, nt + 1) * dt
+    rs = synth.EARTH_RADIUS - sdep
+    hf = 0.01
+    tf = np.arange(0.0, nt * dt + 60.0, hf)
+    m = oracle.stf("RICKER", 1.5 * width, width, hf, len(tf))
+    md = np.gradient(m, hf)
+    mdd = np.gradient(md, hf)
+    for i, (d, tol) in enumerate(zip(dists, (0.06, 0.02))):
+        R = 2 * rs * math.sin(math.radians(d) / 2)
+        num = o.trace(i, 0)            # v_theta points away from the source
+        win = t < 3.0 * width + R / a
+        best = (np.inf, 0.0)
+        for sh in np.arange(-4.0, 4.01, 0.05):
+            tt = t - R / a - sh * dt
+            ana = (np.interp(tt, tf, md) / R ** 2 + np.interp(tt, tf, mdd) / (a * R)) * M0 / (4 * math.pi * rho * a * a)
+            e = np.linalg.norm((num - ana)[win]) / np.linalg.norm(ana[win])
+            best = min(best, (e, sh))
+        assert best[0] < tol, (d, best)
+        assert abs(best[1]) < 3.0, (d, best)
+        # no transverse motion in the vertical plane through the source
+        assert np.abs(o.trace(i, 1)).max() < 0.05 * np.abs(num).max()
+
+
+def test_visco_with_vanishing_tau_reduces_to_elastic():
+    case = synth.homogeneous(nt=80, n_rec=3)
+    o_el, _ = oracle.build_case(case)
+    o_el.step(80)
+    case_v = synth.homogeneous(nt=80, n_rec=3, visco=True)
+    case_v.params["q"] = 1e30            # tau = 6.0814 Q^-1.015 ~ 1e-30
+    o_v, _ = oracle.build_case(case_v)
+    assert o_v.visco and not o_el.visco
+    o_v.step(80)
+    a = np.array([o_el.trace(r, c) for r in range(3) for c in range(3)])
+    b = np.array([o_v.trace(r, c) for r in range(3) for c in range(3)])
+    assert rel_l2(b, a) < 1e-12
+
+
+def test_memory_variables_follow_the_geometric_series():
+    """Under a constant strain rate e, M_k(n) = -(tau/3) e (1 - (1 - dt/tau_k)^n)
+    with tau = 6.0814 Q^-1.015 and tau_k = 1000, 70, 5.25 s."""
+    g = small_grid(nx=2, ny=2, nz=2, lpd=3, taper_width=0, dt=0.5)
+    Q = 100.0
+    o = _oracle(g, q=Q)
+    th, ph, r = node_coords(o)
+    c = 1e-9
+    tau = float(np.float32(6.0814)) * Q ** float(np.float32(-1.015))
+    n = 25
+    for _ in range(n):
+        _set_v(o, np.zeros_like(r), np.zeros_like(r), c * r)   # e = c I exactly
+        o.evolution()
+    for k, tk in enumerate((1000.0, 70.0, 5.25)):
+        expect = -(tau / 3.0) * c * (1.0 - (1.0 - g.dt / tk) ** n)
+        for comp, diag in ((0, True), (1, False), (3, True), (5, True)):
+            m = o.get(oracle.MXX + 6 * k + comp)
+            if diag:
+                assert np.max(np.abs(m - expect)) < 1e-9 * abs(expect), (k, comp)
+            else:
+                assert np.max(np.abs(m)) < 1e-9 * abs(expect)
