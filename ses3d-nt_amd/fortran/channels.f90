@@ -1,0 +1,197 @@
+!> Sources and receivers with the reference's class interface
+!! (source_cls%add, receiver_cls%record / %write;
+!! src/config/source.f90:359-370, src/config/receiver.f90:136-192).
+!!
+!! On the GPU the moment-tensor injection and the receiver sampling run inside
+!! the time step itself (s3d_step), so add() and record() only check that the
+!! device has advanced to step it; write() copies the recorded traces back and
+!! writes SAC files.
+MODULE channels_mod
+    USE, INTRINSIC :: ISO_C_BINDING
+    USE ses3d_hip_c
+    USE configuration_mod, ONLY : rk, configuration, source_entry, receiver_entry, abort
+    USE sac_io_mod, ONLY : write_sac, sac_file_name
+    IMPLICIT NONE
+    PRIVATE
+
+    TYPE, PUBLIC :: source_cls
+        INTEGER :: ex = -1, ey = -1, ez = -1
+    CONTAINS
+        PROCEDURE :: add => source_add
+    END TYPE
+
+    TYPE, PUBLIC :: receiver_cls
+        INTEGER :: id = -1
+        TYPE(receiver_entry) :: info
+        INTEGER :: nch = 0
+        INTEGER(c_int) :: codes(8)
+        CHARACTER(LEN=8) :: names(8)
+    CONTAINS
+        PROCEDURE :: record => receiver_record
+        PROCEDURE :: write => receiver_write
+    END TYPE
+
+    PUBLIC :: parse_source_file, parse_receiver_file
+
+CONTAINS
+
+    INTEGER(c_int) FUNCTION wavelet_code(name) RESULT(code)
+        CHARACTER(LEN=*), INTENT(IN) :: name
+        SELECT CASE (TRIM(name))
+        CASE ('RICKER'); code = S3D_RICKER
+        CASE ('GAUSS');  code = S3D_GAUSS
+        CASE ('SIN**3'); code = S3D_SIN3
+        CASE ('BRUNE');  code = S3D_BRUNE
+        CASE ('DELTA');  code = S3D_DELTA
+        CASE DEFAULT;    code = -1
+        END SELECT
+    END FUNCTION
+
+    !> Read an ASCII source-time function, one value per line
+    !! (reference read_stf, src/config/source.f90:662-687).
+    SUBROUTINE read_stf(file, out)
+        CHARACTER(LEN=*), INTENT(IN) :: file
+        REAL(rk), INTENT(OUT) :: out(:)
+        INTEGER :: lun, ios, i
+        OPEN(NEWUNIT=lun, FILE=file, STATUS='old', ACTION='read', IOSTAT=ios)
+        IF (ios /= 0) CALL abort('Source-time function file not found: ' // TRIM(file))
+        out = 0.0_rk
+        DO i = 1, SIZE(out)
+            READ(lun, *, IOSTAT=ios) out(i)
+            IF (ios /= 0) EXIT
+        END DO
+        CLOSE(lun)
+    END SUBROUTINE
+
+    !> Locate every source, build its signature and hand it to the device
+    !! (src/config/source.f90:274-327, 449-536).
+    SUBROUTINE parse_source_file(ctx, sources)
+        TYPE(c_ptr), INTENT(IN) :: ctx
+        TYPE(source_cls), ALLOCATABLE, INTENT(OUT) :: sources(:)
+        INTEGER :: i, c, nt, n
+        INTEGER(c_int) :: exyz(3), code
+        REAL(c_double), ALLOCATABLE :: vm(:), stf(:), comp(:)
+        TYPE(source_entry) :: s
+        TYPE(source_cls), ALLOCATABLE :: tmp(:)
+
+        nt = configuration%nt
+        ALLOCATE(vm((configuration%lpd + 1)**3), stf(nt), tmp(SIZE(configuration%sources)))
+        n = 0
+        DO i = 1, SIZE(configuration%sources)
+            s = configuration%sources(i)
+            IF (s3d_locate(ctx, s%lat, s%lon, s%depth, exyz, vm) /= 0) THEN
+                WRITE(*, '("WARNING: source (", G0, ", ", G0, ", ", G0, ") not in computational domain!")') &
+                    s%lat, s%lon, s%depth
+                CYCLE
+            END IF
+            code = wavelet_code(s%wavelet)
+            IF (code >= 0) THEN
+                CALL check(s3d_wavelet(code, s%onset, s%width, configuration%dt, INT(nt, c_int), stf), &
+                           'wavelet')
+            ELSE
+                CALL read_stf(s%wavelet, stf)
+            END IF
+            IF (s%moment) THEN
+                ALLOCATE(comp(6 * nt))
+                DO c = 1, 6
+                    comp((c - 1) * nt + 1:c * nt) = stf * s%mt(c)
+                END DO
+                CALL check(s3d_add_source_mt(ctx, exyz, vm, comp), 'add_source_mt')
+            ELSE
+                ALLOCATE(comp(3 * nt))
+                DO c = 1, 3
+                    comp((c - 1) * nt + 1:c * nt) = stf * s%dir(c)
+                END DO
+                CALL check(s3d_add_source_sf(ctx, exyz, vm, comp), 'add_source_sf')
+            END IF
+            DEALLOCATE(comp)
+            n = n + 1
+            tmp(n) = source_cls(exyz(1), exyz(2), exyz(3))
+        END DO
+        sources = tmp(1:n)
+    END SUBROUTINE
+
+    !> Locate every receiver and register its channels on the device
+    !! (src/config/receiver.f90:435-492).
+    SUBROUTINE parse_receiver_file(ctx, receivers)
+        TYPE(c_ptr), INTENT(IN) :: ctx
+        TYPE(receiver_cls), ALLOCATABLE, INTENT(OUT) :: receivers(:)
+        INTEGER :: i, j, n
+        INTEGER(c_int) :: exyz(3), id
+        REAL(c_double), ALLOCATABLE :: vm(:)
+        TYPE(receiver_cls), ALLOCATABLE :: tmp(:)
+        TYPE(receiver_cls) :: r
+
+        ALLOCATE(vm((configuration%lpd + 1)**3), tmp(SIZE(configuration%receivers)))
+        n = 0
+        DO i = 1, SIZE(configuration%receivers)
+            r%info = configuration%receivers(i)
+            r%nch = 0
+            DO j = 1, SIZE(r%info%attributes)
+                IF (LEN_TRIM(r%info%attributes(j)) == 0) CYCLE
+                IF (r%nch == 8) CALL abort('At most 8 channels per receiver')
+                r%nch = r%nch + 1
+                r%names(r%nch) = r%info%attributes(j)
+                SELECT CASE (TRIM(r%info%attributes(j)))
+                CASE ('vx'); r%codes(r%nch) = S3D_CH_VX
+                CASE ('vy'); r%codes(r%nch) = S3D_CH_VY
+                CASE ('vz'); r%codes(r%nch) = S3D_CH_VZ
+                CASE ('n');  r%codes(r%nch) = S3D_CH_N
+                CASE ('e');  r%codes(r%nch) = S3D_CH_E
+                CASE ('z');  r%codes(r%nch) = S3D_CH_Z
+                CASE ('pressure'); r%codes(r%nch) = S3D_CH_PRESSURE
+                CASE DEFAULT
+                    CALL abort('Receiver channel not supported on the GPU path: ' // &
+                               TRIM(r%info%attributes(j)))
+                END SELECT
+            END DO
+            IF (r%nch == 0) CYCLE
+            IF (s3d_locate(ctx, r%info%lat, r%info%lon, r%info%depth, exyz, vm) /= 0) THEN
+                WRITE(*, '("WARNING: receiver ", A, " not in computational domain!")') &
+                    TRIM(r%info%station)
+                CYCLE
+            END IF
+            CALL check(s3d_add_receiver(ctx, exyz, vm, INT(r%nch, c_int), r%codes, id), &
+                       'add_receiver')
+            r%id = id
+            n = n + 1
+            tmp(n) = r
+        END DO
+        receivers = tmp(1:n)
+    END SUBROUTINE
+
+    SUBROUTINE source_add(self, it)
+        CLASS(source_cls), INTENT(IN) :: self
+        INTEGER, INTENT(IN) :: it
+        ! injected inside the device step (elem_kernel); nothing to do here
+        IF (it < 1 .OR. self%ex < -1) RETURN
+    END SUBROUTINE
+
+    SUBROUTINE receiver_record(self, it)
+        CLASS(receiver_cls), INTENT(IN) :: self
+        INTEGER, INTENT(IN) :: it
+        ! sampled inside the device step (record_kernel); nothing to do here
+        IF (it < 1 .OR. self%id < -1) RETURN
+    END SUBROUTINE
+
+    !> Write one SAC file per channel (src/config/receiver.f90:154-192).
+    SUBROUTINE receiver_write(self, ctx)
+        CLASS(receiver_cls), INTENT(IN) :: self
+        TYPE(c_ptr), INTENT(IN) :: ctx
+        REAL(c_double), ALLOCATABLE :: tr(:)
+        INTEGER :: q, ios
+        CHARACTER(LEN=:), ALLOCATABLE :: fn
+        ALLOCATE(tr(configuration%nt))
+        DO q = 1, self%nch
+            CALL check(s3d_get_trace(ctx, INT(self%id, c_int), INT(q - 1, c_int), tr), 'get_trace')
+            fn = TRIM(self%info%prefix) // sac_file_name(self%info%network, self%info%location, &
+                                                         self%info%station, self%names(q))
+            CALL write_sac(fn, tr, configuration%dt, self%info%lat, self%info%lon, &
+                           self%info%depth, self%info%network, self%info%station, &
+                           self%info%location, self%names(q), configuration%event_name, &
+                           configuration%date_time, self%info%override, ios)
+            IF (ios /= 0) WRITE(*, '("WARNING: could not write ", A)') fn
+        END DO
+    END SUBROUTINE
+
+END MODULE channels_mod

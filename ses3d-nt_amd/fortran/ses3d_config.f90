@@ -1,0 +1,296 @@
+!> Configuration file reader: the reference's NAMELIST format
+!! (src/parser/parser.f90:148-638) with its consistency checks
+!! (src/config/config.F90:97-232), for one GPU holding the whole mesh.
+!!
+!! Groups: &grid, &model, &time, &general (exactly one each), &source_mt,
+!! &source_sf, &receiver (any number, read in file order), and the optional
+!! &gpu group of this build: precision ('fp32' | 'fp64'), kahan_v,
+!! kahan_sigma, device.
+MODULE configuration_mod
+    USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY : ERROR_UNIT, IOSTAT_END
+    USE, INTRINSIC :: ISO_C_BINDING, ONLY : c_double, c_int
+    IMPLICIT NONE
+
+    PRIVATE
+    INTEGER, PARAMETER, PUBLIC :: rk = c_double
+    INTEGER, PARAMETER, PUBLIC :: fnl = 512
+    INTEGER, PARAMETER, PUBLIC :: max_attributes = 30
+    REAL(rk), PARAMETER :: default_real = -HUGE(1.0_rk)
+
+    TYPE, PUBLIC :: source_entry
+        LOGICAL :: moment = .TRUE.
+        REAL(rk) :: lat, lon, depth, onset, width
+        REAL(rk) :: mt(6) = 0.0_rk
+        REAL(rk) :: dir(3) = 0.0_rk
+        CHARACTER(LEN=fnl) :: wavelet
+    END TYPE
+
+    TYPE, PUBLIC :: receiver_entry
+        REAL(rk) :: lat, lon, depth
+        CHARACTER(LEN=8) :: network, station, location
+        CHARACTER(LEN=8) :: attributes(max_attributes)
+        LOGICAL :: override
+        CHARACTER(LEN=fnl) :: prefix
+    END TYPE
+
+    TYPE, PUBLIC :: configuration_typ
+        ! &grid
+        INTEGER :: nx, ny, nz, lpd, taper_width
+        REAL(rk) :: taper_slope
+        ! &model
+        REAL(rk) :: lat_min, lat_max, lon_min, lon_max, rad_min, rad_max
+        INTEGER :: model_type
+        CHARACTER(LEN=fnl) :: file(7)   ! rhoinv, lambda, mu, a, b, c, q
+        ! &time
+        REAL(rk) :: dt
+        INTEGER :: nt, date_time(8)
+        ! &general
+        CHARACTER(LEN=16) :: workflow, event_name
+        CHARACTER(LEN=fnl) :: log_file_dir
+        ! &gpu
+        CHARACTER(LEN=8) :: precision
+        LOGICAL :: kahan_v, kahan_sigma
+        INTEGER :: device
+        TYPE(source_entry), ALLOCATABLE :: sources(:)
+        TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
+    END TYPE
+
+    TYPE(configuration_typ), PUBLIC :: configuration
+
+    PUBLIC :: parse_conf_file, abort
+
+CONTAINS
+
+    SUBROUTINE abort(msg)
+        CHARACTER(LEN=*), INTENT(IN) :: msg
+        WRITE(ERROR_UNIT, '("ERROR: ", A)') msg
+        ERROR STOP 1
+    END SUBROUTINE
+
+    INTEGER FUNCTION open_conf(file) RESULT(lun)
+        CHARACTER(LEN=*), INTENT(IN) :: file
+        INTEGER :: ios
+        OPEN(NEWUNIT=lun, FILE=file, STATUS='old', ACTION='read', IOSTAT=ios)
+        IF (ios /= 0) CALL abort('Config-File not found: ' // TRIM(file))
+    END FUNCTION
+
+    SUBROUTINE parse_conf_file(file)
+        CHARACTER(LEN=*), INTENT(IN) :: file
+        ! &grid
+        INTEGER :: nx, ny, nz, lpd, taper_width
+        REAL(rk) :: taper_slope
+        NAMELIST /grid/ nx, ny, nz, lpd, taper_width, taper_slope
+        ! &model
+        REAL(rk) :: lat_max, lat_min, lon_max, lon_min, rad_max, rad_min
+        INTEGER :: model_type
+        CHARACTER(LEN=fnl) :: rhoinv, lambda, mu, a, b, c, q
+        LOGICAL :: override
+        NAMELIST /model/ lat_max, lat_min, lon_max, lon_min, rad_max, rad_min, &
+            model_type, rhoinv, lambda, mu, a, b, c, q, override
+        ! &time
+        INTEGER :: nt, date_time(8)
+        REAL(rk) :: dt
+        NAMELIST /time/ dt, nt, date_time
+        ! &general
+        CHARACTER(LEN=16) :: event_name, workflow
+        CHARACTER(LEN=fnl) :: log_file_dir
+        NAMELIST /general/ log_file_dir, event_name, workflow
+        ! &gpu
+        CHARACTER(LEN=8) :: precision
+        LOGICAL :: kahan_v, kahan_sigma
+        INTEGER :: device
+        NAMELIST /gpu/ precision, kahan_v, kahan_sigma, device
+        INTEGER :: lun, ios, n
+        TYPE(configuration_typ) :: cf
+
+        lun = open_conf(file)
+        ! ---- &grid (src/config/config.F90:111-128) ----
+        nx = -1; ny = -1; nz = -1; lpd = -1; taper_width = -1; taper_slope = -1.0_rk
+        n = count_groups(lun, 'grid')
+        IF (n /= 1) CALL abort('Found ' // i2c(n) // ' &grid-groups in config-file!')
+        READ(lun, NML=grid, IOSTAT=ios)
+        IF (ios /= 0) CALL abort('Error while parsing &grid-group!')
+        IF (nx < 4) CALL abort('Number of x-elements must be greater than 3!')
+        IF (ny < 4) CALL abort('Number of y-elements must be greater than 3!')
+        IF (nz < 4) CALL abort('Number of z-elements must be greater than 3!')
+        IF (lpd < 2 .OR. lpd > 7) CALL abort('Lagrange polynomial degree must be between 2 and 7!')
+        IF (ANY(2 * taper_width + 1 > [nx, ny, nz])) CALL abort('Relaxing boundaries are too wide!')
+        cf%nx = nx; cf%ny = ny; cf%nz = nz; cf%lpd = lpd
+        cf%taper_width = taper_width; cf%taper_slope = taper_slope
+
+        ! ---- &general ----
+        event_name = 'N/A'; workflow = ''; log_file_dir = 'N/A'
+        n = count_groups(lun, 'general')
+        IF (n /= 1) CALL abort('Found ' // i2c(n) // ' &general-groups in config-file!')
+        READ(lun, NML=general, IOSTAT=ios)
+        IF (ios /= 0) CALL abort('Error while parsing &general-group!')
+        IF (TRIM(workflow) /= 'forward') CALL abort('Unknown workflow ' // TRIM(workflow))
+        cf%workflow = workflow; cf%event_name = event_name; cf%log_file_dir = log_file_dir
+
+        ! ---- &model (src/config/config.F90:160-189) ----
+        lat_max = 0.0_rk; lat_min = 0.0_rk; lon_max = 0.0_rk; lon_min = 0.0_rk
+        rad_max = 0.0_rk; rad_min = 0.0_rk; model_type = -1
+        rhoinv = 'N/A'; lambda = 'N/A'; mu = 'N/A'; a = 'N/A'; b = 'N/A'; c = 'N/A'; q = 'N/A'
+        override = .FALSE.
+        n = count_groups(lun, 'model')
+        IF (n /= 1) CALL abort('Found ' // i2c(n) // ' &model-groups in config-file!')
+        READ(lun, NML=model, IOSTAT=ios)
+        IF (ios /= 0) CALL abort('Error while parsing &model-group!')
+        IF (lat_min > lat_max) CALL abort('lat_max < lat_min!')
+        IF (lon_min > lon_max) CALL abort('lon_max < lon_min!')
+        IF (rad_min > rad_max) CALL abort('rad_max < rad_min!')
+        IF (ANY([lat_min, lat_max] < -90.0_rk) .OR. ANY([lat_min, lat_max] > 90.0_rk)) &
+            CALL abort('Invalid latitude')
+        IF (ANY([lon_min, lon_max] < -180.0_rk) .OR. ANY([lon_min, lon_max] > 360.0_rk)) &
+            CALL abort('Invalid longitude')
+        IF (ANY([rad_min, rad_max] < 1.0_rk)) CALL abort('Invalid Radius')
+        IF (model_type == -1) THEN
+            IF (rhoinv == 'N/A') CALL abort('"rhoinv" model-file must be given!')
+            IF (lambda == 'N/A') CALL abort('"lambda" model-file must be given!')
+            IF (mu == 'N/A') CALL abort('"mu" model-file must be given!')
+        ELSE IF (model_type /= 1) THEN
+            CALL abort('Only model files and model_type 1 (homogeneous) are supported')
+        END IF
+        cf%lat_min = lat_min; cf%lat_max = lat_max; cf%lon_min = lon_min; cf%lon_max = lon_max
+        cf%rad_min = rad_min; cf%rad_max = rad_max; cf%model_type = model_type
+        cf%file = [CHARACTER(LEN=fnl) :: rhoinv, lambda, mu, a, b, c, q]
+
+        ! ---- &time (src/config/config.F90:192-210) ----
+        nt = -1; date_time = -1; dt = -1.0_rk
+        n = count_groups(lun, 'time')
+        IF (n /= 1) CALL abort('Found ' // i2c(n) // ' &time-groups in config-file!')
+        READ(lun, NML=time, IOSTAT=ios)
+        IF (ios /= 0) CALL abort('Error while parsing &time-group!')
+        IF (2 > nt .OR. nt > 50000) CALL abort('Non-conforming nt value in &time-group!')
+        IF (dt <= 0.0_rk) CALL abort('Non-conforming dt value in &time-group!')
+        IF (ALL(date_time == -1)) CALL DATE_AND_TIME(VALUES=date_time)
+        cf%nt = nt; cf%dt = dt; cf%date_time = date_time
+
+        ! ---- &gpu (optional) ----
+        precision = 'fp32'; kahan_v = .TRUE.; kahan_sigma = .FALSE.; device = 0
+        n = count_groups(lun, 'gpu')
+        IF (n > 1) CALL abort('Found ' // i2c(n) // ' &gpu-groups in config-file!')
+        IF (n == 1) THEN
+            READ(lun, NML=gpu, IOSTAT=ios)
+            IF (ios /= 0) CALL abort('Error while parsing &gpu-group!')
+        END IF
+        IF (precision /= 'fp32' .AND. precision /= 'fp64') &
+            CALL abort("precision must be 'fp32' or 'fp64'")
+        cf%precision = precision; cf%kahan_v = kahan_v; cf%kahan_sigma = kahan_sigma
+        cf%device = device
+
+        CALL parse_sources(lun, cf)
+        CALL parse_receivers(lun, cf)
+        CLOSE(lun)
+        configuration = cf
+    END SUBROUTINE
+
+    !> &source_mt and &source_sf groups (src/parser/parser.f90:265-365)
+    SUBROUTINE parse_sources(lun, cf)
+        INTEGER, INTENT(IN) :: lun
+        TYPE(configuration_typ), INTENT(INOUT) :: cf
+        REAL(rk) :: lat, lon, depth, onset, width, moment_tensor(6), direction(3)
+        CHARACTER(LEN=fnl) :: wavelet
+        NAMELIST /source_mt/ lat, lon, depth, moment_tensor, wavelet, onset, width
+        NAMELIST /source_sf/ lat, lon, depth, direction, wavelet, onset, width
+        INTEGER :: nmt, nsf, i, ios
+        TYPE(source_entry) :: s
+
+        nmt = count_groups(lun, 'source_mt')
+        nsf = count_groups(lun, 'source_sf')
+        ALLOCATE(cf%sources(nmt + nsf))
+        REWIND(lun)
+        DO i = 1, nmt
+            lat = default_real; lon = default_real; depth = default_real
+            onset = default_real; width = default_real
+            moment_tensor = default_real; wavelet = ''
+            READ(lun, NML=source_mt, IOSTAT=ios)
+            IF (ios /= 0) CALL abort('Error while parsing &source_mt-group!')
+            IF (ANY(moment_tensor == default_real)) CALL abort('Moment-tensor of dipol force invalid!')
+            s%moment = .TRUE.
+            s%lat = lat; s%lon = lon; s%depth = depth; s%onset = onset; s%width = width
+            s%mt = moment_tensor; s%wavelet = wavelet
+            cf%sources(i) = s
+        END DO
+        REWIND(lun)
+        DO i = 1, nsf
+            lat = default_real; lon = default_real; depth = default_real
+            onset = default_real; width = default_real
+            direction = default_real; wavelet = ''
+            READ(lun, NML=source_sf, IOSTAT=ios)
+            IF (ios /= 0) CALL abort('Error while parsing &source_sf-group!')
+            IF (ANY(direction == default_real)) CALL abort('Direction of single force invalid!')
+            s%moment = .FALSE.
+            s%lat = lat; s%lon = lon; s%depth = depth; s%onset = onset; s%width = width
+            s%dir = direction; s%wavelet = wavelet
+            cf%sources(nmt + i) = s
+        END DO
+    END SUBROUTINE
+
+    !> &receiver groups (src/parser/parser.f90:368-420)
+    SUBROUTINE parse_receivers(lun, cf)
+        INTEGER, INTENT(IN) :: lun
+        TYPE(configuration_typ), INTENT(INOUT) :: cf
+        REAL(rk) :: lat, lon, depth
+        CHARACTER(LEN=8) :: network, station, location, attributes(max_attributes)
+        LOGICAL :: override
+        CHARACTER(LEN=fnl) :: prefix
+        NAMELIST /receiver/ lat, lon, depth, network, station, location, &
+                            attributes, override, prefix
+        INTEGER :: n, i, ios
+
+        n = count_groups(lun, 'receiver')
+        ALLOCATE(cf%receivers(n))
+        REWIND(lun)
+        DO i = 1, n
+            lat = default_real; lon = default_real; depth = default_real
+            network = 'Ses3d-NT'; station = ''; location = ''; attributes = ''
+            override = .FALSE.; prefix = './'
+            READ(lun, NML=receiver, IOSTAT=ios)
+            IF (ios /= 0) CALL abort('Error while parsing &receiver-group!')
+            cf%receivers(i) = receiver_entry(lat, lon, depth, network, station, &
+                                             location, attributes, override, prefix)
+        END DO
+    END SUBROUTINE
+
+    !> Number of occurrences of &name in the file; leaves the file rewound.
+    INTEGER FUNCTION count_groups(lun, name) RESULT(n)
+        INTEGER, INTENT(IN) :: lun
+        CHARACTER(LEN=*), INTENT(IN) :: name
+        CHARACTER(LEN=1024) :: line
+        CHARACTER(LEN=64) :: tok
+        INTEGER :: ios, p
+        n = 0
+        REWIND(lun)
+        DO
+            READ(lun, '(A)', IOSTAT=ios) line
+            IF (ios /= 0) EXIT
+            line = ADJUSTL(line)
+            IF (line(1:1) /= '&') CYCLE
+            tok = line(2:)
+            p = SCAN(tok, ' ,/!')
+            IF (p > 0) tok = tok(1:p-1)
+            IF (lower(TRIM(tok)) == name) n = n + 1
+        END DO
+        REWIND(lun)
+    END FUNCTION
+
+    FUNCTION lower(s) RESULT(t)
+        CHARACTER(LEN=*), INTENT(IN) :: s
+        CHARACTER(LEN=LEN(s)) :: t
+        INTEGER :: i
+        t = s
+        DO i = 1, LEN(s)
+            IF (IACHAR(s(i:i)) >= 65 .AND. IACHAR(s(i:i)) <= 90) t(i:i) = ACHAR(IACHAR(s(i:i)) + 32)
+        END DO
+    END FUNCTION
+
+    FUNCTION i2c(i) RESULT(s)
+        INTEGER, INTENT(IN) :: i
+        CHARACTER(LEN=:), ALLOCATABLE :: s
+        CHARACTER(LEN=16) :: b
+        WRITE(b, '(I0)') i
+        s = TRIM(b)
+    END FUNCTION
+
+END MODULE configuration_mod

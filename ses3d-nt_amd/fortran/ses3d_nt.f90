@@ -1,0 +1,53 @@
+!> Forward workflow of Ses3d-NT on one MI355X (reference main program,
+!! src/ses3d-nt.F90:30-359, workflow 'forward').
+!!
+!!   ses3d_nt <config-file>
+!!
+!! Reads the reference's NAMELIST config, uploads the model, runs nt steps on
+!! the GPU and writes one SAC file per receiver channel.
+PROGRAM ses3d_nt
+    USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY : OUTPUT_UNIT
+    USE configuration_mod, ONLY : configuration, parse_conf_file, abort, rk
+    USE evolution_mod, ONLY : ctx, init_temp, ses3d_evolution, v_ptp_loc, finalize_evolution
+    USE channels_mod, ONLY : source_cls, receiver_cls, parse_source_file, parse_receiver_file
+    IMPLICIT NONE
+    CHARACTER(LEN=512) :: conf_file_name
+    TYPE(source_cls), ALLOCATABLE :: sources(:)
+    TYPE(receiver_cls), ALLOCATABLE :: receivers(:)
+    INTEGER :: it, i, every
+    REAL(rk) :: ptp(2)
+    CHARACTER(LEN=*), PARAMETER :: fmt = '("Iteration ",I0,", ",I0,"%: ",G0," < v < ",G0)'
+
+    IF (COMMAND_ARGUMENT_COUNT() < 1) CALL abort('Config-file missing!')
+    CALL GET_COMMAND_ARGUMENT(1, conf_file_name)
+    CALL parse_conf_file(TRIM(conf_file_name))
+    CALL init_temp()
+    CALL parse_receiver_file(ctx, receivers)
+    WRITE(OUTPUT_UNIT, '("Found ", I0, " receivers.")') SIZE(receivers)
+    CALL parse_source_file(ctx, sources)
+    WRITE(OUTPUT_UNIT, '("Found ", I0, " sources.")') SIZE(sources)
+    WRITE(OUTPUT_UNIT, '(A, /, "Processing workflow ''", A, "'' of event ''", A, "''", /, A)') &
+        REPEAT('=', 50), TRIM(configuration%workflow), TRIM(configuration%event_name), REPEAT('=', 50)
+
+    every = MAX(1, configuration%nt / 10)
+    DO it = 1, configuration%nt
+        CALL ses3d_evolution()
+        DO i = 1, SIZE(sources)
+            CALL sources(i)%add(it)
+        END DO
+        DO i = 1, SIZE(receivers)
+            CALL receivers(i)%record(it)
+        END DO
+        IF (MOD(it, every) == 0 .OR. it == configuration%nt) THEN
+            ptp = v_ptp_loc()
+            WRITE(OUTPUT_UNIT, fmt) it, 100 * it / configuration%nt, ptp
+        END IF
+    END DO
+
+    DO i = 1, SIZE(receivers)
+        CALL receivers(i)%write(ctx)
+    END DO
+    CALL finalize_evolution()
+    WRITE(OUTPUT_UNIT, '(A, /, "Workflow ''forward'' successfully completed! ", /, A)') &
+        REPEAT('=', 50), REPEAT('=', 50)
+END PROGRAM ses3d_nt

@@ -70,3 +70,54 @@ def node_coords(o):
     """(theta, phi, r) at every element-local node of an initialised oracle."""
     import oracle
     return o.get(oracle.THETA), o.get(oracle.PHI), o.get(oracle.R)
+
+
+def write_config(case, d, precision="fp32", nt=None, extra=""):
+    """Write the reference's NAMELIST config and raw float64 model files for a
+    synth.Case into directory d; returns the config path."""
+    import os
+    g = case.grid
+    m = case.model_full()
+    files = {}
+    for k in ("rhoinv", "lambda", "mu", "q"):
+        if k in m:
+            p = os.path.join(d, k)
+            np.ascontiguousarray(m[k], dtype="<f8").tofile(p)
+            files[k] = p
+    lines = ["&grid", " nx = %d" % g.nx, " ny = %d" % g.ny, " nz = %d" % g.nz,
+             " lpd = %d" % g.lpd, " taper_width = %d" % g.taper_width,
+             " taper_slope = %r" % g.taper_slope, "/", "",
+             "&model", " lat_min = %r, lat_max = %r," % (g.lat_min, g.lat_max),
+             " lon_min = %r, lon_max = %r," % (g.lon_min, g.lon_max),
+             " rad_min = %r, rad_max = %r," % (g.rad_min, g.rad_max)]
+    for k, p in files.items():
+        lines.append(" %s = '%s'" % (k, p))
+    lines += ["/", "", "&time", " dt = %r" % float(g.dt), " nt = %d" % (nt or g.nt),
+              " date_time = 2009, 8, 20, 0, 6, 35, 0, 0", "/", "",
+              "&general", " workflow = 'forward'", " event_name = 'synthetic'",
+              " log_file_dir = './'", "/", "",
+              "&gpu", " precision = '%s'" % precision, "/", ""]
+    for s in case.sources:
+        lines += ["&source_mt", " lat = %r" % s.lat, " lon = %r" % s.lon,
+                  " depth = %r" % s.depth, " wavelet = '%s'" % s.wavelet,
+                  " onset = %r" % s.onset, " width = %r" % s.width,
+                  " moment_tensor = " + " ".join("%r" % x for x in s.moment_tensor), "/", ""]
+    for r in case.receivers:
+        lines += ["&receiver", " lat = %r" % r.lat, " lon = %r" % r.lon,
+                  " depth = %r" % r.depth, " network = 'XX'", " station = '%s'" % r.station,
+                  " location = '00'",
+                  " attributes = " + " ".join("'%s'" % c for c in r.channels),
+                  " override = .TRUE.", " prefix = '%s/'" % d, "/", ""]
+    path = os.path.join(d, "config")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + extra)
+    return path
+
+
+def read_sac(path):
+    """(header floats, header ints, data) of a binary SAC file."""
+    raw = open(path, "rb").read()
+    f = np.frombuffer(raw[:280], dtype="<f4")
+    i = np.frombuffer(raw[280:420], dtype="<i4")
+    data = np.frombuffer(raw[632:], dtype="<f4")
+    return f, i, data
