@@ -310,3 +310,72 @@ def test_memory_variables_follow_the_geometric_series():
                 assert np.max(np.abs(m - expect)) < 1e-9 * abs(expect), (k, comp)
             else:
                 assert np.max(np.abs(m)) < 1e-9 * abs(expect)
+
+
+def test_pressure_channel_of_an_isotropic_stress():
+    """pressure = -(sxx + syy + szz) / 3 sampled at a receiver: an isotropic
+    stress c I gives -c at any point (interpolation of a constant is exact)."""
+    g = small_grid(nx=4, ny=4, nz=4, lpd=4, taper_width=0, nt=3)
+    o = _oracle(g)
+    c = 3.0e6
+    o.set(oracle.SXX, np.full(o.shape, c))
+    o.set(oracle.SYY, np.full(o.shape, c))
+    o.set(oracle.SZZ, np.full(o.shape, c))
+    loc = o.locate(0.3, 14.2, 100e3)
+    o.add_receiver(loc[0], loc[1], ["pressure"])
+    # one step with v = 0: the stress does not change (no taper)
+    o.step(1)
+    p = o.trace(0, 0)[0]
+    assert abs(p - (-c)) < 1e-7 * c      # the 1/3 is the reference's REAL32 literal
+
+
+def test_single_force_injects_total_force():
+    """A point force F(t) enters as -F(t) vm on the element's nodes; with zero
+    stress the divergence sums to zero, so sum_u M_u (v_u(n+1) - v_u(n)) / dt
+    equals -F at the sampled time, for any location inside an element."""
+    g = small_grid(nx=4, ny=4, nz=4, lpd=4, taper_width=0, nt=4)
+    o = _oracle(g)
+    loc = o.locate(0.77, 13.4, 77e3)
+    F = np.array([2.0e12, 3.0e12, 4.0e12, 5.0e12])
+    o.add_source_sf(loc[0], loc[1], np.stack([F, 0 * F, 0 * F]))
+    o.step(1)                               # add(1): src_x = -F(1) vm
+    mm = local_to_unique(o.get(oracle.MM), g)
+    v1 = local_to_unique(o.get(oracle.VX), g)
+    o.step(1)                               # evolution uses F(1)
+    v2 = local_to_unique(o.get(oracle.VX), g)
+    total = (mm * (v2 - v1)).sum() / g.dt
+    assert abs(total - (-F[0])) < 1e-9 * F[0]
+
+
+def test_kinetic_energy_of_a_uniform_flow():
+    """E = 1/2 rho |v|^2 V for a uniform tangential velocity field."""
+    o = _oracle(GRID, rho=2700.0)
+    _, _, r = node_coords(o)
+    o.set(oracle.VY, np.full(r.shape, 0.25))
+    vol = _section_integrals(GRID)["vol"]
+    assert abs(o.kinetic_energy() - 0.5 * 2700.0 * 0.0625 * vol) < 1e-11 * 2700.0 * vol
+
+
+def test_anisotropic_stress_of_a_dilatation():
+    """Pure dilatation e = c I through the anisotropic law of
+    src/evolution.f90:287-296: sxx = syy = (3 lambda + 2 mu + C + 2 A) c dt,
+    szz = (3 lambda + 2 mu + 2 C) c dt after one step from rest."""
+    g = GRID
+    o = oracle.Oracle(g)
+    lam, mu, A, C = 3.0e10, 2.5e10, 1.5e9, -2.0e9
+    o.set_model(oracle.RHOINV, np.full(o.shape, 1 / 3000.0))
+    o.set_model(oracle.LAMBDA, np.full(o.shape, lam))
+    o.set_model(oracle.MU, np.full(o.shape, mu))
+    o.set_model(oracle.A, np.full(o.shape, A))
+    o.set_model(oracle.C_, np.full(o.shape, C))
+    o.init()
+    _, _, r = node_coords(o)
+    c = 1e-12
+    _set_v(o, np.zeros_like(r), np.zeros_like(r), c * r)
+    o.evolution()
+    sxx, syy, szz = o.get(oracle.SXX), o.get(oracle.SYY), o.get(oracle.SZZ)
+    exp_h = (3 * lam + 2 * mu + C + 2 * A) * c * g.dt
+    exp_z = (3 * lam + 2 * mu + 2 * C) * c * g.dt
+    assert np.max(np.abs(sxx - exp_h)) < 1e-12 * abs(exp_h)
+    assert np.max(np.abs(syy - exp_h)) < 1e-12 * abs(exp_h)
+    assert np.max(np.abs(szz - exp_z)) < 1e-12 * abs(exp_z)
