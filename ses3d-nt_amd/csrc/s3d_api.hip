@@ -186,6 +186,12 @@ struct s3d_ctx {
     double *d_rec_vm = nullptr, *d_traces = nullptr;
     int nrec_dev = 0;
     int it = 0;
+    int *d_it = nullptr;       // device copy of it, advanced by advance_kernel
+    // captured chunk of steps (hipGraph)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
+    bool use_graph = true;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev[3];   // start/stop pairs per kernel class
@@ -221,14 +227,14 @@ inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); 
 
 // ---- template dispatch helpers ----
 template <int P, typename R>
-ElemArgs<R> elem_args(s3d_ctx *c, int it)
+ElemArgs<R> elem_args(s3d_ctx *c)
 {
     ElemArgs<R> a;
     std::memset(&a, 0, sizeof(a));
     const int P1 = P + 1;
     a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz; a.nel = (int)c->nel;
     a.remap = c->o.xcd_remap;
-    a.it = it;
+    a.it_dev = c->d_it;
     a.nt = c->g.nt;
     a.kahan_sigma = c->o.kahan_sigma && c->fp32;
     a.vx = (const R *)c->v[0]; a.vy = (const R *)c->v[1]; a.vz = (const R *)c->v[2];
@@ -297,11 +303,11 @@ void ev_begin(s3d_ctx *c, int k)
 void ev_end(s3d_ctx *c, int k) { ev_begin(c, k); }
 
 template <int P, typename R>
-int launch_step(s3d_ctx *c, int it)
+int launch_step(s3d_ctx *c)
 {
     using S = Shape<P>;
     const unsigned ge = nblocks(c->nel, S::EPB);
-    ElemArgs<R> ea = elem_args<P, R>(c, it);
+    ElemArgs<R> ea = elem_args<P, R>(c);
     ev_begin(c, 0);
     if (c->visco)
         hipLaunchKernelGGL((elem_kernel<P, R, true>), dim3(ge), dim3(S::NT), 0, c->stream, ea);
@@ -324,25 +330,64 @@ int launch_step(s3d_ctx *c, int it)
                            c->g.nx, c->g.ny, (int)c->nel, (const R *)c->v[0], (const R *)c->v[1],
                            (const R *)c->v[2], (const R *)c->lo[0], (const R *)c->lo[1],
                            (const R *)c->lo[2], (const R *)c->sig, c->d_rec_e, c->d_rec_vm,
-                           c->d_rec_nch, c->d_rec_ch, c->d_traces, c->g.nt, it);
+                           c->d_rec_nch, c->d_rec_ch, c->d_traces, c->g.nt, (const int *)c->d_it);
         ev_end(c, 2);
         HIPCHK(hipGetLastError());
     }
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(64), 0, c->stream, c->d_it);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 
 template <typename R>
-int dispatch_step(s3d_ctx *c, int it)
+int dispatch_step(s3d_ctx *c)
 {
     switch (c->p) {
-    case 2: return launch_step<2, R>(c, it);
-    case 3: return launch_step<3, R>(c, it);
-    case 4: return launch_step<4, R>(c, it);
-    case 5: return launch_step<5, R>(c, it);
-    case 6: return launch_step<6, R>(c, it);
-    case 7: return launch_step<7, R>(c, it);
+    case 2: return launch_step<2, R>(c);
+    case 3: return launch_step<3, R>(c);
+    case 4: return launch_step<4, R>(c);
+    case 5: return launch_step<5, R>(c);
+    case 6: return launch_step<6, R>(c);
+    case 7: return launch_step<7, R>(c);
     default: return fail("unsupported lpd");
     }
+}
+
+int launch_one(s3d_ctx *c) { return c->fp32 ? dispatch_step<float>(c) : dispatch_step<double>(c); }
+
+void drop_graph(s3d_ctx *c)
+{
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    c->graph_steps = 0;
+}
+
+// Capture `chunk` consecutive iterations into one graph. The kernels read the
+// iteration number from the device counter, so the same graph replays any
+// chunk of the run.
+int capture_graph(s3d_ctx *c, int chunk)
+{
+    drop_graph(c);
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int s = 0; s < chunk && rc == 0; ++s) rc = launch_one(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(std::string("graph capture: ") + hipGetErrorString(e));
+    c->graph = g;
+    e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph(c);
+        return fail(std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    c->graph_steps = chunk;
+    return 0;
 }
 
 // Generic P-dispatch for setup / IO kernels.
@@ -466,9 +511,12 @@ struct KineticRun {
     }
 };
 
+void drop_graph(s3d_ctx *c);
+
 int upload_sources(s3d_ctx *c)
 {
     if (!c->src_dirty) return 0;
+    drop_graph(c);
     dfree(c->d_slot); dfree(c->d_chan); dfree(c->d_coef); dfree(c->d_stf);
     c->d_slot = nullptr; c->d_chan = nullptr; c->d_coef = nullptr; c->d_stf = nullptr;
     if (!c->slots.empty()) {
@@ -495,6 +543,7 @@ int upload_sources(s3d_ctx *c)
 int upload_receivers(s3d_ctx *c)
 {
     if (!c->rec_dirty) return 0;
+    drop_graph(c);
     const int n = (int)c->rec_e.size();
     const size_t tr = (size_t)n * 8 * c->g.nt;
     double *traces = nullptr;
@@ -621,6 +670,8 @@ int s3d_destroy(s3d_ctx *c)
     for (void *b : bufs) dfree(b);
     for (int k = 0; k < 3; ++k)
         for (hipEvent_t e : c->ev[k]) (void)hipEventDestroy(e);
+    drop_graph(c);
+    dfree(c->d_it);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -836,6 +887,8 @@ int s3d_init(s3d_ctx *c)
     HIPCHK(hipStreamSynchronize(c->stream));
     (void)hipFree(d_sin); (void)hipFree(d_r); (void)hipFree(d_w); (void)hipFree(d_th); (void)hipFree(d_ph);
     for (int q = 0; q < 7; ++q) { dfree(c->stage[q]); c->stage[q] = nullptr; }
+    if (dalloc(c, (void **)&c->d_it, sizeof(int))) return 1;
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->slot_of.assign(c->nel, -1);
     c->inited = true;
     c->it = 0;
@@ -968,8 +1021,23 @@ int s3d_step(s3d_ctx *c, int nsteps)
     if (c->it + nsteps > c->g.nt) return fail("stepping past nt");
     HIPCHK(hipSetDevice(c->dev));
     if (upload_sources(c) || upload_receivers(c)) return 1;
-    for (int s = 0; s < nsteps; ++s) {
-        int rc = c->fp32 ? dispatch_step<float>(c, c->it + 1) : dispatch_step<double>(c, c->it + 1);
+    // Launch-bound meshes replay a captured chunk of GRAPH_CHUNK iterations;
+    // the rest (and every step while kernel timing is on) launches directly.
+    constexpr int GRAPH_CHUNK = 16;
+    int left = nsteps;
+    if (c->use_graph && !c->timing && left >= GRAPH_CHUNK) {
+        if (c->graph_steps != GRAPH_CHUNK) {
+            int rc = capture_graph(c, GRAPH_CHUNK);
+            if (rc) return rc;
+        }
+        while (left >= GRAPH_CHUNK) {
+            HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+            c->it += GRAPH_CHUNK;
+            left -= GRAPH_CHUNK;
+        }
+    }
+    for (; left > 0; --left) {
+        int rc = launch_one(c);
         if (rc) return rc;
         c->it += 1;
     }

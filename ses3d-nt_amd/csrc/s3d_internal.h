@@ -38,7 +38,8 @@ template <typename R>
 struct ElemArgs {
     int nx, ny, nz, nel;
     int remap;                 // XCD-aware block order
-    int it;                    // 1-based iteration being computed
+    const int *it_dev;         // iterations completed so far (device counter);
+                               // this launch computes iteration *it_dev + 1
     int nt;
     int kahan_sigma;
     const R *vx, *vy, *vz;     // unique-node velocity

@@ -206,15 +206,17 @@ elem_kernel(const ElemArgs<R> a)
     }
 
     // ---- K7: moment-tensor glut and single force of the previous add() ----
+    // iteration it = *it_dev + 1 uses the STF sample it-1 (zero for it = 1)
     R src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (a.src_slot && valid && a.it > 1) {
+    const int it = *a.it_dev + 1;
+    if (a.src_slot && valid && it > 1 && it <= a.nt) {
         const int sl = a.src_slot[ee];
         if (sl >= 0) {
 #pragma unroll
             for (int c = 0; c < 9; ++c) {
                 const int ch = a.src_chan[sl * 9 + c];
                 if (ch >= 0) {
-                    const double d = a.stf[(long)ch * a.nt + (a.it - 2)];
+                    const double d = a.stf[(long)ch * a.nt + (it - 2)];
                     src[c] = (R)(-(d * a.src_coef[((long)sl * 9 + c) * NP + node]));
                 }
             }
@@ -392,8 +394,11 @@ __global__ void __launch_bounds__(64)
 record_kernel(int nx, int ny, int nel, const R *vx, const R *vy, const R *vz,
               const R *lx, const R *ly, const R *lz,
               const R *sig, const int *rec_e, const double *rec_vm,
-              const int *rec_nch, const int *rec_ch, double *traces, int nt, int it)
+              const int *rec_nch, const int *rec_ch, double *traces, int nt, const int *it_dev)
 {
+    // records iteration it = *it_dev + 1 (the counter advances after this kernel)
+    const int it = *it_dev + 1;
+    if (it < 1 || it > nt) return;
     constexpr int P1 = P + 1, NP = P1 * P1 * P1;
     const int r = blockIdx.x;
     const int lane = threadIdx.x;
@@ -434,6 +439,13 @@ record_kernel(int nx, int ny, int nel, const R *vx, const R *vy, const R *vz,
             traces[((long)r * 8 + q) * nt + (it - 1)] = val;
         }
     }
+}
+
+// End of one iteration: advance the device step counter. A single lane
+// stores through the vector path.
+__global__ void __launch_bounds__(64) advance_kernel(int *it_dev)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) it_dev[0] = it_dev[0] + 1;
 }
 
 // ---------------------------------------------------------------------------

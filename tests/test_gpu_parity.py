@@ -99,3 +99,24 @@ def test_homogeneous_traces(precision, tol):
             errs.append(rel_l2(sim.trace(r, ch), o.trace(r, ch)))
     print(precision, "max relL2", max(errs), "median", np.median(errs))
     assert max(errs) < tol
+
+
+def test_graph_replay_matches_eager_steps():
+    """Chunks of 16 steps replay one captured hipGraph; single steps launch
+    eagerly. Every replay is checked against the eager run, with eager work
+    (field reads, v range) and synchronisation between replays."""
+    case = synth.homogeneous(nt=64, nx=6, ny=6, nz=4, n_rec=4)
+    a, recs = api.build_case(case, precision="fp32")
+    b, _ = api.build_case(case, precision="fp32")
+    for chunk in range(4):
+        for _ in range(16):
+            a.step(1)
+        b.step(16)
+        assert a.it == b.it == 16 * (chunk + 1)
+        for r in range(len(recs)):
+            for ch in range(3):
+                assert np.array_equal(a.trace(r, ch), b.trace(r, ch)), (chunk, r, ch)
+        assert np.array_equal(a.get(api.VX), b.get(api.VX))
+        assert np.array_equal(a.vrange(), b.vrange())
+    with pytest.raises(api.S3dError, match="past nt"):
+        b.step(1)
