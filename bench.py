@@ -109,6 +109,22 @@ def _torch_sync():
         t.cuda.synchronize()
 
 
+def copy_probe(gb):
+    """A plain device-to-device copy of `gb` GB by a shader kernel (torch),
+    run three times: reads and writes exactly gb GB per launch, which
+    calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiling run."""
+    import torch
+    n = int(gb * 1e9 / 4)
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log("[bench] copy probe: 3 x %d bytes" % (4 * n))
+    del a, b
+    torch.cuda.empty_cache()
+
+
 def cpu_baseline(case_kw, target_s=15.0):
     """FP64 oracle on a bounded sub-mesh of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -145,6 +161,8 @@ def main():
     ap.add_argument("--slab", type=int, default=4)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="first run a plain device copy of this many GB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole mesh on a single "
@@ -155,6 +173,8 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
     nt = args.warmup + args.steps
     case = synth.layered(seed=1, nt=nt, nx=args.nx, ny=args.ny, nz=args.nz, n_rec=256)
     g = case.grid
