@@ -53,6 +53,36 @@ __device__ inline long unique_index(int ex, int ey, int ez, int i, int j, int k,
     return b * (P * P * P) + li + P * (lj + P * lk);
 }
 
+// Force copies are stored per owning block: block b holds, for each of its
+// p^3 owned nodes, every element copy of that node, (p+1)^3 slots in all:
+//   [0, p^3)            copy of element b          at (li,   lj,   lk)
+//   +p^2 (li = 0)       copy of element b - x      at (p,    lj,   lk)
+//   +p^2 (lj = 0)       copy of element b - y      at (li,   p,    lk)
+//   +p   (li = lj = 0)  copy of element b - x - y  at (p,    p,    lk)
+//   +p^2 (lk = 0)       copy of element b - z      at (li,   lj,   p)
+//   +p   (li = lk = 0)  copy of element b - x - z  at (p,    lj,   p)
+//   +p   (lj = lk = 0)  copy of element b - y - z  at (li,   p,    p)
+//   +1                  copy of element b - x-y-z  at (p,    p,    p)
+// Element (ex,ey,ez) node (i,j,k) writes into block (ex+dx, ey+dy, ez+dz)
+// with d = (i==p, j==p, k==p); the node kernel then reads one contiguous
+// (p+1)^3 chunk per block and component.
+template <int P>
+__device__ inline int copy_slot(int dx, int dy, int dz, int li, int lj, int lk)
+{
+    constexpr int P2 = P * P, P3 = P * P * P;
+    const int m = dx + 2 * dy + 4 * dz;
+    switch (m) {
+    case 0: return li + P * (lj + P * lk);
+    case 1: return P3 + lj + P * lk;
+    case 2: return P3 + P2 + li + P * lk;
+    case 3: return P3 + 2 * P2 + lk;
+    case 4: return P3 + 2 * P2 + P + li + P * lj;
+    case 5: return P3 + 3 * P2 + P + lj;
+    case 6: return P3 + 3 * P2 + 2 * P + li;
+    default: return P3 + 3 * P2 + 3 * P;
+    }
+}
+
 template <typename R>
 __device__ inline void kahan_add(R &s, R &c, R inc)
 {
@@ -285,9 +315,15 @@ elem_kernel(const ElemArgs<R> a)
     const R Cz = a.w[i] * a.w[j] * th.sin_t * a.d3;
 
     if (!valid) return;
+    {
+        const int dx = i == P, dy = j == P, dz = k == P;
+        const long b = (long)(ex + dx) + (long)(a.nx + 1) * ((long)(ey + dy) + (long)(a.ny + 1) * (ez + dz));
+        const long fo = b * NP + copy_slot<P>(dx, dy, dz, dx ? 0 : i, dy ? 0 : j, dz ? 0 : k);
+        const long fbs = a.nblk * NP;     // component stride of the force buffer
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-        a.force[c * fs + eb] = src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c];
+        for (int c = 0; c < 3; ++c)
+            a.force[c * fbs + fo] = src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c];
+    }
 
     // ---- store tapered stress (K13) and memory variables ----
     if (a.kahan_sigma) {
@@ -323,15 +359,17 @@ node_kernel(const NodeArgs<R> a)
     const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
     if ((bx == a.nx && li) || (by == a.ny && lj) || (bz == a.nz && lk)) return;
 
-    const long fs = (long)a.nel * NP;
-    // copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P, lk+dz*P)
-    bool px[2] = {bx < a.nx, li == 0 && bx > 0};
-    bool py[2] = {by < a.ny, lj == 0 && by > 0};
-    bool pz[2] = {bz < a.nz, lk == 0 && bz > 0};
+    // copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P, lk+dz*P),
+    // stored in this block's slots (copy_slot)
+    const bool px[2] = {bx < a.nx, li == 0 && bx > 0};
+    const bool py[2] = {by < a.ny, lj == 0 && by > 0};
+    const bool pz[2] = {bz < a.nz, lk == 0 && bz > 0};
+    const long fbs = a.nblk * NP;
+    const long base = b * NP;
     R acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const R *f = a.force + c * fs;
+        const R *f = a.force + c * fbs + base;
         R zsum = 0;
         bool zhas = false;
 #pragma unroll
@@ -347,9 +385,7 @@ node_kernel(const NodeArgs<R> a)
 #pragma unroll
                 for (int dx = 0; dx < 2; ++dx) {
                     if (!px[dx]) continue;
-                    const long e = (long)(bx - dx) + (long)a.nx * ((long)(by - dy) + (long)a.ny * (bz - dz));
-                    const int n = (li + dx * P) + P1 * ((lj + dy * P) + P1 * (lk + dz * P));
-                    const R val = f[e * NP + n];
+                    const R val = f[copy_slot<P>(dx, dy, dz, li, lj, lk)];
                     xsum = xhas ? xsum + val : val;
                     xhas = true;
                 }
