@@ -110,19 +110,15 @@ def _torch_sync():
 
 
 def copy_probe(gb):
-    """A plain device-to-device copy of `gb` GB by a shader kernel (torch),
-    run three times: reads and writes exactly gb GB per launch, which
-    calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiling run."""
-    import torch
-    n = int(gb * 1e9 / 4)
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    log("[bench] copy probe: 3 x %d bytes" % (4 * n))
-    del a, b
-    torch.cuda.empty_cache()
+    """Run ses3d-nt_amd/bin/copy_probe: a plain device copy of `gb` GB
+    (three launches) that calibrates the FETCH_SIZE / WRITE_SIZE counters of
+    a profiling run and measures the attainable copy bandwidth."""
+    import subprocess
+    exe = os.path.join(ROOT, "ses3d-nt_amd", "bin", "copy_probe")
+    r = subprocess.run([exe, "%g" % gb, "3"], capture_output=True, text=True)
+    log(r.stdout.strip())
+    if r.returncode != 0:
+        raise RuntimeError("copy_probe failed: " + r.stderr)
 
 
 def cpu_baseline(case_kw, target_s=15.0):

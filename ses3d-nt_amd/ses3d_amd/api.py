@@ -16,7 +16,9 @@ import os
 import numpy as np
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG, "libses3d_hip.so")
+# SES3D_HIP_LIB selects another build of the same library (e.g. a tuning
+# variant); the default is the in-tree build.
+LIB_PATH = os.environ.get("SES3D_HIP_LIB", os.path.join(_PKG, "libses3d_hip.so"))
 _LIB = None
 
 FP64, FP32 = 0, 1

@@ -45,6 +45,16 @@ def build_hip():
     return out
 
 
+def build_copy_probe():
+    src = os.path.join(PKG, "csrc", "copy_probe.hip")
+    bindir = os.path.join(PKG, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    exe = os.path.join(bindir, "copy_probe")
+    flags = [f for f in HIP_FLAGS if f not in ("-shared", "-fPIC")]
+    _run([HIPCC] + flags + [src, "-o", exe])
+    return exe
+
+
 def build_fortran():
     fdir = os.path.join(PKG, "fortran")
     srcs = [os.path.join(fdir, f) for f in FORTRAN_SOURCES]
@@ -70,6 +80,7 @@ def build_fortran():
 
 def build_all():
     lib = build_hip()
+    build_copy_probe()
     exe = build_fortran()
     return lib, exe
 
