@@ -51,11 +51,11 @@ def elem_kernel_bytes(s, u, visco, kahan_sigma):
     return s * (12 + 12 * kahan_sigma + 36 * visco + (2 + visco) + 3 * u + 3)
 
 
-def load_sims(case, variants, slab):
+def load_sims(case, variants, slab, remap=True):
     """Build one Simulation per (precision, kahan_v) variant from a single
     pass over the model slabs; returns a list of (name, sim)."""
     g = case.grid
-    sims = [(name, api.Simulation(g, prec, kahan_v=kv)) for name, prec, kv in variants]
+    sims = [(name, api.Simulation(g, prec, kahan_v=kv, xcd_remap=remap)) for name, prec, kv in variants]
     t0 = time.time()
     for ez0 in range(0, g.nz, slab):
         nez = min(slab, g.nz - ez0)
@@ -157,6 +157,7 @@ def main():
     ap.add_argument("--slab", type=int, default=4)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-remap", action="store_true", help="plain workgroup order (no XCD remap)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="first run a plain device copy of this many GB (counter calibration)")
     args = ap.parse_args()
@@ -180,7 +181,7 @@ def main():
         % (g.nx, g.ny, g.nz, g.lpd, N, u, g.dt))
 
     variants = [("fp32", "fp32", True)] + ([] if args.no_fp64 else [("fp64", "fp64", False)])
-    sims = dict(load_sims(case, variants, args.slab))
+    sims = dict(load_sims(case, variants, args.slab, remap=not args.no_remap))
     wall, ms, cnt, vr = time_gpu(sims["fp32"], args.steps, args.warmup)
     value = N * args.steps / wall
     log("[bench] fp32 %.3f ms/step, elem %.3f ms, node %.3f ms, v range %s"

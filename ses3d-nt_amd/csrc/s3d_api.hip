@@ -233,7 +233,6 @@ ElemArgs<R> elem_args(s3d_ctx *c)
     std::memset(&a, 0, sizeof(a));
     const int P1 = P + 1;
     a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz; a.nel = (int)c->nel;
-    a.nblk = c->nblk;
     a.remap = c->o.xcd_remap;
     a.it_dev = c->d_it;
     a.nt = c->g.nt;
@@ -855,7 +854,7 @@ int s3d_init(s3d_ctx *c)
     if (dalloc(c, &c->sig, rs * 6 * c->nn)) return 1;
     if (c->fp32 && c->o.kahan_sigma && dalloc(c, &c->sig_lo, rs * 6 * c->nn)) return 1;
     if (c->visco && dalloc(c, &c->mem, rs * 18 * c->nn)) return 1;
-    if (dalloc(c, &c->force, rs * 3 * c->nblk * c->NP, false)) return 1;
+    if (dalloc(c, &c->force, rs * 3 * c->nn, false)) return 1;
     if (dalloc(c, &c->lam, rs * c->nn, false)) return 1;
     if (dalloc(c, &c->mu, rs * c->nn, false)) return 1;
     if (c->visco && dalloc(c, &c->tau, rs * c->nn, false)) return 1;

@@ -37,7 +37,6 @@ struct RTab {                  // per (ez, k): radius dependent factors
 template <typename R>
 struct ElemArgs {
     int nx, ny, nz, nel;
-    long nblk;                 // (nx+1)(ny+1)(nz+1) unique-node blocks
     int remap;                 // XCD-aware block order
     const int *it_dev;         // iterations completed so far (device counter);
                                // this launch computes iteration *it_dev + 1
@@ -49,7 +48,7 @@ struct ElemArgs {
     R *mem;                    // [18][nel][NP] (visco) or null
     const R *lam, *mu, *tau;   // [nel][NP]
     const R *ca, *cb, *cc;     // anisotropy A, B, C or null
-    R *force;                  // [3][nblk][NP] per-block copy slots
+    R *force;                  // [3][nel][NP]
     const ThTab<R> *tth;       // [nx*P1]
     const RTab<R> *tr;         // [nz*P1]
     const double *px, *py, *pz;  // 1-D taper profiles [n*P1]
@@ -76,7 +75,7 @@ struct NodeArgs {
     long nblk;                 // (nx+1)(ny+1)(nz+1)
     R *vx, *vy, *vz;
     R *lx, *ly, *lz;           // compensation terms or null
-    const R *force;            // [3][nblk][NP] per-block copy slots
+    const R *force;            // [3][nel][NP]
     const R *inv_mm;           // unique
     const double *px, *py, *pz;
     double alpha;

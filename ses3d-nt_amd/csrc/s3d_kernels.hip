@@ -53,36 +53,6 @@ __device__ inline long unique_index(int ex, int ey, int ez, int i, int j, int k,
     return b * (P * P * P) + li + P * (lj + P * lk);
 }
 
-// Force copies are stored per owning block: block b holds, for each of its
-// p^3 owned nodes, every element copy of that node, (p+1)^3 slots in all:
-//   [0, p^3)            copy of element b          at (li,   lj,   lk)
-//   +p^2 (li = 0)       copy of element b - x      at (p,    lj,   lk)
-//   +p^2 (lj = 0)       copy of element b - y      at (li,   p,    lk)
-//   +p   (li = lj = 0)  copy of element b - x - y  at (p,    p,    lk)
-//   +p^2 (lk = 0)       copy of element b - z      at (li,   lj,   p)
-//   +p   (li = lk = 0)  copy of element b - x - z  at (p,    lj,   p)
-//   +p   (lj = lk = 0)  copy of element b - y - z  at (li,   p,    p)
-//   +1                  copy of element b - x-y-z  at (p,    p,    p)
-// Element (ex,ey,ez) node (i,j,k) writes into block (ex+dx, ey+dy, ez+dz)
-// with d = (i==p, j==p, k==p); the node kernel then reads one contiguous
-// (p+1)^3 chunk per block and component.
-template <int P>
-__device__ inline int copy_slot(int dx, int dy, int dz, int li, int lj, int lk)
-{
-    constexpr int P2 = P * P, P3 = P * P * P;
-    const int m = dx + 2 * dy + 4 * dz;
-    switch (m) {
-    case 0: return li + P * (lj + P * lk);
-    case 1: return P3 + lj + P * lk;
-    case 2: return P3 + P2 + li + P * lk;
-    case 3: return P3 + 2 * P2 + lk;
-    case 4: return P3 + 2 * P2 + P + li + P * lj;
-    case 5: return P3 + 3 * P2 + P + lj;
-    case 6: return P3 + 3 * P2 + 2 * P + li;
-    default: return P3 + 3 * P2 + 3 * P;
-    }
-}
-
 template <typename R>
 __device__ inline void kahan_add(R &s, R &c, R inc)
 {
@@ -315,15 +285,9 @@ elem_kernel(const ElemArgs<R> a)
     const R Cz = a.w[i] * a.w[j] * th.sin_t * a.d3;
 
     if (!valid) return;
-    {
-        const int dx = i == P, dy = j == P, dz = k == P;
-        const long b = (long)(ex + dx) + (long)(a.nx + 1) * ((long)(ey + dy) + (long)(a.ny + 1) * (ez + dz));
-        const long fo = b * NP + copy_slot<P>(dx, dy, dz, dx ? 0 : i, dy ? 0 : j, dz ? 0 : k);
-        const long fbs = a.nblk * NP;     // component stride of the force buffer
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            a.force[c * fbs + fo] = src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c];
-    }
+    for (int c = 0; c < 3; ++c)
+        a.force[c * fs + eb] = src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c];
 
     // ---- store tapered stress (K13) and memory variables ----
     if (a.kahan_sigma) {
@@ -359,43 +323,38 @@ node_kernel(const NodeArgs<R> a)
     const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
     if ((bx == a.nx && li) || (by == a.ny && lj) || (bz == a.nz && lk)) return;
 
-    // copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P, lk+dz*P),
-    // stored in this block's slots (copy_slot)
-    const bool px[2] = {bx < a.nx, li == 0 && bx > 0};
-    const bool py[2] = {by < a.ny, lj == 0 && by > 0};
-    const bool pz[2] = {bz < a.nz, lk == 0 && bz > 0};
-    const long fbs = a.nblk * NP;
-    const long base = b * NP;
+    const long fs = (long)a.nel * NP;
+    // Copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P,
+    // lk+dz*P). All eight loads are issued unconditionally from clamped
+    // (always valid) addresses and absent copies are replaced by +0, so the
+    // loads overlap instead of waiting behind divergent branches. Adding +0
+    // is exact: the sum equals the reference's theta -> phi -> r sweep.
+    const bool px0 = bx < a.nx, px1 = li == 0 && bx > 0;
+    const bool py0 = by < a.ny, py1 = lj == 0 && by > 0;
+    const bool pz0 = bz < a.nz, pz1 = lk == 0 && bz > 0;
+    const int ex0 = bx < a.nx ? bx : a.nx - 1, ex1 = bx > 0 ? bx - 1 : 0;
+    const int ey0 = by < a.ny ? by : a.ny - 1, ey1 = by > 0 ? by - 1 : 0;
+    const int ez0 = bz < a.nz ? bz : a.nz - 1, ez1 = bz > 0 ? bz - 1 : 0;
+    long addr[8];
+    bool pres[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
+        const long e = (long)(dx ? ex1 : ex0) + (long)a.nx * ((long)(dy ? ey1 : ey0) + (long)a.ny * (dz ? ez1 : ez0));
+        const int n = (li + dx * P) + P1 * ((lj + dy * P) + P1 * (lk + dz * P));
+        addr[m] = e * NP + (n < NP ? n : NP - 1);
+        pres[m] = (dx ? px1 : px0) && (dy ? py1 : py0) && (dz ? pz1 : pz0);
+    }
     R acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const R *f = a.force + c * fbs + base;
-        R zsum = 0;
-        bool zhas = false;
+        const R *f = a.force + c * fs;
+        R v[8];
 #pragma unroll
-        for (int dz = 0; dz < 2; ++dz) {
-            if (!pz[dz]) continue;
-            R ysum = 0;
-            bool yhas = false;
+        for (int m = 0; m < 8; ++m) v[m] = f[addr[m]];
 #pragma unroll
-            for (int dy = 0; dy < 2; ++dy) {
-                if (!py[dy]) continue;
-                R xsum = 0;
-                bool xhas = false;
-#pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                    if (!px[dx]) continue;
-                    const R val = f[copy_slot<P>(dx, dy, dz, li, lj, lk)];
-                    xsum = xhas ? xsum + val : val;
-                    xhas = true;
-                }
-                ysum = yhas ? ysum + xsum : xsum;
-                yhas = true;
-            }
-            zsum = zhas ? zsum + ysum : ysum;
-            zhas = true;
-        }
-        acc[c] = zsum;
+        for (int m = 0; m < 8; ++m) v[m] = pres[m] ? v[m] : R(0);
+        acc[c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
     }
     const long u = tid;
     const R im = a.inv_mm[u];
