@@ -146,6 +146,54 @@ def cpu_baseline(case_kw, target_s=15.0):
                       % (g.nx, g.ny, g.nz, n, wall)}
 
 
+def precision_study(args):
+    """FP32 variants against the FP64 GPU path on the same seeded mesh:
+    relative L2 misfit of every receiver trace (vx, vy, vz) and of the final
+    velocity snapshot. Prints one JSON line."""
+    case = synth.layered(seed=1, nt=args.study_steps, nx=args.nx, ny=args.ny, nz=args.nz,
+                         n_rec=args.study_receivers)
+    g = case.grid
+    variants = [("fp64", "fp64", False, False), ("fp32+kahan_v", "fp32", True, False),
+                ("fp32", "fp32", False, False), ("fp32+kahan_v+kahan_sigma", "fp32", True, True)]
+    if args.study_variants:
+        keep = set(args.study_variants.split(","))
+        variants = [v for v in variants if v[0] in keep or v[0] == "fp64"]
+    res = {}
+    ref_tr = ref_snap = None
+    for name, prec, kv, ks in variants:
+        t0 = time.time()
+        sim, recs = api.build_case(case, precision=prec, kahan_v=kv, kahan_sigma=ks, slab=args.slab)
+        chunk = 500
+        done = 0
+        while done < g.nt:
+            n = min(chunk, g.nt - done)
+            sim.step(n)
+            done += n
+            sim.synchronize()
+            log("[study] %s step %d/%d (%.0f s)" % (name, done, g.nt, time.time() - t0))
+        tr = np.array([[sim.trace(r, c) for c in range(3)] for r in range(len(recs))])
+        snap = [sim.get(f) for f in (api.VX, api.VY, api.VZ)]
+        sim.close()
+        if ref_tr is None:
+            ref_tr, ref_snap = tr, snap
+            continue
+        e = []
+        for r in range(tr.shape[0]):
+            for c in range(3):
+                d = np.linalg.norm(ref_tr[r, c])
+                if d > 0:
+                    e.append(np.linalg.norm(tr[r, c] - ref_tr[r, c]) / d)
+        e = np.array(e)
+        sn = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(snap, ref_snap)]
+        res[name] = {"trace_relL2_median": float(np.median(e)), "trace_relL2_max": float(e.max()),
+                     "trace_relL2_p95": float(np.percentile(e, 95)), "traces": int(e.size),
+                     "snapshot_relL2": sn}
+        log("[study] %s: median %.3e max %.3e snapshot %s" % (name, np.median(e), e.max(), sn))
+    print(json.dumps({"study": "precision", "workload": "layered visco-elastic %dx%dx%d p=4"
+                      % (g.nx, g.ny, g.nz), "steps": g.nt, "dt": g.dt,
+                      "reference": "FP64 GPU path", "results": res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +206,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-remap", action="store_true", help="plain workgroup order (no XCD remap)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="FP32 variants vs FP64 seismogram / snapshot misfit instead of timing")
+    ap.add_argument("--study-steps", type=int, default=4000)
+    ap.add_argument("--study-receivers", type=int, default=256)
+    ap.add_argument("--study-variants", default="")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="first run a plain device copy of this many GB (counter calibration)")
     args = ap.parse_args()
@@ -169,6 +222,9 @@ def main():
     if api.device_count() < 1:
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.precision_study:
+        precision_study(args)
+        return
 
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
