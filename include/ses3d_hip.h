@@ -179,6 +179,21 @@ int s3d_kernel_times(s3d_ctx *ctx, double *ms3, int *count3);
 /* Device bytes held by the context. */
 int s3d_device_bytes(s3d_ctx *ctx, double *bytes);
 
+/* Batches: several initialised simulations on one device advance together.
+ * Every step runs, for all members with the same GLL order / precision /
+ * visco / compensation, ONE element kernel, ONE node kernel and ONE receiver
+ * kernel (workgroups partitioned across members), and chunks of steps replay
+ * one captured hipGraph. Members keep their own state, sources, receivers
+ * and step counters, and may differ in mesh size and GLL order. No
+ * counterpart in the reference, which runs one event per process group;
+ * this is the batch axis of the workloads with many independent regional
+ * simulations. A member may not be stepped on its own while its batch steps. */
+typedef struct s3d_batch s3d_batch;
+int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out);
+int s3d_batch_step(s3d_batch *batch, int nsteps);
+int s3d_batch_synchronize(s3d_batch *batch);
+int s3d_batch_destroy(s3d_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -291,6 +291,21 @@ NodeArgs<R> node_args(s3d_ctx *c)
     return a;
 }
 
+template <typename R>
+RecArgs<R> rec_args(s3d_ctx *c)
+{
+    RecArgs<R> a;
+    a.nx = c->g.nx; a.ny = c->g.ny; a.nel = (int)c->nel; a.nt = c->g.nt;
+    a.vx = (const R *)c->v[0]; a.vy = (const R *)c->v[1]; a.vz = (const R *)c->v[2];
+    a.lx = (const R *)c->lo[0]; a.ly = (const R *)c->lo[1]; a.lz = (const R *)c->lo[2];
+    a.sig = (const R *)c->sig;
+    a.rec_e = c->d_rec_e; a.rec_nch = c->d_rec_nch; a.rec_ch = c->d_rec_ch;
+    a.rec_vm = c->d_rec_vm;
+    a.traces = c->d_traces;
+    a.it_dev = c->d_it;
+    return a;
+}
+
 void ev_begin(s3d_ctx *c, int k)
 {
     if (!c->timing) return;
@@ -327,10 +342,7 @@ int launch_step(s3d_ctx *c)
     if (c->nrec_dev > 0) {
         ev_begin(c, 2);
         hipLaunchKernelGGL((record_kernel<P, R>), dim3(c->nrec_dev), dim3(64), 0, c->stream,
-                           c->g.nx, c->g.ny, (int)c->nel, (const R *)c->v[0], (const R *)c->v[1],
-                           (const R *)c->v[2], (const R *)c->lo[0], (const R *)c->lo[1],
-                           (const R *)c->lo[2], (const R *)c->sig, c->d_rec_e, c->d_rec_vm,
-                           c->d_rec_nch, c->d_rec_ch, c->d_traces, c->g.nt, (const int *)c->d_it);
+                           rec_args<R>(c));
         ev_end(c, 2);
         HIPCHK(hipGetLastError());
     }
@@ -565,6 +577,192 @@ int upload_receivers(s3d_ctx *c)
     HIPCHK(hipMemcpy(c->d_rec_vm, c->rec_vm.data(), sizeof(double) * c->NP * (size_t)n, hipMemcpyHostToDevice));
     c->nrec_dev = n;
     c->rec_dirty = false;
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Batches: simulations of one device advanced together. Simulations with the
+// same (order, precision, visco, compensation) form a group; each group costs
+// one element, one node and one receiver launch per step, with per-simulation
+// arguments in device arrays and workgroups partitioned by prefix sums.
+// ---------------------------------------------------------------------------
+struct BatchGroup {
+    int p = 0;
+    bool fp32 = false, visco = false, kahan = false;
+    std::vector<s3d_ctx *> sims;
+    void *d_eargs = nullptr, *d_nargs = nullptr, *d_rargs = nullptr;
+    int *d_estart = nullptr, *d_nstart = nullptr, *d_rstart = nullptr;
+    int ewg = 0, nwg = 0, rwg = 0;
+};
+
+struct s3d_batch {
+    std::vector<s3d_ctx *> sims;
+    std::vector<BatchGroup> groups;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int **d_its = nullptr;
+    bool built = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_steps = 0;
+    hipEvent_t ev = nullptr;
+};
+
+namespace {
+
+template <int P, typename R>
+int batch_build_group(BatchGroup &G)
+{
+    using S = Shape<P>;
+    const int n = (int)G.sims.size();
+    std::vector<ElemArgs<R>> ea(n);
+    std::vector<NodeArgs<R>> na(n);
+    std::vector<RecArgs<R>> ra(n);
+    std::vector<int> es(n + 1, 0), ns(n + 1, 0), rs(n + 1, 0);
+    for (int q = 0; q < n; ++q) {
+        s3d_ctx *c = G.sims[q];
+        ea[q] = elem_args<P, R>(c);
+        ea[q].remap = 0;
+        na[q] = node_args<P, R>(c);
+        ra[q] = rec_args<R>(c);
+        es[q + 1] = es[q] + (int)nblocks(c->nel, S::EPB);
+        ns[q + 1] = ns[q] + (int)nblocks(c->nuniq, 256);
+        rs[q + 1] = rs[q] + c->nrec_dev;
+    }
+    G.ewg = es[n];
+    G.nwg = ns[n];
+    G.rwg = rs[n];
+    HIPCHK(hipMalloc(&G.d_eargs, sizeof(ElemArgs<R>) * n));
+    HIPCHK(hipMalloc(&G.d_nargs, sizeof(NodeArgs<R>) * n));
+    HIPCHK(hipMalloc(&G.d_rargs, sizeof(RecArgs<R>) * n));
+    HIPCHK(hipMalloc(&G.d_estart, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&G.d_nstart, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&G.d_rstart, sizeof(int) * (n + 1)));
+    HIPCHK(hipMemcpy(G.d_eargs, ea.data(), sizeof(ElemArgs<R>) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_nargs, na.data(), sizeof(NodeArgs<R>) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_rargs, ra.data(), sizeof(RecArgs<R>) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_estart, es.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_nstart, ns.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_rstart, rs.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <int P, typename R>
+int batch_launch_group(const BatchGroup &G, hipStream_t st)
+{
+    using S = Shape<P>;
+    const int n = (int)G.sims.size();
+    if (G.visco)
+        hipLaunchKernelGGL((elem_kernel_batch<P, R, true>), dim3(G.ewg), dim3(S::NT), 0, st,
+                           (const ElemArgs<R> *)G.d_eargs, G.d_estart, n);
+    else
+        hipLaunchKernelGGL((elem_kernel_batch<P, R, false>), dim3(G.ewg), dim3(S::NT), 0, st,
+                           (const ElemArgs<R> *)G.d_eargs, G.d_estart, n);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <int P, typename R>
+int batch_launch_node(const BatchGroup &G, hipStream_t st)
+{
+    const int n = (int)G.sims.size();
+    if (G.kahan)
+        hipLaunchKernelGGL((node_kernel_batch<P, R, true>), dim3(G.nwg), dim3(256), 0, st,
+                           (const NodeArgs<R> *)G.d_nargs, G.d_nstart, n);
+    else
+        hipLaunchKernelGGL((node_kernel_batch<P, R, false>), dim3(G.nwg), dim3(256), 0, st,
+                           (const NodeArgs<R> *)G.d_nargs, G.d_nstart, n);
+    HIPCHK(hipGetLastError());
+    if (G.rwg > 0) {
+        hipLaunchKernelGGL((record_kernel_batch<P, R>), dim3(G.rwg), dim3(64), 0, st,
+                           (const RecArgs<R> *)G.d_rargs, G.d_rstart, n);
+        HIPCHK(hipGetLastError());
+    }
+    return 0;
+}
+
+int batch_group_call(int what, BatchGroup &G, hipStream_t st)
+{
+#define S3D_BATCH_CASE(PP)                                                                  \
+    case PP:                                                                                \
+        if (what == 0) return G.fp32 ? batch_build_group<PP, float>(G) : batch_build_group<PP, double>(G); \
+        if (what == 1) return G.fp32 ? batch_launch_group<PP, float>(G, st) : batch_launch_group<PP, double>(G, st); \
+        return G.fp32 ? batch_launch_node<PP, float>(G, st) : batch_launch_node<PP, double>(G, st);
+    switch (G.p) {
+        S3D_BATCH_CASE(2)
+        S3D_BATCH_CASE(3)
+        S3D_BATCH_CASE(4)
+        S3D_BATCH_CASE(5)
+        S3D_BATCH_CASE(6)
+        S3D_BATCH_CASE(7)
+    default: return fail("unsupported lpd");
+    }
+#undef S3D_BATCH_CASE
+}
+
+void batch_free_groups(s3d_batch *b)
+{
+    for (auto &G : b->groups) {
+        void *bufs[] = {G.d_eargs, G.d_nargs, G.d_rargs, G.d_estart, G.d_nstart, G.d_rstart};
+        for (void *p : bufs) dfree(p);
+    }
+    b->groups.clear();
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    b->graph_steps = 0;
+    b->built = false;
+}
+
+// one batched iteration: every group's element kernel, then every group's
+// node and receiver kernels, then every counter advances
+int batch_launch_step(s3d_batch *b)
+{
+    for (auto &G : b->groups)
+        if (int rc = batch_group_call(1, G, b->stream)) return rc;
+    for (auto &G : b->groups)
+        if (int rc = batch_group_call(2, G, b->stream)) return rc;
+    const int n = (int)b->sims.size();
+    hipLaunchKernelGGL(advance_many_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_its, n);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+int batch_build(s3d_batch *b)
+{
+    for (s3d_ctx *c : b->sims) {
+        if (c->src_dirty || c->rec_dirty) b->built = false;
+    }
+    if (b->built) return 0;
+    batch_free_groups(b);
+    for (s3d_ctx *c : b->sims) {
+        if (upload_sources(c) || upload_receivers(c)) return 1;
+        drop_graph(c);
+    }
+    for (s3d_ctx *c : b->sims) {
+        bool placed = false;
+        for (auto &G : b->groups)
+            if (G.p == c->p && G.fp32 == c->fp32 && G.visco == c->visco && G.kahan == (c->lo[0] != nullptr)) {
+                G.sims.push_back(c);
+                placed = true;
+                break;
+            }
+        if (!placed) {
+            BatchGroup G;
+            G.p = c->p;
+            G.fp32 = c->fp32;
+            G.visco = c->visco;
+            G.kahan = c->lo[0] != nullptr;
+            G.sims.push_back(c);
+            b->groups.push_back(G);
+        }
+    }
+    for (auto &G : b->groups)
+        if (int rc = batch_group_call(0, G, b->stream)) return rc;
+    b->built = true;
     return 0;
 }
 
@@ -1237,6 +1435,105 @@ int s3d_kernel_times(s3d_ctx *c, double *ms3, int *count3)
         ms3[k] = c->ms[k];
         count3[k] = c->cnt[k];
     }
+    return 0;
+}
+
+int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out)
+{
+    if (!sims || !out || n < 1) return fail("a batch needs at least one simulation");
+    for (int q = 0; q < n; ++q) {
+        if (!sims[q] || !sims[q]->inited) return fail("batch members must be initialised (s3d_init)");
+        if (sims[q]->dev != sims[0]->dev) return fail("batch members must share one device");
+        for (int r = 0; r < q; ++r)
+            if (sims[r] == sims[q]) return fail("a simulation appears twice in the batch");
+    }
+    HIPCHK(hipSetDevice(sims[0]->dev));
+    s3d_batch *b = new s3d_batch();
+    b->sims.assign(sims, sims + n);
+    b->dev = sims[0]->dev;
+    hipError_t e = hipStreamCreate(&b->stream);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(&b->d_its, sizeof(int *) * n);
+    if (e == hipSuccess) {
+        std::vector<int *> its(n);
+        for (int q = 0; q < n; ++q) its[q] = sims[q]->d_it;
+        e = hipMemcpy(b->d_its, its.data(), sizeof(int *) * n, hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return fail(std::string("batch: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+int s3d_batch_step(s3d_batch *b, int nsteps)
+{
+    if (!b) return fail("null argument");
+    if (nsteps < 0) return fail("negative step count");
+    for (s3d_ctx *c : b->sims)
+        if (c->it + nsteps > c->g.nt) return fail("stepping past nt in a batch member");
+    HIPCHK(hipSetDevice(b->dev));
+    if (int rc = batch_build(b)) return rc;
+    // order the batch stream after each member's own stream
+    for (s3d_ctx *c : b->sims) {
+        HIPCHK(hipEventRecord(b->ev, c->stream));
+        HIPCHK(hipStreamWaitEvent(b->stream, b->ev, 0));
+    }
+    constexpr int GRAPH_CHUNK = 16;
+    int left = nsteps;
+    if (left >= GRAPH_CHUNK) {
+        if (b->graph_steps != GRAPH_CHUNK) {
+            HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int rc = 0;
+            for (int q = 0; q < GRAPH_CHUNK && rc == 0; ++q) rc = batch_launch_step(b);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return fail(std::string("batch capture: ") + hipGetErrorString(e));
+            b->graph = g;
+            e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) return fail(std::string("batch instantiate: ") + hipGetErrorString(e));
+            b->graph_steps = GRAPH_CHUNK;
+        }
+        while (left >= GRAPH_CHUNK) {
+            HIPCHK(hipGraphLaunch(b->graph_exec, b->stream));
+            for (s3d_ctx *c : b->sims) c->it += GRAPH_CHUNK;
+            left -= GRAPH_CHUNK;
+        }
+    }
+    for (; left > 0; --left) {
+        if (int rc = batch_launch_step(b)) return rc;
+        for (s3d_ctx *c : b->sims) c->it += 1;
+    }
+    // and each member's stream after the batch
+    HIPCHK(hipEventRecord(b->ev, b->stream));
+    for (s3d_ctx *c : b->sims) HIPCHK(hipStreamWaitEvent(c->stream, b->ev, 0));
+    return 0;
+}
+
+int s3d_batch_synchronize(s3d_batch *b)
+{
+    if (!b) return fail("null argument");
+    HIPCHK(hipSetDevice(b->dev));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+int s3d_batch_destroy(s3d_batch *b)
+{
+    if (!b) return 0;
+    (void)hipSetDevice(b->dev);
+    (void)hipStreamSynchronize(b->stream);
+    batch_free_groups(b);
+    dfree(b->d_its);
+    if (b->ev) (void)hipEventDestroy(b->ev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
     return 0;
 }
 

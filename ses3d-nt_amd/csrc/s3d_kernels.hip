@@ -66,8 +66,7 @@ __device__ inline void kahan_add(R &s, R &c, R inc)
 // element kernel
 // ---------------------------------------------------------------------------
 template <int P, typename R, bool VISCO>
-__global__ void __launch_bounds__(Shape<P>::NT)
-elem_kernel(const ElemArgs<R> a)
+__device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
     using S = Shape<P>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
@@ -76,7 +75,6 @@ elem_kernel(const ElemArgs<R> a)
     const int t = threadIdx.x;
     const int slot = t / NP;
     const int node = t - slot * NP;
-    const int blk = remap_block(blockIdx.x, gridDim.x, a.remap);
     const int e = blk * EPB + slot;
     const bool inside = slot < EPB;
     const bool valid = inside && e < a.nel;
@@ -306,15 +304,42 @@ elem_kernel(const ElemArgs<R> a)
     }
 }
 
+template <int P, typename R, bool VISCO>
+__global__ void __launch_bounds__(Shape<P>::NT)
+elem_kernel(const ElemArgs<R> a)
+{
+    elem_body<P, R, VISCO>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
+}
+
+// Index of the simulation that owns workgroup `wg` of a batched launch:
+// start[s] is the first workgroup of simulation s, start[n] the total.
+__device__ inline int batch_owner(const int *start, int n, int wg)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= wg) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Element kernel over a batch of simulations with the same order and
+// precision: one launch, workgroups partitioned by `start`.
+template <int P, typename R, bool VISCO>
+__global__ void __launch_bounds__(Shape<P>::NT)
+elem_kernel_batch(const ElemArgs<R> *args, const int *start, int n)
+{
+    const int s = batch_owner(start, n, blockIdx.x);
+    elem_body<P, R, VISCO>(args[s], blockIdx.x - start[s]);
+}
+
 // ---------------------------------------------------------------------------
 // node kernel: assembly, 1/M, velocity integration, taper
 // ---------------------------------------------------------------------------
 template <int P, typename R, bool KV>
-__global__ void __launch_bounds__(256)
-node_kernel(const NodeArgs<R> a)
+__device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
 {
     constexpr int P1 = P + 1, NP = P1 * P1 * P1, PB = P * P * P;
-    const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long b = tid / PB;
     const int o = (int)(tid - b * PB);
     if (b >= a.nblk) return;
@@ -381,45 +406,65 @@ node_kernel(const NodeArgs<R> a)
     }
 }
 
+template <int P, typename R, bool KV>
+__global__ void __launch_bounds__(256)
+node_kernel(const NodeArgs<R> a)
+{
+    node_body<P, R, KV>(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int P, typename R, bool KV>
+__global__ void __launch_bounds__(256)
+node_kernel_batch(const NodeArgs<R> *args, const int *start, int n)
+{
+    const int s = batch_owner(start, n, blockIdx.x);
+    node_body<P, R, KV>(args[s], (long)(blockIdx.x - start[s]) * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // receivers: one wave per receiver, all its channels
 // ---------------------------------------------------------------------------
+template <typename R>
+struct RecArgs {
+    int nx, ny, nel, nt;
+    const R *vx, *vy, *vz, *lx, *ly, *lz, *sig;
+    const int *rec_e, *rec_nch, *rec_ch;
+    const double *rec_vm;
+    double *traces;
+    const int *it_dev;
+};
+
 template <int P, typename R>
-__global__ void __launch_bounds__(64)
-record_kernel(int nx, int ny, int nel, const R *vx, const R *vy, const R *vz,
-              const R *lx, const R *ly, const R *lz,
-              const R *sig, const int *rec_e, const double *rec_vm,
-              const int *rec_nch, const int *rec_ch, double *traces, int nt, const int *it_dev)
+__device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
 {
-    // records iteration it = *it_dev + 1 (the counter advances after this kernel)
-    const int it = *it_dev + 1;
-    if (it < 1 || it > nt) return;
     constexpr int P1 = P + 1, NP = P1 * P1 * P1;
-    const int r = blockIdx.x;
+    // records iteration it = *it_dev + 1 (the counter advances after this kernel)
+    const int it = *a.it_dev + 1;
+    if (it < 1 || it > a.nt) return;
     const int lane = threadIdx.x;
-    const int e = rec_e[r];
-    const int ex = e % nx, ey = (e / nx) % ny, ez = e / (nx * ny);
-    const long fs = (long)nel * NP;
+    const int e = a.rec_e[r];
+    const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
+    const long fs = (long)a.nel * NP;
     double acc[4] = {0, 0, 0, 0};   // vx, vy, vz, pressure
     for (int n = lane; n < NP; n += 64) {
         const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
-        const long u = unique_index<P>(ex, ey, ez, i, j, k, nx, ny);
-        const double w = rec_vm[(long)r * NP + n];
+        const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
+        const double w = a.rec_vm[(long)r * NP + n];
         // compensated velocity: value = hi - lo
-        acc[0] += w * ((double)vx[u] - (lx ? (double)lx[u] : 0.0));
-        acc[1] += w * ((double)vy[u] - (ly ? (double)ly[u] : 0.0));
-        acc[2] += w * ((double)vz[u] - (lz ? (double)lz[u] : 0.0));
+        acc[0] += w * ((double)a.vx[u] - (a.lx ? (double)a.lx[u] : 0.0));
+        acc[1] += w * ((double)a.vy[u] - (a.ly ? (double)a.ly[u] : 0.0));
+        acc[2] += w * ((double)a.vz[u] - (a.lz ? (double)a.lz[u] : 0.0));
         const long eb = (long)e * NP + n;
-        const double tr = (double)sig[XX * fs + eb] + (double)sig[YY * fs + eb] + (double)sig[ZZ * fs + eb];
+        const double tr = (double)a.sig[XX * fs + eb] + (double)a.sig[YY * fs + eb] + (double)a.sig[ZZ * fs + eb];
         acc[3] += -((double)(1.0f / 3.0f) * tr) * w;
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
         for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_down(acc[c], off, 64);
     if (lane == 0) {
-        const int nch = rec_nch[r];
+        const int nch = a.rec_nch[r];
         for (int q = 0; q < nch; ++q) {
-            const int ch = rec_ch[r * 8 + q];
+            const int ch = a.rec_ch[r * 8 + q];
             double val = 0.0;
             switch (ch) {
             case 0: val = acc[0]; break;          // vx
@@ -431,9 +476,22 @@ record_kernel(int nx, int ny, int nel, const R *vx, const R *vy, const R *vz,
             case 6: val = acc[3]; break;          // pressure
             default: break;
             }
-            traces[((long)r * 8 + q) * nt + (it - 1)] = val;
+            a.traces[((long)r * 8 + q) * a.nt + (it - 1)] = val;
         }
     }
+}
+
+template <int P, typename R>
+__global__ void __launch_bounds__(64) record_kernel(const RecArgs<R> a)
+{
+    record_body<P, R>(a, blockIdx.x);
+}
+
+template <int P, typename R>
+__global__ void __launch_bounds__(64) record_kernel_batch(const RecArgs<R> *args, const int *start, int n)
+{
+    const int s = batch_owner(start, n, blockIdx.x);
+    record_body<P, R>(args[s], blockIdx.x - start[s]);
 }
 
 // End of one iteration: advance the device step counter. A single lane
@@ -441,6 +499,13 @@ record_kernel(int nx, int ny, int nel, const R *vx, const R *vy, const R *vz,
 __global__ void __launch_bounds__(64) advance_kernel(int *it_dev)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) it_dev[0] = it_dev[0] + 1;
+}
+
+// End of one batched iteration: advance every simulation's counter.
+__global__ void __launch_bounds__(64) advance_many_kernel(int *const *its, int n)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) its[s][0] = its[s][0] + 1;
 }
 
 // ---------------------------------------------------------------------------

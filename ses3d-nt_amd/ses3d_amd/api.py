@@ -37,7 +37,8 @@ EXPORTS = [
     "s3d_add_receiver", "s3d_step", "s3d_current_step", "s3d_get_trace",
     "s3d_get_field", "s3d_set_field", "s3d_vrange", "s3d_kinetic_energy",
     "s3d_synchronize", "s3d_timing_enable", "s3d_kernel_times",
-    "s3d_device_bytes",
+    "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step",
+    "s3d_batch_synchronize", "s3d_batch_destroy",
 ]
 
 
@@ -97,6 +98,10 @@ def lib():
         L.s3d_timing_enable.argtypes = [vp, C.c_int]
         L.s3d_kernel_times.argtypes = [vp, dp, ip]
         L.s3d_device_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        L.s3d_batch_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp)]
+        L.s3d_batch_step.argtypes = [vp, C.c_int]
+        L.s3d_batch_synchronize.argtypes = [vp]
+        L.s3d_batch_destroy.argtypes = [vp]
         _LIB = L
     return _LIB
 
@@ -254,6 +259,35 @@ class Simulation:
         x = C.c_double()
         _chk(lib().s3d_device_bytes(self._h, C.byref(x)))
         return x.value
+
+
+class Batch:
+    """Several Simulations on one device advanced together (one launch per
+    kernel class and GLL order per step, chunks replayed as one hipGraph)."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        h = C.c_void_p()
+        _chk(lib().s3d_batch_create(len(self.sims), arr, C.byref(h)))
+        self._h = h
+
+    def step(self, n=1):
+        _chk(lib().s3d_batch_step(self._h, n))
+
+    def synchronize(self):
+        _chk(lib().s3d_batch_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().s3d_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def build_case(case, precision="fp32", kahan_v=True, kahan_sigma=False,
