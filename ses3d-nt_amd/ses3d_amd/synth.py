@@ -370,3 +370,24 @@ def scaled(case, **kw):
     else:
         g.dt = cfl_dt(g, lambda r: layered_1d(r)[0])
     return c
+
+
+def uneven_batch(seed=4, n_sims=64, nt=2000, n_rec=32, min_e=(8, 8, 8), max_e=(48, 48, 24),
+                 orders=(4, 6)):
+    """Independent regional simulations of uneven size: element counts drawn
+    between min_e and max_e, GLL order from `orders`, each with its own seeded
+    perturbed layered model, source and receivers. Element size is kept near
+    that of the layered workload (0.42 deg x 0.42 deg x 43.75 km)."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for s in range(n_sims):
+        nx = int(rng.integers(min_e[0], max_e[0] + 1))
+        ny = int(rng.integers(min_e[1], max_e[1] + 1))
+        nz = int(rng.integers(min_e[2], max_e[2] + 1))
+        p = int(rng.choice(orders))
+        half_lat = 0.2083 * nx
+        half_lon = 0.2083 * ny
+        c = layered(seed=1000 + s, nt=nt, nx=nx, ny=ny, nz=nz, lpd=p, n_rec=n_rec,
+                    lat=half_lat, lon=half_lon, depth_km=43.75 * nz)
+        cases.append(c)
+    return cases
