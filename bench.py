@@ -70,6 +70,8 @@ def load_sims(case, variants, slab, remap=True):
             log("[bench] model layers %d..%d uploaded (%.0f s)" % (ez0, ez0 + nez - 1, time.time() - t0))
     for name, sim in sims:
         sim.init()
+        tile, tms = sim.autotune()
+        log("[bench] %s autotune: %d elements per workgroup, element kernel %.3f ms" % (name, tile, tms))
         nrec = 0
         for s_ in case.sources:
             loc = sim.locate(s_.lat, s_.lon, s_.depth)
@@ -201,7 +203,7 @@ def batch_bench(args):
     cases = synth.uneven_batch(seed=4, n_sims=args.batch_sims, nt=args.warmup + 2 * args.steps + 1)
     N = sum(c.grid.nodes for c in cases)
     t0 = time.time()
-    sims = [api.build_case(c, precision="fp32")[0] for c in cases]
+    sims = [api.build_case(c, precision="fp32", autotune=True)[0] for c in cases]
     log("[batch] %d simulations, %d GLL nodes, orders %s, setup %.0f s"
         % (len(sims), N, sorted(set(c.grid.lpd for c in cases)), time.time() - t0))
     b = api.Batch(sims)

@@ -107,6 +107,14 @@ int s3d_init(s3d_ctx *ctx);
  * src/model_paras.F90:529-561). Requires s3d_init. */
 int s3d_cfl(s3d_ctx *ctx, double *courant);
 
+/* Autotuner: times the element-kernel tile variants (elements per
+ * workgroup) and the XCD-aware block order on this mesh and keeps the
+ * fastest; *tile gets the elements per workgroup chosen and *ms the element
+ * kernel time. Needs the state at rest (after s3d_init, before s3d_set_field
+ * and s3d_step): the trial launches then leave every field unchanged. No
+ * counterpart in the reference. */
+int s3d_autotune(s3d_ctx *ctx, int *tile, double *ms);
+
 /* GLL points, weights and derivative matrix dl[i + (p+1)*j] = l_i'(xi_j)
  * (reference get_knots / get_weights / get_dlgll, src/gll_mod.f90:178-275). */
 int s3d_gll(int lpd, double *knots, double *weights, double *dl);

@@ -8,6 +8,7 @@
 #include "s3d_kernels.hip"
 
 #include <cmath>
+#include <type_traits>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -192,6 +193,8 @@ struct s3d_ctx {
     hipGraphExec_t graph_exec = nullptr;
     int graph_steps = 0;
     bool use_graph = true;
+    int epb = 0;               // element tile (0 = default), see s3d_autotune
+    bool fields_set = false;   // s3d_set_field was called (state not at rest)
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev[3];   // start/stop pairs per kernel class
@@ -317,19 +320,94 @@ void ev_begin(s3d_ctx *c, int k)
 }
 void ev_end(s3d_ctx *c, int k) { ev_begin(c, k); }
 
+// Element-tile variants offered to the autotuner: FP32 kernels exist for 1,
+// 2 and 4 elements per workgroup (where that fits 1024 lanes) besides the
+// default; FP64 uses the default tile only.
+template <int P, int E, typename R>
+int launch_elem_e(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st)
+{
+    using S = Shape<P, E>;
+    const unsigned ge = nblocks(c->nel, S::EPB);
+    if (c->visco)
+        hipLaunchKernelGGL((elem_kernel<P, E, R, true>), dim3(ge), dim3(S::NT), 0, st, ea);
+    else
+        hipLaunchKernelGGL((elem_kernel<P, E, R, false>), dim3(ge), dim3(S::NT), 0, st, ea);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <int P>
+constexpr bool tile_ok(int e) { return e * (P + 1) * (P + 1) * (P + 1) <= 1024; }
+
+template <int P, typename R>
+int launch_elem(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st, int epb)
+{
+    if constexpr (std::is_same<R, float>::value) {
+        if (epb == 1) return launch_elem_e<P, 1, R>(c, ea, st);
+        if constexpr (tile_ok<P>(2)) if (epb == 2) return launch_elem_e<P, 2, R>(c, ea, st);
+        if constexpr (tile_ok<P>(4)) if (epb == 4) return launch_elem_e<P, 4, R>(c, ea, st);
+    }
+    return launch_elem_e<P, 0, R>(c, ea, st);
+}
+
+template <int P>
+std::vector<int> tile_candidates(bool fp32)
+{
+    std::vector<int> v = {0};
+    if (!fp32) return v;
+    for (int e : {1, 2, 4})
+        if (e * (P + 1) * (P + 1) * (P + 1) <= 1024 && e != default_epb<P>()) v.push_back(e);
+    return v;
+}
+
+void drop_graph(s3d_ctx *c);
+
+// Time every tile variant x block order on this mesh (state at rest, so
+// the trial launches write zeros over zeros) and keep the fastest.
+template <int P, typename R>
+int autotune_p(s3d_ctx *c, int *tile_out, double *ms_out)
+{
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e300;
+    int best_tile = c->epb, best_remap = c->o.xcd_remap;
+    for (int tile : tile_candidates<P>(c->fp32)) {
+        for (int remap = 0; remap < 2; ++remap) {
+            c->o.xcd_remap = remap;
+            ElemArgs<R> ea = elem_args<P, R>(c);
+            if (int rc = launch_elem<P, R>(c, ea, c->stream, tile)) return rc;   // warm-up
+            HIPCHK(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < 3; ++r)
+                if (int rc = launch_elem<P, R>(c, ea, c->stream, tile)) return rc;
+            HIPCHK(hipEventRecord(e1, c->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms / 3.0 < best) {
+                best = ms / 3.0;
+                best_tile = tile;
+                best_remap = remap;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->epb = best_tile;
+    c->o.xcd_remap = best_remap;
+    drop_graph(c);
+    if (tile_out) *tile_out = best_tile == 0 ? default_epb<P>() : best_tile;
+    if (ms_out) *ms_out = best;
+    return 0;
+}
+
 template <int P, typename R>
 int launch_step(s3d_ctx *c)
 {
-    using S = Shape<P>;
-    const unsigned ge = nblocks(c->nel, S::EPB);
     ElemArgs<R> ea = elem_args<P, R>(c);
     ev_begin(c, 0);
-    if (c->visco)
-        hipLaunchKernelGGL((elem_kernel<P, R, true>), dim3(ge), dim3(S::NT), 0, c->stream, ea);
-    else
-        hipLaunchKernelGGL((elem_kernel<P, R, false>), dim3(ge), dim3(S::NT), 0, c->stream, ea);
+    if (int rc = launch_elem<P, R>(c, ea, c->stream, c->epb)) return rc;
     ev_end(c, 0);
-    HIPCHK(hipGetLastError());
     NodeArgs<R> na = node_args<P, R>(c);
     const unsigned gn = nblocks(c->nuniq, 256);
     ev_begin(c, 1);
@@ -590,6 +668,7 @@ int upload_receivers(s3d_ctx *c)
 // ---------------------------------------------------------------------------
 struct BatchGroup {
     int p = 0;
+    int epb = 0;
     bool fp32 = false, visco = false, kahan = false;
     std::vector<s3d_ctx *> sims;
     void *d_eargs = nullptr, *d_nargs = nullptr, *d_rargs = nullptr;
@@ -612,10 +691,10 @@ struct s3d_batch {
 
 namespace {
 
-template <int P, typename R>
-int batch_build_group(BatchGroup &G)
+template <int P, int E, typename R>
+int batch_build_group_e(BatchGroup &G)
 {
-    using S = Shape<P>;
+    using S = Shape<P, E>;
     const int n = (int)G.sims.size();
     std::vector<ElemArgs<R>> ea(n);
     std::vector<NodeArgs<R>> na(n);
@@ -649,19 +728,41 @@ int batch_build_group(BatchGroup &G)
     return 0;
 }
 
-template <int P, typename R>
-int batch_launch_group(const BatchGroup &G, hipStream_t st)
+template <int P, int E, typename R>
+int batch_launch_group_e(const BatchGroup &G, hipStream_t st)
 {
-    using S = Shape<P>;
+    using S = Shape<P, E>;
     const int n = (int)G.sims.size();
     if (G.visco)
-        hipLaunchKernelGGL((elem_kernel_batch<P, R, true>), dim3(G.ewg), dim3(S::NT), 0, st,
+        hipLaunchKernelGGL((elem_kernel_batch<P, E, R, true>), dim3(G.ewg), dim3(S::NT), 0, st,
                            (const ElemArgs<R> *)G.d_eargs, G.d_estart, n);
     else
-        hipLaunchKernelGGL((elem_kernel_batch<P, R, false>), dim3(G.ewg), dim3(S::NT), 0, st,
+        hipLaunchKernelGGL((elem_kernel_batch<P, E, R, false>), dim3(G.ewg), dim3(S::NT), 0, st,
                            (const ElemArgs<R> *)G.d_eargs, G.d_estart, n);
     HIPCHK(hipGetLastError());
     return 0;
+}
+
+template <int P, typename R>
+int batch_build_group(BatchGroup &G)
+{
+    if constexpr (std::is_same<R, float>::value) {
+        if (G.epb == 1) return batch_build_group_e<P, 1, R>(G);
+        if constexpr (tile_ok<P>(2)) if (G.epb == 2) return batch_build_group_e<P, 2, R>(G);
+        if constexpr (tile_ok<P>(4)) if (G.epb == 4) return batch_build_group_e<P, 4, R>(G);
+    }
+    return batch_build_group_e<P, 0, R>(G);
+}
+
+template <int P, typename R>
+int batch_launch_group(const BatchGroup &G, hipStream_t st)
+{
+    if constexpr (std::is_same<R, float>::value) {
+        if (G.epb == 1) return batch_launch_group_e<P, 1, R>(G, st);
+        if constexpr (tile_ok<P>(2)) if (G.epb == 2) return batch_launch_group_e<P, 2, R>(G, st);
+        if constexpr (tile_ok<P>(4)) if (G.epb == 4) return batch_launch_group_e<P, 4, R>(G, st);
+    }
+    return batch_launch_group_e<P, 0, R>(G, st);
 }
 
 template <int P, typename R>
@@ -745,7 +846,8 @@ int batch_build(s3d_batch *b)
     for (s3d_ctx *c : b->sims) {
         bool placed = false;
         for (auto &G : b->groups)
-            if (G.p == c->p && G.fp32 == c->fp32 && G.visco == c->visco && G.kahan == (c->lo[0] != nullptr)) {
+            if (G.p == c->p && G.fp32 == c->fp32 && G.visco == c->visco &&
+                G.kahan == (c->lo[0] != nullptr) && G.epb == c->epb) {
                 G.sims.push_back(c);
                 placed = true;
                 break;
@@ -756,6 +858,7 @@ int batch_build(s3d_batch *b)
             G.fp32 = c->fp32;
             G.visco = c->visco;
             G.kahan = c->lo[0] != nullptr;
+            G.epb = c->epb;
             G.sims.push_back(c);
             b->groups.push_back(G);
         }
@@ -1093,6 +1196,23 @@ int s3d_init(s3d_ctx *c)
     return 0;
 }
 
+int s3d_autotune(s3d_ctx *c, int *tile, double *ms)
+{
+    if (!c) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    if (c->it != 0 || c->fields_set)
+        return fail("s3d_autotune needs the state at rest: call it after s3d_init, before s3d_set_field / s3d_step");
+    HIPCHK(hipSetDevice(c->dev));
+    int rc;
+    switch (c->p) {
+#define S3D_TUNE(PP) case PP: rc = c->fp32 ? autotune_p<PP, float>(c, tile, ms) : autotune_p<PP, double>(c, tile, ms); break;
+        S3D_TUNE(2) S3D_TUNE(3) S3D_TUNE(4) S3D_TUNE(5) S3D_TUNE(6) S3D_TUNE(7)
+#undef S3D_TUNE
+    default: return fail("unsupported lpd");
+    }
+    return rc;
+}
+
 int s3d_cfl(s3d_ctx *c, double *courant)
 {
     if (!c || !courant) return fail("null argument");
@@ -1342,6 +1462,7 @@ int s3d_set_field(s3d_ctx *c, int field, const double *in)
     if (!c || !in) return fail("null argument");
     if (!c->inited) return fail("s3d_init first");
     HIPCHK(hipSetDevice(c->dev));
+    c->fields_set = true;
     const long nn = c->nn;
     double *buf = nullptr;
     HIPCHK(hipMalloc(&buf, sizeof(double) * nn));

@@ -25,12 +25,17 @@
 
 namespace s3d {
 
+// Element tile: EPB elements per workgroup, one lane per element-local node.
+// EPB = 0 selects the default, as many elements as fit in 256 lanes.
 template <int P>
+constexpr int default_epb() { return (256 / ((P + 1) * (P + 1) * (P + 1))) > 0 ? (256 / ((P + 1) * (P + 1) * (P + 1))) : 1; }
+
+template <int P, int E = 0>
 struct Shape {
     static constexpr int P1 = P + 1;
     static constexpr int NP = P1 * P1 * P1;
     static constexpr int PB = P * P * P;
-    static constexpr int EPB = (256 / NP) > 0 ? (256 / NP) : 1;
+    static constexpr int EPB = E > 0 ? E : default_epb<P>();
     static constexpr int NT = ((EPB * NP + 63) / 64) * 64;
 };
 
@@ -65,10 +70,10 @@ __device__ inline void kahan_add(R &s, R &c, R inc)
 // ---------------------------------------------------------------------------
 // element kernel
 // ---------------------------------------------------------------------------
-template <int P, typename R, bool VISCO>
+template <int P, int E, typename R, bool VISCO>
 __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
-    using S = Shape<P>;
+    using S = Shape<P, E>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
     __shared__ R lds[6][EPB * NP];
 
@@ -304,11 +309,11 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     }
 }
 
-template <int P, typename R, bool VISCO>
-__global__ void __launch_bounds__(Shape<P>::NT)
+template <int P, int E, typename R, bool VISCO>
+__global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel(const ElemArgs<R> a)
 {
-    elem_body<P, R, VISCO>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
+    elem_body<P, E, R, VISCO>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
 }
 
 // Index of the simulation that owns workgroup `wg` of a batched launch:
@@ -325,12 +330,12 @@ __device__ inline int batch_owner(const int *start, int n, int wg)
 
 // Element kernel over a batch of simulations with the same order and
 // precision: one launch, workgroups partitioned by `start`.
-template <int P, typename R, bool VISCO>
-__global__ void __launch_bounds__(Shape<P>::NT)
+template <int P, int E, typename R, bool VISCO>
+__global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel_batch(const ElemArgs<R> *args, const int *start, int n)
 {
     const int s = batch_owner(start, n, blockIdx.x);
-    elem_body<P, R, VISCO>(args[s], blockIdx.x - start[s]);
+    elem_body<P, E, R, VISCO>(args[s], blockIdx.x - start[s]);
 }
 
 // ---------------------------------------------------------------------------

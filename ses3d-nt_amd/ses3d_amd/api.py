@@ -33,7 +33,7 @@ WAVELET = {"RICKER": 0, "GAUSS": 1, "SIN**3": 2, "BRUNE": 3, "DELTA": 4}
 EXPORTS = [
     "s3d_last_error", "s3d_device_count", "s3d_create", "s3d_destroy",
     "s3d_set_model_slab", "s3d_set_model", "s3d_init", "s3d_cfl", "s3d_gll",
-    "s3d_locate", "s3d_wavelet", "s3d_add_source_mt", "s3d_add_source_sf",
+    "s3d_autotune", "s3d_locate", "s3d_wavelet", "s3d_add_source_mt", "s3d_add_source_sf",
     "s3d_add_receiver", "s3d_step", "s3d_current_step", "s3d_get_trace",
     "s3d_get_field", "s3d_set_field", "s3d_vrange", "s3d_kinetic_energy",
     "s3d_synchronize", "s3d_timing_enable", "s3d_kernel_times",
@@ -81,6 +81,7 @@ def lib():
         L.s3d_set_model.argtypes = [vp, C.c_int, dp]
         L.s3d_init.argtypes = [vp]
         L.s3d_cfl.argtypes = [vp, C.POINTER(C.c_double)]
+        L.s3d_autotune.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.s3d_gll.argtypes = [C.c_int, dp, dp, dp]
         L.s3d_locate.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, dp]
         L.s3d_wavelet.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp]
@@ -177,6 +178,12 @@ class Simulation:
 
     def init(self):
         _chk(lib().s3d_init(self._h))
+
+    def autotune(self):
+        """(elements per workgroup, element kernel ms) of the fastest tile."""
+        t, ms = C.c_int(), C.c_double()
+        _chk(lib().s3d_autotune(self._h, C.byref(t), C.byref(ms)))
+        return t.value, ms.value
 
     def cfl(self):
         x = C.c_double()
@@ -291,7 +298,7 @@ class Batch:
 
 
 def build_case(case, precision="fp32", kahan_v=True, kahan_sigma=False,
-               slab=None, xcd_remap=True):
+               slab=None, xcd_remap=True, autotune=False):
     """Simulation loaded with a synth.Case (model, sources, receivers).
     The model is uploaded `slab` element layers at a time (all at once when
     None). Returns (simulation, list of receivers that were placed)."""
@@ -309,6 +316,8 @@ def build_case(case, precision="fp32", kahan_v=True, kahan_sigma=False,
             sim.set_model_slab(Q, ez0, m["q"])
         del m
     sim.init()
+    if autotune:
+        sim.autotune()
     for s in case.sources:
         loc = sim.locate(s.lat, s.lon, s.depth)
         if loc is None:

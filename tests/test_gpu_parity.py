@@ -147,3 +147,19 @@ def test_batch_matches_individual_runs(precision):
                 assert rel_l2(s2.trace(r, ch), s1.trace(r, ch)) < 1e-12
         assert rel_l2(s2.get(api.VZ), s1.get(api.VZ)) < 1e-12
     b.close()
+
+
+def test_autotuned_tiles_give_identical_results():
+    """Every element tile computes the same per-node arithmetic, so the
+    autotuned run is bit-identical to the default one; the tuner refuses a
+    state that is not at rest."""
+    case = synth.layered(seed=5, nt=30, nx=7, ny=5, nz=4, lpd=4, n_rec=3)
+    a, recs = api.build_case(case, precision="fp32")
+    b, _ = api.build_case(case, precision="fp32", autotune=True)
+    a.step(30)
+    b.step(30)
+    for r in range(len(recs)):
+        for ch in range(3):
+            assert np.array_equal(a.trace(r, ch), b.trace(r, ch))
+    with pytest.raises(api.S3dError, match="at rest"):
+        b.autotune()
