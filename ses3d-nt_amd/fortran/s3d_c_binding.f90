@@ -32,7 +32,9 @@ MODULE ses3d_hip_c
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_FP64 = 0, S3D_FP32 = 1
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_RHOINV = 0, S3D_LAMBDA = 1, &
         S3D_MU = 2, S3D_A = 3, S3D_B = 4, S3D_C = 5, S3D_Q = 6
-    INTEGER(c_int), PARAMETER, PUBLIC :: S3D_VX = 0, S3D_VY = 1, S3D_VZ = 2
+    INTEGER(c_int), PARAMETER, PUBLIC :: S3D_VX = 0, S3D_VY = 1, S3D_VZ = 2, &
+        S3D_SXX = 3, S3D_SXY = 4, S3D_SXZ = 5, S3D_SYY = 6, S3D_SYZ = 7, S3D_SZZ = 8, &
+        S3D_MM = 18, S3D_TAPER = 19
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_CH_VX = 0, S3D_CH_VY = 1, &
         S3D_CH_VZ = 2, S3D_CH_N = 3, S3D_CH_E = 4, S3D_CH_Z = 5, &
         S3D_CH_PRESSURE = 6

@@ -33,6 +33,13 @@ MODULE configuration_mod
         CHARACTER(LEN=fnl) :: prefix
     END TYPE
 
+    TYPE, PUBLIC :: output_entry
+        INTEGER :: ts_start, ts_increment, ts_end
+        CHARACTER(LEN=8) :: attributes(max_attributes)
+        LOGICAL :: override
+        CHARACTER(LEN=fnl) :: prefix
+    END TYPE
+
     TYPE, PUBLIC :: configuration_typ
         ! &grid
         INTEGER :: nx, ny, nz, lpd, taper_width
@@ -53,6 +60,7 @@ MODULE configuration_mod
         INTEGER :: device
         TYPE(source_entry), ALLOCATABLE :: sources(:)
         TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
+        TYPE(output_entry), ALLOCATABLE :: outputs(:)
     END TYPE
 
     TYPE(configuration_typ), PUBLIC :: configuration
@@ -181,6 +189,7 @@ CONTAINS
 
         CALL parse_sources(lun, cf)
         CALL parse_receivers(lun, cf)
+        CALL parse_outputs(lun, cf)
         CLOSE(lun)
         configuration = cf
     END SUBROUTINE
@@ -250,6 +259,33 @@ CONTAINS
             IF (ios /= 0) CALL abort('Error while parsing &receiver-group!')
             cf%receivers(i) = receiver_entry(lat, lon, depth, network, station, &
                                              location, attributes, override, prefix)
+        END DO
+    END SUBROUTINE
+
+    !> &output_raw groups (src/parser/parser.f90:592-638)
+    SUBROUTINE parse_outputs(lun, cf)
+        INTEGER, INTENT(IN) :: lun
+        TYPE(configuration_typ), INTENT(INOUT) :: cf
+        INTEGER :: timestep_start, timestep_increment, timestep_end
+        LOGICAL :: override
+        CHARACTER(LEN=fnl) :: prefix
+        CHARACTER(LEN=8) :: attributes(max_attributes)
+        NAMELIST /output_raw/ timestep_start, timestep_increment, timestep_end, &
+            attributes, override, prefix
+        INTEGER :: n, i, ios
+
+        n = count_groups(lun, 'output_raw')
+        ALLOCATE(cf%outputs(n))
+        REWIND(lun)
+        DO i = 1, n
+            timestep_start = -1; timestep_increment = -1; timestep_end = -1
+            override = .FALSE.; prefix = './'; attributes = ''
+            READ(lun, NML=output_raw, IOSTAT=ios)
+            IF (ios /= 0) CALL abort('Error while parsing &output_raw-group!')
+            IF (timestep_start < 1 .OR. timestep_increment < 1 .OR. timestep_end < timestep_start) &
+                CALL abort('Non-conforming time steps in &output_raw-group!')
+            cf%outputs(i) = output_entry(timestep_start, timestep_increment, timestep_end, &
+                                         attributes, override, prefix)
         END DO
     END SUBROUTINE
 

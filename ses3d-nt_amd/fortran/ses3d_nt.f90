@@ -10,6 +10,7 @@ PROGRAM ses3d_nt
     USE configuration_mod, ONLY : configuration, parse_conf_file, abort, rk
     USE evolution_mod, ONLY : ctx, init_temp, ses3d_evolution, v_ptp_loc, finalize_evolution
     USE channels_mod, ONLY : source_cls, receiver_cls, parse_source_file, parse_receiver_file
+    USE snapshots_mod, ONLY : write_snapshots, check_snapshot_attributes
     IMPLICIT NONE
     CHARACTER(LEN=512) :: conf_file_name
     TYPE(source_cls), ALLOCATABLE :: sources(:)
@@ -21,6 +22,7 @@ PROGRAM ses3d_nt
     IF (COMMAND_ARGUMENT_COUNT() < 1) CALL abort('Config-file missing!')
     CALL GET_COMMAND_ARGUMENT(1, conf_file_name)
     CALL parse_conf_file(TRIM(conf_file_name))
+    CALL check_snapshot_attributes()
     CALL init_temp()
     CALL parse_receiver_file(ctx, receivers)
     WRITE(OUTPUT_UNIT, '("Found ", I0, " receivers.")') SIZE(receivers)
@@ -38,6 +40,7 @@ PROGRAM ses3d_nt
         DO i = 1, SIZE(receivers)
             CALL receivers(i)%record(it)
         END DO
+        CALL write_snapshots(ctx, it)
         IF (MOD(it, every) == 0 .OR. it == configuration%nt) THEN
             ptp = v_ptp_loc()
             WRITE(OUTPUT_UNIT, fmt) it, 100 * it / configuration%nt, ptp

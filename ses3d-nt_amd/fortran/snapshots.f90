@@ -1,0 +1,97 @@
+!> Raw volume snapshots (reference output_raw_cls, src/output_raw.f90:117-168,
+!! and volume_snapshot_cls%process, src/volume_snapshot.f90:123-135): at
+!! every step it with start <= it <= end and MOD(it - start, increment) == 0,
+!! each attribute is written to prefix//attribute//'_'//it//'.vol' as the
+!! global 6-D array, native endian, in the step precision (float32 for
+!! 'fp32', float64 for 'fp64', like the reference's real_kind).
+!! Attributes: vx vy vz sxx sxy sxz syy syz szz pressure taper.
+MODULE snapshots_mod
+    USE, INTRINSIC :: ISO_C_BINDING
+    USE ses3d_hip_c
+    USE configuration_mod, ONLY : configuration, output_entry, abort
+    IMPLICIT NONE
+    PRIVATE
+    PUBLIC :: write_snapshots, check_snapshot_attributes
+
+CONTAINS
+
+    LOGICAL FUNCTION due(o, it)
+        TYPE(output_entry), INTENT(IN) :: o
+        INTEGER, INTENT(IN) :: it
+        due = it >= o%ts_start .AND. it <= o%ts_end .AND. MOD(it - o%ts_start, o%ts_increment) == 0
+    END FUNCTION
+
+    SUBROUTINE check_snapshot_attributes()
+        INTEGER :: i, j
+        DO i = 1, SIZE(configuration%outputs)
+            DO j = 1, SIZE(configuration%outputs(i)%attributes)
+                SELECT CASE (TRIM(configuration%outputs(i)%attributes(j)))
+                CASE ('', 'vx', 'vy', 'vz', 'sxx', 'sxy', 'sxz', 'syy', 'syz', 'szz', 'pressure', 'taper')
+                CASE DEFAULT
+                    CALL abort('Snapshot attribute not supported on the GPU path: ' // &
+                               TRIM(configuration%outputs(i)%attributes(j)))
+                END SELECT
+            END DO
+        END DO
+    END SUBROUTINE
+
+    SUBROUTINE get(ctx, id, f)
+        TYPE(c_ptr), INTENT(IN) :: ctx
+        INTEGER(c_int), INTENT(IN) :: id
+        REAL(c_double), INTENT(OUT) :: f(:)
+        CALL check(s3d_get_field(ctx, id, f), 'get_field')
+    END SUBROUTINE
+
+    SUBROUTINE write_snapshots(ctx, it)
+        TYPE(c_ptr), INTENT(IN) :: ctx
+        INTEGER, INTENT(IN) :: it
+        INTEGER :: i, j, lun, ios
+        INTEGER(8) :: n
+        REAL(c_double), ALLOCATABLE :: f(:), g(:)
+        CHARACTER(LEN=1024) :: fn
+        CHARACTER(LEN=8) :: att
+        CHARACTER(LEN=7) :: status
+
+        n = INT(configuration%nx, 8) * configuration%ny * configuration%nz * (configuration%lpd + 1)**3
+        DO i = 1, SIZE(configuration%outputs)
+            IF (.NOT. due(configuration%outputs(i), it)) CYCLE
+            IF (.NOT. ALLOCATED(f)) ALLOCATE(f(n))
+            DO j = 1, SIZE(configuration%outputs(i)%attributes)
+                att = configuration%outputs(i)%attributes(j)
+                IF (LEN_TRIM(att) == 0) CYCLE
+                SELECT CASE (TRIM(att))
+                CASE ('vx');  CALL get(ctx, S3D_VX, f)
+                CASE ('vy');  CALL get(ctx, S3D_VY, f)
+                CASE ('vz');  CALL get(ctx, S3D_VZ, f)
+                CASE ('sxx'); CALL get(ctx, S3D_SXX, f)
+                CASE ('sxy'); CALL get(ctx, S3D_SXY, f)
+                CASE ('sxz'); CALL get(ctx, S3D_SXZ, f)
+                CASE ('syy'); CALL get(ctx, S3D_SYY, f)
+                CASE ('syz'); CALL get(ctx, S3D_SYZ, f)
+                CASE ('szz'); CALL get(ctx, S3D_SZZ, f)
+                CASE ('taper'); CALL get(ctx, S3D_TAPER, f)
+                CASE ('pressure')
+                    ! -(sxx + syy + szz) / 3 (src/config/channel_mod.f90:795)
+                    IF (.NOT. ALLOCATED(g)) ALLOCATE(g(n))
+                    CALL get(ctx, S3D_SXX, f)
+                    CALL get(ctx, S3D_SYY, g)
+                    f = f + g
+                    CALL get(ctx, S3D_SZZ, g)
+                    f = -(REAL(1.0 / 3.0, c_double) * (f + g))
+                END SELECT
+                WRITE(fn, '(A, A, "_", I0, ".vol")') TRIM(configuration%outputs(i)%prefix), TRIM(att), it
+                status = MERGE('replace', 'new    ', configuration%outputs(i)%override)
+                OPEN(NEWUNIT=lun, FILE=TRIM(fn), ACCESS='stream', FORM='unformatted', &
+                     ACTION='write', STATUS=TRIM(status), IOSTAT=ios)
+                IF (ios /= 0) CALL abort('Cannot write snapshot ' // TRIM(fn))
+                IF (configuration%precision == 'fp32') THEN
+                    WRITE(lun) REAL(f, 4)
+                ELSE
+                    WRITE(lun) f
+                END IF
+                CLOSE(lun)
+            END DO
+        END DO
+    END SUBROUTINE
+
+END MODULE snapshots_mod

@@ -53,13 +53,23 @@ def test_forward_run_matches_oracle(tmp_path, precision, tol):
     import oracle
     nt = 200
     case = synth.homogeneous(nt=nt, nx=8, ny=8, nz=5, n_rec=5)
-    cfg = write_config(case, str(tmp_path), precision=precision)
+    snap = ("\n&output_raw\n timestep_start = 100\n timestep_increment = 100\n"
+            " timestep_end = 200\n attributes = 'vz' 'pressure'\n override = .TRUE.\n"
+            " prefix = '%s/'\n/\n" % str(tmp_path))
+    cfg = write_config(case, str(tmp_path), precision=precision, extra=snap)
     r = run(cfg)
     print(r.stdout[-600:], r.stderr[-600:])
     assert r.returncode == 0
     assert "successfully completed" in r.stdout
     o, recs = oracle.build_case(case)
-    o.step(nt)
+    o.step(100)
+    dtype = "<f4" if precision == "fp32" else "<f8"
+    vz100 = np.fromfile(os.path.join(str(tmp_path), "vz_100.vol"), dtype=dtype)
+    assert rel_l2(vz100, o.get(oracle.VZ).ravel()) < tol
+    o.step(nt - 100)
+    p200 = np.fromfile(os.path.join(str(tmp_path), "pressure_200.vol"), dtype=dtype)
+    pref = -(o.get(oracle.SXX) + o.get(oracle.SYY) + o.get(oracle.SZZ)).ravel() / 3.0
+    assert rel_l2(p200, pref) < 10 * tol
     for ri, rec in enumerate(recs):
         for ci, ch in enumerate(rec.channels):
             f, i, data = read_sac(os.path.join(str(tmp_path), "XX.00.%s.%s.sac" % (rec.station, ch)))
