@@ -316,10 +316,13 @@ def random_moment_tensor(rng, scale=1e19):
 def receivers_uniform(rng, grid, n, channels=("vx", "vy", "vz"), margin=None):
     if margin is None:
         margin = grid.taper_width + 0.5
+    # keep at least one element's width to draw from on small meshes
+    mx = min(margin, 0.5 * (grid.nx - 1))
+    my = min(margin, 0.5 * (grid.ny - 1))
     dlat = (grid.lat_max - grid.lat_min) / grid.nx
     dlon = (grid.lon_max - grid.lon_min) / grid.ny
-    lat = rng.uniform(grid.lat_min + margin * dlat, grid.lat_max - margin * dlat, n)
-    lon = rng.uniform(grid.lon_min + margin * dlon, grid.lon_max - margin * dlon, n)
+    lat = rng.uniform(grid.lat_min + mx * dlat, grid.lat_max - mx * dlat, n)
+    lon = rng.uniform(grid.lon_min + my * dlon, grid.lon_max - my * dlon, n)
     return [Receiver(float(a), float(b), 0.0, "S%03d" % i, tuple(channels))
             for i, (a, b) in enumerate(zip(lat, lon))]
 
