@@ -296,6 +296,11 @@ def main():
             "algorithmic_bytes_per_launch": eb}
     roof["achieved"] = achieved / 1e9
     roof["peak"] = HBM_PEAK / 1e9
+    # HBM traffic per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE,
+    # calibrated on a plain copy kernel) for this workload, when recorded
+    tf = os.path.join(ROOT, "profiles", "traffic_layered_192x192x96_p4_fp32.json")
+    if (g.nx, g.ny, g.nz) == (192, 192, 96) and os.path.exists(tf):
+        roof["traffic"] = json.load(open(tf))["traffic_bytes_per_launch"]["elem_kernel"]
     step_frac = Bstep * value / HBM_PEAK
 
     out = {

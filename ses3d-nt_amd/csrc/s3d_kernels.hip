@@ -367,6 +367,7 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
     const int ez0 = bz < a.nz ? bz : a.nz - 1, ez1 = bz > 0 ? bz - 1 : 0;
     long addr[8];
     bool pres[8];
+    long fallback = -1;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
@@ -374,7 +375,12 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
         const int n = (li + dx * P) + P1 * ((lj + dy * P) + P1 * (lk + dz * P));
         addr[m] = e * NP + (n < NP ? n : NP - 1);
         pres[m] = (dx ? px1 : px0) && (dy ? py1 : py0) && (dz ? pz1 : pz0);
+        if (pres[m] && fallback < 0) fallback = addr[m];
     }
+    // absent copies load a present copy's word: no extra cache lines
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+        if (!pres[m]) addr[m] = fallback;
     R acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
