@@ -165,7 +165,7 @@ struct s3d_ctx {
     std::vector<double> x1, y1, z1, px1, py1, pz1;
     double courant = 0.0;
     // device state
-    void *v[3] = {}, *lo[3] = {}, *inv_mm = nullptr, *sig = nullptr, *sig_lo = nullptr,
+    void *v[3] = {}, *lo[3] = {}, *dt_inv_mm = nullptr, *sig = nullptr, *sig_lo = nullptr,
          *mem = nullptr, *lam = nullptr, *mu = nullptr, *tau = nullptr, *rho = nullptr,
          *ca = nullptr, *cb = nullptr, *cc = nullptr, *force = nullptr, *tth = nullptr,
          *tr = nullptr, *wdev = nullptr;
@@ -224,6 +224,29 @@ void dfree(void *p)
     if (p) (void)hipFree(p);
 }
 
+// 1-D colatitude and radius tables of the element kernel, computed in FP64
+// and stored in the kernel's coefficient precision C.
+template <typename C>
+int upload_tables(s3d_ctx *c, const std::vector<double> &sin1)
+{
+    const int P1 = c->p + 1;
+    std::vector<ThTab<C>> tt(c->g.nx * P1);
+    std::vector<RTab<C>> tr(c->g.nz * P1);
+    for (size_t q = 0; q < tt.size(); ++q) {
+        double s = sin1[q];
+        tt[q] = {(C)s, (C)(1.0 / s), (C)(std::cos(c->x1[q]) / s), (C)0};
+    }
+    for (size_t q = 0; q < tr.size(); ++q) {
+        double r = c->z1[q];
+        tr[q] = {(C)r, (C)(1.0 / r), (C)(r * r), (C)0};
+    }
+    if (dalloc(c, &c->tth, sizeof(tt[0]) * tt.size())) return 1;
+    if (dalloc(c, &c->tr, sizeof(tr[0]) * tr.size())) return 1;
+    HIPCHK(hipMemcpy(c->tth, tt.data(), sizeof(tt[0]) * tt.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->tr, tr.data(), sizeof(tr[0]) * tr.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
 size_t rsize(const s3d_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double); }
 
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -247,8 +270,9 @@ ElemArgs<R> elem_args(s3d_ctx *c)
     a.lam = (const R *)c->lam; a.mu = (const R *)c->mu; a.tau = (const R *)c->tau;
     a.ca = (const R *)c->ca; a.cb = (const R *)c->cb; a.cc = (const R *)c->cc;
     a.force = (R *)c->force;
-    a.tth = (const ThTab<R> *)c->tth;
-    a.tr = (const RTab<R> *)c->tr;
+    using C = coef_t<R>;
+    a.tth = (const ThTab<C> *)c->tth;
+    a.tr = (const RTab<C> *)c->tr;
     a.px = c->dpx; a.py = c->dpy; a.pz = c->dpz;
     a.alpha = c->g.taper_slope;
     a.taper_on = c->g.taper_width > 0;
@@ -256,22 +280,18 @@ ElemArgs<R> elem_args(s3d_ctx *c)
     a.src_chan = c->d_chan;
     a.src_coef = c->d_coef;
     a.stf = c->d_stf;
-    a.dt = (R)c->g.dt;
-    a.d1 = (R)((2.0 / c->dx) * c->jac);
-    a.d2 = (R)((2.0 / c->dy) * c->jac);
-    a.d3 = (R)(-((2.0 / c->dz) * c->jac));
-    for (int m = 0; m < NRDISS; ++m) {
-        a.mem_a[m] = (R)(1.0 - c->g.dt / TAU_P[m]);
-        a.mem_c[m] = (R)(c->g.dt / (NRDISS * TAU_P[m]));
-    }
-    for (int q = 0; q < P1; ++q) a.w[q] = (R)c->weights[q];
+    a.d1 = (C)((2.0 / c->dx) * c->jac);
+    a.d2 = (C)((2.0 / c->dy) * c->jac);
+    a.d3 = (C)(-((2.0 / c->dz) * c->jac));
+    for (int m = 0; m < NRDISS; ++m) a.rate[m] = (R)(c->g.dt / TAU_P[m]);
+    for (int q = 0; q < P1; ++q) a.w[q] = (C)c->weights[q];
     for (int i = 0; i < P1; ++i)
         for (int q = 0; q < P1; ++q) {
             // dl(q,i) = l_q'(xi_i) is stored at c->dl[q + P1*i]
-            a.Dx[i * P1 + q] = (R)(2.0 * c->dl[q + P1 * i] / c->dx);
-            a.Dy[i * P1 + q] = (R)(2.0 * c->dl[q + P1 * i] / c->dy);
-            a.Dz[i * P1 + q] = (R)(2.0 * c->dl[q + P1 * i] / c->dz);
-            a.dl[i * P1 + q] = (R)c->dl[i + P1 * q];
+            a.Dx[i * P1 + q] = (C)(2.0 * c->dl[q + P1 * i] / c->dx);
+            a.Dy[i * P1 + q] = (C)(2.0 * c->dl[q + P1 * i] / c->dy);
+            a.Dz[i * P1 + q] = (C)(2.0 * c->dl[q + P1 * i] / c->dz);
+            a.dlw[i * P1 + q] = (C)(c->dl[i + P1 * q] * c->weights[q]);
         }
     return a;
 }
@@ -286,11 +306,10 @@ NodeArgs<R> node_args(s3d_ctx *c)
     a.vx = (R *)c->v[0]; a.vy = (R *)c->v[1]; a.vz = (R *)c->v[2];
     a.lx = (R *)c->lo[0]; a.ly = (R *)c->lo[1]; a.lz = (R *)c->lo[2];
     a.force = (const R *)c->force;
-    a.inv_mm = (const R *)c->inv_mm;
+    a.dt_inv_mm = (const R *)c->dt_inv_mm;
     a.px = c->dpx; a.py = c->dpy; a.pz = c->dpz;
     a.alpha = c->g.taper_slope;
     a.taper_on = c->g.taper_width > 0;
-    a.dt = (R)c->g.dt;
     return a;
 }
 
@@ -500,7 +519,8 @@ struct MassRun {
     static int run(s3d_ctx *c, const double *sin_t, const double *rad, const double *w)
     {
         hipLaunchKernelGGL((mass_kernel<P, R>), dim3(nblocks(c->nuniq, 256)), dim3(256), 0, c->stream,
-                           c->stage[S3D_RHOINV], sin_t, rad, w, c->jac, c->mm64, (R *)c->inv_mm,
+                           c->stage[S3D_RHOINV], sin_t, rad, w, c->jac, c->g.dt, c->mm64,
+                           (R *)c->dt_inv_mm,
                            c->g.nx, c->g.ny, c->g.nz, c->nblk);
         HIPCHK(hipGetLastError());
         return 0;
@@ -587,7 +607,8 @@ struct KineticRun {
         HIPCHK(hipMalloc(&part, sizeof(double) * nb));
         hipLaunchKernelGGL((kinetic_kernel<P, R>), dim3(nb), dim3(256), 0, c->stream, (const R *)c->v[0],
                            (const R *)c->v[1], (const R *)c->v[2], (const R *)c->rho,
-                           (const ThTab<R> *)c->tth, (const RTab<R> *)c->tr, (const R *)c->wdev,
+                           (const ThTab<coef_t<R>> *)c->tth, (const RTab<coef_t<R>> *)c->tr,
+                           (const double *)c->wdev,
                            c->g.nx, c->g.ny, c->g.nz, part);
         std::vector<double> h(nb);
         hipError_t e = hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream);
@@ -963,7 +984,7 @@ int s3d_destroy(s3d_ctx *c)
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
-    void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->inv_mm, c->sig,
+    void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->sig,
                     c->sig_lo, c->mem, c->lam, c->mu, c->tau, c->rho, c->ca, c->cb, c->cc,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
@@ -1088,40 +1109,8 @@ int s3d_init(s3d_ctx *c)
     const size_t rs = rsize(c);
     std::vector<double> sin1(g.nx * P1), th1 = c->x1, ph1 = c->y1, r1 = c->z1;
     for (size_t q = 0; q < sin1.size(); ++q) sin1[q] = std::sin(c->x1[q]);
-    if (c->fp32) {
-        std::vector<ThTab<float>> tt(g.nx * P1);
-        std::vector<RTab<float>> tr(g.nz * P1);
-        for (size_t q = 0; q < tt.size(); ++q) {
-            double s = sin1[q];
-            tt[q] = {(float)s, (float)(1.0 / s), (float)(std::cos(c->x1[q]) / s), 0.f};
-        }
-        for (size_t q = 0; q < tr.size(); ++q) {
-            double r = c->z1[q];
-            tr[q] = {(float)r, (float)(1.0 / r), (float)(r * r), 0.f};
-        }
-        if (dalloc(c, &c->tth, sizeof(tt[0]) * tt.size())) return 1;
-        if (dalloc(c, &c->tr, sizeof(tr[0]) * tr.size())) return 1;
-        HIPCHK(hipMemcpy(c->tth, tt.data(), sizeof(tt[0]) * tt.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->tr, tr.data(), sizeof(tr[0]) * tr.size(), hipMemcpyHostToDevice));
-        std::vector<float> w(P1);
-        for (int q = 0; q < P1; ++q) w[q] = (float)c->weights[q];
-        if (dalloc(c, &c->wdev, sizeof(float) * P1)) return 1;
-        HIPCHK(hipMemcpy(c->wdev, w.data(), sizeof(float) * P1, hipMemcpyHostToDevice));
-    } else {
-        std::vector<ThTab<double>> tt(g.nx * P1);
-        std::vector<RTab<double>> tr(g.nz * P1);
-        for (size_t q = 0; q < tt.size(); ++q) {
-            double s = sin1[q];
-            tt[q] = {s, 1.0 / s, std::cos(c->x1[q]) / s, 0.0};
-        }
-        for (size_t q = 0; q < tr.size(); ++q) {
-            double r = c->z1[q];
-            tr[q] = {r, 1.0 / r, r * r, 0.0};
-        }
-        if (dalloc(c, &c->tth, sizeof(tt[0]) * tt.size())) return 1;
-        if (dalloc(c, &c->tr, sizeof(tr[0]) * tr.size())) return 1;
-        HIPCHK(hipMemcpy(c->tth, tt.data(), sizeof(tt[0]) * tt.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(c->tr, tr.data(), sizeof(tr[0]) * tr.size(), hipMemcpyHostToDevice));
+    if (c->fp32 ? upload_tables<coef_t<float>>(c, sin1) : upload_tables<double>(c, sin1)) return 1;
+    {
         if (dalloc(c, &c->wdev, sizeof(double) * P1)) return 1;
         HIPCHK(hipMemcpy(c->wdev, c->weights, sizeof(double) * P1, hipMemcpyHostToDevice));
     }
@@ -1150,7 +1139,7 @@ int s3d_init(s3d_ctx *c)
         if (dalloc(c, &c->v[q], rs * c->nuniq)) return 1;
         if (c->fp32 && c->o.kahan_v && dalloc(c, &c->lo[q], rs * c->nuniq)) return 1;
     }
-    if (dalloc(c, &c->inv_mm, rs * c->nuniq)) return 1;
+    if (dalloc(c, &c->dt_inv_mm, rs * c->nuniq)) return 1;
     if (dalloc(c, (void **)&c->mm64, sizeof(double) * c->nuniq)) return 1;
     if (dalloc(c, &c->sig, rs * 6 * c->nn)) return 1;
     if (c->fp32 && c->o.kahan_sigma && dalloc(c, &c->sig_lo, rs * 6 * c->nn)) return 1;
@@ -1171,13 +1160,13 @@ int s3d_init(s3d_ctx *c)
                            c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q], c->stage[S3D_A],
                            c->stage[S3D_B], c->stage[S3D_C], (float *)c->lam, (float *)c->mu,
                            (float *)c->tau, (float *)c->rho, (float *)c->ca, (float *)c->cb,
-                           (float *)c->cc, c->nel, c->NP);
+                           (float *)c->cc, c->nel, c->NP, c->g.dt);
     else
         hipLaunchKernelGGL(material_kernel<double>, dim3(gm), dim3(256), 0, c->stream, c->stage[S3D_RHOINV],
                            c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q], c->stage[S3D_A],
                            c->stage[S3D_B], c->stage[S3D_C], (double *)c->lam, (double *)c->mu,
                            (double *)c->tau, (double *)c->rho, (double *)c->ca, (double *)c->cb,
-                           (double *)c->cc, c->nel, c->NP);
+                           (double *)c->cc, c->nel, c->NP, c->g.dt);
     HIPCHK(hipGetLastError());
     int rc = c->fp32 ? with_p<MassRun, float>(c->p, c, (const double *)d_sin, (const double *)d_r, (const double *)d_w)
                      : with_p<MassRun, double>(c->p, c, (const double *)d_sin, (const double *)d_r, (const double *)d_w);

@@ -10,8 +10,18 @@
 //     element block b "owns" the p^3 nodes with local indices 0..p-1; the
 //     padding blocks bx = nx (by = ny, bz = nz) hold the last node plane.
 //     Global node (gx, gy, gz) = (bx*p + li, by*p + lj, bz*p + lk).
-//   1-D tables: per (element, GLL index) along each axis, FP64 computed,
-//     rounded once to the step precision.
+//   1-D tables: per (element, GLL index) along each axis, FP64.
+//
+// Precision split of the FP32 step: the state (velocity, stress, memory
+// variables), the material and the element force are stored and updated in
+// FP32, but every operator coefficient that is the same for many nodes (the
+// GLL derivative matrix and weights, the 2/dx Jacobian factors, the
+// spherical tables) is held in FP64 and applied in FP64 registers. An FP32
+// rounding of such a shared coefficient is a systematic error (a wave-speed or
+// time-axis scaling that grows into a phase error with simulated time);
+// rounding data-dependent results is not. The time step is folded into the
+// per-node coefficients (dt*lambda, dt*mu, dt*A/B/C, dt/M) in FP64 before
+// their one FP32 rounding, for the same reason.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,13 +35,17 @@ constexpr int NRDISS = 3;      // relaxation mechanisms
 // strain-rate / stress component order used everywhere: xx xy xz yy yz zz
 enum { XX = 0, XY, XZ, YY, YZ, ZZ };
 
-template <typename R>
+// Precision of the operator coefficients and of the sums they enter.
+template <typename R> struct CoefType { using T = double; };
+template <typename R> using coef_t = typename CoefType<R>::T;
+
+template <typename C>
 struct ThTab {                 // per (ex, i): colatitude dependent factors
-    R sin_t, inv_sin, cot, pad;
+    C sin_t, inv_sin, cot, pad;
 };
-template <typename R>
+template <typename C>
 struct RTab {                  // per (ez, k): radius dependent factors
-    R r, inv_r, r2, pad;
+    C r, inv_r, r2, pad;
 };
 
 template <typename R>
@@ -46,11 +60,13 @@ struct ElemArgs {
     R *sig;                    // [6][nel][NP]
     R *sig_lo;                 // [6][nel][NP] compensation or null
     R *mem;                    // [18][nel][NP] (visco) or null
-    const R *lam, *mu, *tau;   // [nel][NP]
-    const R *ca, *cb, *cc;     // anisotropy A, B, C or null
+    // [nel][NP] moduli times dt: elastic lam, mu; visco-elastic
+    // lam - 2/3 tau mu and mu (1 + tau) (relaxed-to-unrelaxed form)
+    const R *lam, *mu, *tau;
+    const R *ca, *cb, *cc;     // anisotropy dt*A, dt*B, dt*C or null
     R *force;                  // [3][nel][NP]
-    const ThTab<R> *tth;       // [nx*P1]
-    const RTab<R> *tr;         // [nz*P1]
+    const ThTab<coef_t<R>> *tth;   // [nx*P1]
+    const RTab<coef_t<R>> *tr;     // [nz*P1]
     const double *px, *py, *pz;  // 1-D taper profiles [n*P1]
     double alpha;              // taper slope
     int taper_on;
@@ -58,15 +74,13 @@ struct ElemArgs {
     const int *src_chan;       // [nslot][9] channel row in stf, -1 = none
     const double *src_coef;    // [nslot][9][NP]
     const double *stf;         // [nchan][nt]
-    R dt;
-    R d1, d2, d3;              // (2/dx)J, (2/dy)J, -(2/dz)J
-    R mem_a[NRDISS];           // 1 - dt/tau_p
-    R mem_c[NRDISS];           // dt/(3 tau_p)
-    R w[MAXP1];                // GLL weights
-    R Dx[MAXP1 * MAXP1];       // Dx[i*P1+q] = 2 l_q'(xi_i) / dx
-    R Dy[MAXP1 * MAXP1];
-    R Dz[MAXP1 * MAXP1];
-    R dl[MAXP1 * MAXP1];       // dl[i*P1+q] = l_i'(xi_q)
+    coef_t<R> d1, d2, d3;      // (2/dx)J, (2/dy)J, -(2/dz)J
+    R rate[NRDISS];            // dt/tau_p
+    coef_t<R> w[MAXP1];        // GLL weights
+    coef_t<R> Dx[MAXP1 * MAXP1];  // Dx[i*P1+q] = 2 l_q'(xi_i) / dx
+    coef_t<R> Dy[MAXP1 * MAXP1];
+    coef_t<R> Dz[MAXP1 * MAXP1];
+    coef_t<R> dlw[MAXP1 * MAXP1]; // dlw[i*P1+q] = l_i'(xi_q) w_q
 };
 
 template <typename R>
@@ -76,11 +90,10 @@ struct NodeArgs {
     R *vx, *vy, *vz;
     R *lx, *ly, *lz;           // compensation terms or null
     const R *force;            // [3][nel][NP]
-    const R *inv_mm;           // unique
+    const R *dt_inv_mm;        // unique, dt / M
     const double *px, *py, *pz;
     double alpha;
     int taper_on;
-    R dt;
 };
 
 // taper = exp(-0.1 * prof^2) after corner normalisation
@@ -97,6 +110,22 @@ __host__ __device__ inline double taper_value(double px, double py, double pz,
     double px2 = 2.0 * px / den;
     double p2 = px2 + py2 + pz2;
     return exp(-((double)0.1f * (p2 * p2)));
+}
+
+// 1 - taper, from the same profile: the kernels apply the taper as
+// f - f * damp. A factor such as 0.9999 rounded to FP32 keeps only a few
+// digits of its distance from 1, and the taper acts every step, so the
+// damping is carried to full relative precision instead.
+__host__ __device__ inline double taper_damp(double px, double py, double pz, double alpha)
+{
+    double prof = px + py + pz;
+    if (prof == 0.0) return 0.0;
+    double den = 1.0 + prof / alpha;
+    double pz2 = 2.0 * pz / den;
+    double py2 = 2.0 * py / den;
+    double px2 = 2.0 * px / den;
+    double p2 = px2 + py2 + pz2;
+    return -expm1(-((double)0.1f * (p2 * p2)));
 }
 
 }  // namespace s3d

@@ -74,8 +74,11 @@ template <int P, int E, typename R, bool VISCO>
 __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
     using S = Shape<P, E>;
+    using C = coef_t<R>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
     __shared__ R lds[6][EPB * NP];
+    __shared__ C cf[4][P1 * P1];     // Dx, Dy, Dz, dl*w
+    __shared__ C geo[EPB][2 * P1];   // sin(theta), r^2 at the element's GLL points
 
     const int t = threadIdx.x;
     const int slot = t / NP;
@@ -91,13 +94,17 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     const long eb = (long)ee * NP + node;  // element-local address
 
     // ---- velocity gather (unique nodes -> LDS) ----
-    R vx = 0, vy = 0, vz = 0;
-    if (valid) {
-        long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
-        vx = a.vx[u];
-        vy = a.vy[u];
-        vz = a.vz[u];
+    // Lanes without an element (the tail of the last workgroup) read element
+    // 0's addresses: every load below is unconditional and in bounds, so the
+    // loads issue back to back; such lanes store nothing.
+    const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
+    const R vx = a.vx[u], vy = a.vy[u], vz = a.vz[u];
+    for (int x = t; x < 4 * P1 * P1; x += S::NT) {
+        const int m = x / (P1 * P1), y = x - m * P1 * P1;
+        cf[m][y] = m == 0 ? a.Dx[y] : m == 1 ? a.Dy[y] : m == 2 ? a.Dz[y] : a.dlw[y];
     }
+    if (inside && node < 2 * P1)
+        geo[slot][node] = node < P1 ? a.tth[ex * P1 + node].sin_t : a.tr[ez * P1 + node - P1].r2;
     if (inside) {
         lds[0][L + node] = vx;
         lds[1][L + node] = vy;
@@ -106,58 +113,62 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     // ---- state loads (issued before the barrier to hide latency) ----
     R sg[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) sg[c] = valid ? a.sig[c * fs + eb] : R(0);
-    R lam = valid ? a.lam[eb] : R(0);
-    R mu = valid ? a.mu[eb] : R(0);
+    for (int c = 0; c < 6; ++c) sg[c] = a.sig[c * fs + eb];
+    R lam = a.lam[eb];
+    R mu = a.mu[eb];
     R tau = R(0);
     R M[VISCO ? 18 : 1];
     if constexpr (VISCO) {
-        tau = valid ? a.tau[eb] : R(0);
+        tau = a.tau[eb];
 #pragma unroll
-        for (int c = 0; c < 18; ++c) M[c] = valid ? a.mem[c * fs + eb] : R(0);
+        for (int c = 0; c < 18; ++c) M[c] = a.mem[c * fs + eb];
     }
     __syncthreads();
 
     // ---- K1: tensor-product derivatives along theta, phi, r ----
-    R dxv[3] = {0, 0, 0}, dyv[3] = {0, 0, 0}, dzv[3] = {0, 0, 0};
+    C dxv[3] = {0, 0, 0}, dyv[3] = {0, 0, 0}, dzv[3] = {0, 0, 0};
     {
-        const R *Dxi = a.Dx + i * P1;
-        const R *Dyj = a.Dy + j * P1;
-        const R *Dzk = a.Dz + k * P1;
+        const C *Dxi = cf[0] + i * P1;
+        const C *Dyj = cf[1] + j * P1;
+        const C *Dzk = cf[2] + k * P1;
 #pragma unroll
         for (int q = 0; q < P1; ++q) {
             const int nxq = L + q + P1 * (j + P1 * k);
             const int nyq = L + i + P1 * (q + P1 * k);
             const int nzq = L + i + P1 * (j + P1 * q);
-            const R cx = Dxi[q], cy = Dyj[q], cz = Dzk[q];
+            const C cx = Dxi[q], cy = Dyj[q], cz = Dzk[q];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                dxv[c] += lds[c][nxq] * cx;
-                dyv[c] += lds[c][nyq] * cy;
-                dzv[c] += lds[c][nzq] * cz;
+                dxv[c] += (C)lds[c][nxq] * cx;
+                dyv[c] += (C)lds[c][nyq] * cy;
+                dzv[c] += (C)lds[c][nzq] * cz;
             }
         }
     }
 
     // ---- K2: spherical completion, K4: symmetrisation ----
-    const ThTab<R> th = a.tth[ex * P1 + i];
-    const RTab<R> rr = a.tr[ez * P1 + k];
-    const R inv_r = rr.inv_r;
-    const R inv_rs = inv_r * th.inv_sin;
-    const R exx = (vz + dxv[0]) * inv_r;
-    const R eyy = dyv[1] * inv_rs + (vz + vx * th.cot) * inv_r;
-    const R ezz = -dzv[2];
-    const R ezx = -dzv[0];
-    const R exz_c = (dxv[2] - vx) * inv_r;
-    const R ezy = -dzv[1];
-    const R eyz_c = -vy * inv_r + dyv[2] * inv_rs;
-    const R exy_c = dxv[1] * inv_r;
-    const R eyx = -vy * th.cot * inv_r + dyv[0] * inv_rs;
-    const R exy = (exy_c + eyx) * R(0.5);
-    const R exz = (exz_c + ezx) * R(0.5);
-    const R eyz = (eyz_c + ezy) * R(0.5);
+    const ThTab<C> th = a.tth[ex * P1 + i];
+    const RTab<C> rr = a.tr[ez * P1 + k];
+    R exx, eyy, ezz, exy, exz, eyz;
+    {
+        const C ux = vx, uy = vy, uz = vz;
+        const C inv_r = rr.inv_r;
+        const C inv_rs = inv_r * th.inv_sin;
+        const C ezx = -dzv[0];
+        const C exz_c = (dxv[2] - ux) * inv_r;
+        const C ezy = -dzv[1];
+        const C eyz_c = -uy * inv_r + dyv[2] * inv_rs;
+        const C exy_c = dxv[1] * inv_r;
+        const C eyx = -uy * th.cot * inv_r + dyv[0] * inv_rs;
+        exx = (R)((uz + dxv[0]) * inv_r);
+        eyy = (R)(dyv[1] * inv_rs + (uz + ux * th.cot) * inv_r);
+        ezz = (R)(-dzv[2]);
+        exy = (R)((exy_c + eyx) * 0.5);
+        exz = (R)((exz_c + ezx) * 0.5);
+        eyz = (R)((eyz_c + ezy) * 0.5);
+    }
 
-    // ---- K5: stress rate ----
+    // ---- K5: stress rate times dt (lam, mu, A, B, C carry the dt) ----
     R rt[6];
     const R tmu = R(2) * mu;
     if constexpr (!VISCO) {
@@ -174,25 +185,27 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 #pragma unroll
         for (int c = 0; c < 6; ++c) Md[c] = M[c] + M[6 + c] + M[12 + c];
         const R Msum = Md[XX] + Md[YY] + Md[ZZ];
-        const R tmut = tmu * (R(1) + tau);                     // 2 mu_tau
-        const R lk = lam - R(2.0 / 3.0) * tau * mu;            // kappa - 2 mu_tau / 3
-        const R Lr = lk * (exx + eyy + ezz) - tmu * Msum * R(1.0 / 3.0);
-        rt[XX] = Lr + tmut * exx + tmu * Md[XX];
-        rt[YY] = Lr + tmut * eyy + tmu * Md[YY];
-        rt[ZZ] = Lr + tmut * ezz + tmu * Md[ZZ];
-        rt[XY] = tmut * exy + tmu * Md[XY];
-        rt[XZ] = tmut * exz + tmu * Md[XZ];
-        rt[YZ] = tmut * eyz + tmu * Md[YZ];
+        // lam holds kappa - 2 mu_tau / 3 and mu holds mu_tau = mu (1 + tau),
+        // both formed in FP64; 2 mu itself only scales the memory variables
+        const R tmut = tmu;                                    // 2 mu_tau
+        const R tm = tmut / (R(1) + tau);                      // 2 mu
+        const R Lr = lam * (exx + eyy + ezz) - tm * Msum * R(1.0 / 3.0);
+        rt[XX] = Lr + tmut * exx + tm * Md[XX];
+        rt[YY] = Lr + tmut * eyy + tm * Md[YY];
+        rt[ZZ] = Lr + tmut * ezz + tm * Md[ZZ];
+        rt[XY] = tmut * exy + tm * Md[XY];
+        rt[XZ] = tmut * exz + tm * Md[XZ];
+        rt[YZ] = tmut * eyz + tm * Md[YZ];
     }
     if (a.ca) {   // anisotropy (src/evolution.f90:290-296)
-        const R A = valid ? a.ca[eb] : R(0);
-        const R B = valid ? a.cb[eb] : R(0);
-        const R C = valid ? a.cc[eb] : R(0);
-        rt[XX] += C * ezz + A * (eyy + exx);
-        rt[YY] += C * ezz + A * (eyy + exx);
-        rt[ZZ] += C * (eyy + exx);
-        rt[XZ] += R(2) * B * exz;
-        rt[YZ] += R(2) * B * eyz;
+        const R pa = a.ca[eb];
+        const R pb = a.cb[eb];
+        const R pc = a.cc[eb];
+        rt[XX] += pc * ezz + pa * (eyy + exx);
+        rt[YY] += pc * ezz + pa * (eyy + exx);
+        rt[ZZ] += pc * (eyy + exx);
+        rt[XZ] += R(2) * pb * exz;
+        rt[YZ] += R(2) * pb * eyz;
     }
 
     // ---- K6: integrate the stress ----
@@ -200,12 +213,12 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     if (a.kahan_sigma) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            lo[c] = valid ? a.sig_lo[c * fs + eb] : R(0);
-            kahan_add(sg[c], lo[c], a.dt * rt[c]);
+            lo[c] = a.sig_lo[c * fs + eb];
+            kahan_add(sg[c], lo[c], rt[c]);
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) sg[c] = sg[c] + a.dt * rt[c];
+        for (int c = 0; c < 6; ++c) sg[c] = sg[c] + rt[c];
     }
 
     // ---- K7: moment-tensor glut and single force of the previous add() ----
@@ -235,73 +248,77 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     // ---- memory variables (loop form of src/evolution.f90:415-442) ----
     if constexpr (VISCO) {
         const R ev[6] = {exx, exy, exz, eyy, eyz, ezz};
+        // M (1 - dt/tau_p) - dt tau / (3 tau_p) e  ==  M - dt/tau_p (M + tau/3 e):
+        // the decrement form keeps the FP32 relaxation rate exact to one
+        // rounding of dt/tau_p (1 - dt/tau_p would lose its low bits)
+        const R t3 = tau * R(1.0 / 3.0);
 #pragma unroll
         for (int m = 0; m < NRDISS; ++m) {
-            const R cm = a.mem_c[m] * tau;
 #pragma unroll
-            for (int c = 0; c < 6; ++c) M[6 * m + c] = M[6 * m + c] * a.mem_a[m] - cm * ev[c];
+            for (int c = 0; c < 6; ++c) M[6 * m + c] -= a.rate[m] * (M[6 * m + c] + t3 * ev[c]);
         }
     }
 
     // ---- taper factor of this node (K13) ----
-    R tp = R(1);
-    if (a.taper_on && valid) {
-        tp = (R)taper_value(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
+    R dmp = R(0);   // 1 - taper
+    if (a.taper_on) {
+        dmp = (R)taper_damp(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
     }
 
     __syncthreads();
 
     // ---- K8: weak divergence ----
-    R ax[P1], ay[P1], az[P1];
+    C ax[P1], ay[P1], az[P1];
     {
-        const R *dli = a.dl + i * P1;
-        const R *dlj = a.dl + j * P1;
-        const R *dlk = a.dl + k * P1;
+        const C *dli = cf[3] + i * P1;
+        const C *dlj = cf[3] + j * P1;
+        const C *dlk = cf[3] + k * P1;
+        const C *g = geo[inside ? slot : 0];
 #pragma unroll
         for (int q = 0; q < P1; ++q) {
-            ax[q] = dli[q] * a.w[q] * a.tth[ex * P1 + q].sin_t;
-            ay[q] = dlj[q] * a.w[q];
-            az[q] = dlk[q] * a.w[q] * a.tr[ez * P1 + q].r2;
+            ax[q] = dli[q] * g[q];
+            ay[q] = dlj[q];
+            az[q] = dlk[q] * g[P1 + q];
         }
     }
-    R Sx[3] = {0, 0, 0}, Sy[3] = {0, 0, 0}, Sz[3] = {0, 0, 0};
+    C Sx[3] = {0, 0, 0}, Sy[3] = {0, 0, 0}, Sz[3] = {0, 0, 0};
 #pragma unroll
     for (int q = 0; q < P1; ++q) {
         const int nxq = L + q + P1 * (j + P1 * k);
         const int nyq = L + i + P1 * (q + P1 * k);
         const int nzq = L + i + P1 * (j + P1 * q);
         // x contraction: sxx, sxy, sxz feed sx, sy, sz
-        Sx[0] += ax[q] * lds[XX][nxq];
-        Sx[1] += ax[q] * lds[XY][nxq];
-        Sx[2] += ax[q] * lds[XZ][nxq];
+        Sx[0] += ax[q] * (C)lds[XX][nxq];
+        Sx[1] += ax[q] * (C)lds[XY][nxq];
+        Sx[2] += ax[q] * (C)lds[XZ][nxq];
         // y contraction: sxy, syy, syz
-        Sy[0] += ay[q] * lds[XY][nyq];
-        Sy[1] += ay[q] * lds[YY][nyq];
-        Sy[2] += ay[q] * lds[YZ][nyq];
+        Sy[0] += ay[q] * (C)lds[XY][nyq];
+        Sy[1] += ay[q] * (C)lds[YY][nyq];
+        Sy[2] += ay[q] * (C)lds[YZ][nyq];
         // z contraction: sxz, syz, szz
-        Sz[0] += az[q] * lds[XZ][nzq];
-        Sz[1] += az[q] * lds[YZ][nzq];
-        Sz[2] += az[q] * lds[ZZ][nzq];
+        Sz[0] += az[q] * (C)lds[XZ][nzq];
+        Sz[1] += az[q] * (C)lds[YZ][nzq];
+        Sz[2] += az[q] * (C)lds[ZZ][nzq];
     }
-    const R Cx = a.w[j] * a.w[k] * rr.r * a.d1;
-    const R Cy = a.w[i] * a.w[k] * rr.r * a.d2;
-    const R Cz = a.w[i] * a.w[j] * th.sin_t * a.d3;
+    const C Cx = a.w[j] * a.w[k] * rr.r * a.d1;
+    const C Cy = a.w[i] * a.w[k] * rr.r * a.d2;
+    const C Cz = a.w[i] * a.w[j] * th.sin_t * a.d3;
 
     if (!valid) return;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-        a.force[c * fs + eb] = src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c];
+        a.force[c * fs + eb] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
 
     // ---- store tapered stress (K13) and memory variables ----
     if (a.kahan_sigma) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            a.sig[c * fs + eb] = sg[c] * tp;
-            a.sig_lo[c * fs + eb] = lo[c] * tp;
+            a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
+            a.sig_lo[c * fs + eb] = lo[c] - lo[c] * dmp;
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] * tp;
+        for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
     }
     if constexpr (VISCO) {
 #pragma unroll
@@ -393,13 +410,13 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
         acc[c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
     }
     const long u = tid;
-    const R im = a.inv_mm[u];
-    R tp = R(1);
+    const R im = a.dt_inv_mm[u];
+    R dmp = R(0);   // 1 - taper
     if (a.taper_on) {
         const int ex = bx < a.nx ? bx : a.nx - 1, i = bx < a.nx ? li : P;
         const int ey = by < a.ny ? by : a.ny - 1, j = by < a.ny ? lj : P;
         const int ez = bz < a.nz ? bz : a.nz - 1, k = bz < a.nz ? lk : P;
-        tp = (R)taper_value(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
+        dmp = (R)taper_damp(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
     }
     R *v[3] = {a.vx, a.vy, a.vz};
     if constexpr (KV) {
@@ -407,13 +424,16 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             R s = v[c][u], cc = l[c][u];
-            kahan_add(s, cc, a.dt * (acc[c] * im));
-            v[c][u] = s * tp;
-            l[c][u] = cc * tp;
+            kahan_add(s, cc, acc[c] * im);
+            v[c][u] = s - s * dmp;
+            l[c][u] = cc - cc * dmp;
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) v[c][u] = (v[c][u] + a.dt * (acc[c] * im)) * tp;
+        for (int c = 0; c < 3; ++c) {
+            const R s = v[c][u] + acc[c] * im;
+            v[c][u] = s - s * dmp;
+        }
     }
 }
 
@@ -586,39 +606,50 @@ __global__ void gather_unique(const double *src, R *dst, R *lo, int nx, int ny, 
     if (lo) lo[t] = (R)(v - (double)(R)v) * R(-1);
 }
 
-// Material: lambda, mu, tau (= 6.0814 Q^-1.015, src/model_paras.F90:407),
-// rho from the FP64 staging arrays in reference layout.
+// Material from the FP64 staging arrays in reference layout: tau
+// (= 6.0814 Q^-1.015, src/model_paras.F90:407), rho, dt*A, dt*B, dt*C and the
+// moduli times dt, elastic: lambda, mu; visco-elastic (dtau non-null):
+// kappa - 2 mu_tau / 3 = lambda - 2/3 tau mu and mu_tau = mu (1 + tau)
+// (src/evolution.f90:216-284). All formed in FP64 and rounded once: a
+// constant tau makes 1 + tau the same number in every node of a region, and
+// an FP32 rounding of it would be a systematic modulus error.
 template <typename R>
 __global__ void material_kernel(const double *rhoinv, const double *lam, const double *mu,
                                 const double *q, const double *A, const double *B, const double *C,
                                 R *dlam, R *dmu, R *dtau, R *drho, R *dA, R *dB, R *dC,
-                                long nel, int NP)
+                                long nel, int NP, double dt)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nel * NP) return;
     long e = t / NP;
     int n = (int)(t - e * NP);
     long s = (long)n * nel + e;
-    dlam[t] = lam ? (R)lam[s] : R(0);
-    dmu[t] = mu ? (R)mu[s] : R(0);
+    const double l = lam ? lam[s] : 0.0, m = mu ? mu[s] : 0.0;
     if (dtau) {
         double qq = q ? q[s] : 1.79769313486231570815e+308;
-        dtau[t] = (R)((double)6.0814f * pow(qq, (double)-1.015f));
+        const double tau = (double)6.0814f * pow(qq, (double)-1.015f);
+        dtau[t] = (R)tau;
+        dlam[t] = (R)(dt * (l - (2.0 / 3.0) * tau * m));
+        dmu[t] = (R)(dt * (m * (1.0 + tau)));
+    } else {
+        dlam[t] = (R)(dt * l);
+        dmu[t] = (R)(dt * m);
     }
     double ri = rhoinv ? rhoinv[s] : 0.0;
     drho[t] = (R)(ri > 0.0 ? 1.0 / ri : 0.0);
     if (dA) {
-        dA[t] = A ? (R)A[s] : R(0);
-        dB[t] = B ? (R)B[s] : R(0);
-        dC[t] = C ? (R)C[s] : R(0);
+        dA[t] = A ? (R)(dt * A[s]) : R(0);
+        dB[t] = B ? (R)(dt * B[s]) : R(0);
+        dC[t] = C ? (R)(dt * C[s]) : R(0);
     }
 }
 
 // Mass matrix -J r^2 sin(theta) rho W per copy (src/model_paras.F90:411),
 // assembled with the same theta -> phi -> r summation tree as the force.
+// Stores M (FP64) and dt/M (step precision).
 template <int P, typename R>
 __global__ void mass_kernel(const double *rhoinv, const double *sin_t, const double *rad,
-                            const double *w, double jac, double *mm64, R *inv_mm,
+                            const double *w, double jac, double dt, double *mm64, R *dt_inv_mm,
                             int nx, int ny, int nz, long nblk)
 {
     constexpr int P1 = P + 1, PB = P * P * P;
@@ -630,7 +661,7 @@ __global__ void mass_kernel(const double *rhoinv, const double *sin_t, const dou
     int nbx = nx + 1, nby = ny + 1;
     int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
     if ((bx == nx && li) || (by == ny && lj) || (bz == nz && lk)) {
-        inv_mm[t] = R(0);
+        dt_inv_mm[t] = R(0);
         mm64[t] = 0.0;
         return;
     }
@@ -669,7 +700,7 @@ __global__ void mass_kernel(const double *rhoinv, const double *sin_t, const dou
         zhas = true;
     }
     mm64[t] = zsum;
-    inv_mm[t] = (R)(zsum != 0.0 ? 1.0 / zsum : 0.0);
+    dt_inv_mm[t] = (R)(zsum != 0.0 ? dt / zsum : 0.0);
 }
 
 // Courant number partial maxima (src/model_paras.F90:529-561), FP64.
@@ -747,7 +778,7 @@ __global__ void vrange_kernel(const R *vx, const R *vy, const R *vz, int nx, int
 
 template <int P, typename R>
 __global__ void kinetic_kernel(const R *vx, const R *vy, const R *vz, const R *rho,
-                               const ThTab<R> *tth, const RTab<R> *tr, const R *w,
+                               const ThTab<coef_t<R>> *tth, const RTab<coef_t<R>> *tr, const double *w,
                                int nx, int ny, int nz, double *partial)
 {
     constexpr int P1 = P + 1, NP = P1 * P1 * P1;
@@ -761,9 +792,9 @@ __global__ void kinetic_kernel(const R *vx, const R *vy, const R *vz, const R *r
         int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((long)nx * ny));
         long u = unique_index<P>(ex, ey, ez, i, j, k, nx, ny);
         double a0 = vx[u], a1 = vy[u], a2 = vz[u];
-        double r = tr[ez * P1 + k].r;
+        double r = (double)tr[ez * P1 + k].r;
         acc = (a0 * a0 + a1 * a1 + a2 * a2) * (r * r * (double)tth[ex * P1 + i].sin_t) *
-              (double)rho[t] * ((double)w[i] * w[j] * w[k]);
+              (double)rho[t] * (w[i] * w[j] * w[k]);
     }
     __shared__ double s0[256];
     s0[threadIdx.x] = acc;
