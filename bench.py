@@ -36,26 +36,64 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def step_bytes(s_state, s_static, u, visco, kahan_v, kahan_sigma, L=3, S=1):
+# Precision modes: Simulation options and the element sizes of the words
+# each one moves (state, velocity compensation, stress compensation, static).
+MODES = {
+    "fp64": dict(precision="fp64", kahan_v=False, kahan_sigma=0, bf16_static=False),
+    "fp32": dict(precision="fp32", kahan_v=False, kahan_sigma=0, bf16_static=False),
+    "fp32+kahan_v": dict(precision="fp32", kahan_v=True, kahan_sigma=0, bf16_static=False),
+    "fp32+kahan_v+kahan_sigma": dict(precision="fp32", kahan_v=True, kahan_sigma=1,
+                                     bf16_static=False),
+    "fp32+kahan_v+kahan_sigma_bf16": dict(precision="fp32", kahan_v=True, kahan_sigma=2,
+                                          bf16_static=False),
+    "fp32+kahan_v+bf16_static": dict(precision="fp32", kahan_v=True, kahan_sigma=0,
+                                     bf16_static=True),
+}
+
+
+MODE_TEXT = {
+    "fp64": "FP64",
+    "fp32": "FP32",
+    "fp32+kahan_v": "FP32 with compensated velocity integration",
+    "fp32+kahan_v+kahan_sigma": "FP32 with compensated velocity and stress integration",
+    "fp32+kahan_v+kahan_sigma_bf16": "FP32 with compensated velocity integration and "
+                                     "BF16-compensated stress integration",
+    "fp32+kahan_v+bf16_static": "FP32 with compensated velocity integration and BF16 "
+                                "moduli / 1/M",
+}
+DEFAULT_MODE = "fp32+kahan_v+kahan_sigma_bf16"
+
+
+def mode_sizes(mode):
+    """(state, velocity compensation, stress compensation, static) bytes."""
+    m = MODES[mode]
+    s = 8 if m["precision"] == "fp64" else 4
+    return (s, s if m["kahan_v"] else 0, {0: 0, 1: 4, 2: 2}[m["kahan_sigma"]],
+            2 if m["bf16_static"] else s)
+
+
+def step_bytes(mode, u, visco, L=3, S=1):
     """Algorithmic HBM bytes per element-local node per step:
-    s_state*[12 + 12 L visco + (6 + 6 Kv) u + 12 Ks] + s_static*[(2+visco) + u]/S"""
-    return (s_state * (12 + 12 * L * visco + (6 + 6 * kahan_v) * u + 12 * kahan_sigma)
-            + s_static * ((2 + visco) + u) / S)
+    s_state*[12 + 12 L visco + 6 u] + s_lo_v*6 u + s_lo_sigma*12
+    + s_static*[(2 + visco) + u]/S"""
+    s, lv, ls, st = mode_sizes(mode)
+    return s * (12 + 12 * L * visco + 6 * u) + lv * 6 * u + ls * 12 + st * ((2 + visco) + u) / S
 
 
-def elem_kernel_bytes(s, u, visco, kahan_sigma):
+def elem_kernel_bytes(mode, u, visco):
     """Algorithmic bytes per element-local node of one element-kernel launch:
-    stress r+w (12 words, +12 compensation), memory variables r+w (36 words),
-    lambda, mu (+tau) reads, the velocity read once per unique node (3u words)
-    and the element force write (3 words)."""
-    return s * (12 + 12 * kahan_sigma + 36 * visco + (2 + visco) + 3 * u + 3)
+    stress r+w (12 words, +12 compensation words), memory variables r+w
+    (36 words), lambda, mu (+tau) reads, the velocity read once per unique
+    node (3u words) and the element force write (3 words)."""
+    s, _, ls, st = mode_sizes(mode)
+    return s * (12 + 36 * visco + 3 * u + 3) + ls * 12 + st * (2 + visco)
 
 
-def load_sims(case, variants, slab, remap=True):
-    """Build one Simulation per (precision, kahan_v) variant from a single
-    pass over the model slabs; returns a list of (name, sim)."""
+def load_sims(case, modes, slab, remap=True):
+    """Build one Simulation per precision mode from a single pass over the
+    model slabs; returns a list of (mode, sim)."""
     g = case.grid
-    sims = [(name, api.Simulation(g, prec, kahan_v=kv, xcd_remap=remap)) for name, prec, kv in variants]
+    sims = [(name, api.Simulation(g, xcd_remap=remap, **MODES[name])) for name in modes]
     t0 = time.time()
     for ez0 in range(0, g.nz, slab):
         nez = min(slab, g.nz - ez0)
@@ -155,16 +193,18 @@ def precision_study(args):
     case = synth.layered(seed=1, nt=args.study_steps, nx=args.nx, ny=args.ny, nz=args.nz,
                          n_rec=args.study_receivers)
     g = case.grid
-    variants = [("fp64", "fp64", False, False), ("fp32+kahan_v", "fp32", True, False),
-                ("fp32", "fp32", False, False), ("fp32+kahan_v+kahan_sigma", "fp32", True, True)]
+    variants = ["fp64"] + [m for m in MODES if m != "fp64"]
     if args.study_variants:
-        keep = set(args.study_variants.split(","))
-        variants = [v for v in variants if v[0] in keep or v[0] == "fp64"]
+        keep = args.study_variants.split(",")
+        for m in keep:
+            if m not in MODES:
+                raise SystemExit("unknown precision mode %r (known: %s)" % (m, ", ".join(MODES)))
+        variants = ["fp64"] + [m for m in keep if m != "fp64"]
     res = {}
     ref_tr = ref_snap = None
-    for name, prec, kv, ks in variants:
+    for name in variants:
         t0 = time.time()
-        sim, recs = api.build_case(case, precision=prec, kahan_v=kv, kahan_sigma=ks, slab=args.slab)
+        sim, recs = api.build_case(case, slab=args.slab, **MODES[name])
         chunk = 500
         done = 0
         while done < g.nt:
@@ -240,6 +280,8 @@ def main():
     ap.add_argument("--ny", type=int, default=192)
     ap.add_argument("--nz", type=int, default=96)
     ap.add_argument("--slab", type=int, default=4)
+    ap.add_argument("--mode", default=DEFAULT_MODE, choices=list(MODES),
+                    help="precision mode of the timed simulation")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-remap", action="store_true", help="plain workgroup order (no XCD remap)")
@@ -279,26 +321,28 @@ def main():
     log("[bench] mesh %dx%dx%d p=%d: %d element-local nodes, u=%.4f, dt=%.4f s"
         % (g.nx, g.ny, g.nz, g.lpd, N, u, g.dt))
 
-    variants = [("fp32", "fp32", True)] + ([] if args.no_fp64 else [("fp64", "fp64", False)])
-    sims = dict(load_sims(case, variants, args.slab, remap=not args.no_remap))
-    wall, ms, cnt, vr = time_gpu(sims["fp32"], args.steps, args.warmup)
+    mode = args.mode
+    modes = [mode] + ([] if args.no_fp64 or mode == "fp64" else ["fp64"])
+    sims = dict(load_sims(case, modes, args.slab, remap=not args.no_remap))
+    wall, ms, cnt, vr = time_gpu(sims[mode], args.steps, args.warmup)
     value = N * args.steps / wall
-    log("[bench] fp32 %.3f ms/step, elem %.3f ms, node %.3f ms, v range %s"
-        % (1e3 * wall / args.steps, ms[0] / cnt[0], ms[1] / cnt[1], vr))
+    log("[bench] %s %.3f ms/step, elem %.3f ms, node %.3f ms, v range %s"
+        % (mode, 1e3 * wall / args.steps, ms[0] / cnt[0], ms[1] / cnt[1], vr))
 
     elem_ms = ms[0] / max(cnt[0], 1)
-    eb = elem_kernel_bytes(4, u, 1, 0) * N
+    eb = elem_kernel_bytes(mode, u, 1) * N
     achieved = eb / (elem_ms * 1e-3)
-    Bstep = step_bytes(4, 4, u, 1, 1, 0)
+    Bstep = step_bytes(mode, u, 1)
+    rtype = "double" if mode == "fp64" else "float"
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": None,
-            "kernel": "elem_kernel<4,float,visco>",
+            "kernel": "elem_kernel<4,%s,visco>" % rtype,
             "algorithmic_bytes_per_launch": eb}
     roof["achieved"] = achieved / 1e9
     roof["peak"] = HBM_PEAK / 1e9
     # HBM traffic per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE,
     # calibrated on a plain copy kernel) for this workload, when recorded
-    tf = os.path.join(ROOT, "profiles", "traffic_layered_192x192x96_p4_fp32.json")
+    tf = os.path.join(ROOT, "profiles", "traffic_layered_192x192x96_p4_%s.json" % mode)
     if (g.nx, g.ny, g.nz) == (192, 192, 96) and os.path.exists(tf):
         roof["traffic"] = json.load(open(tf))["traffic_bytes_per_launch"]["elem_kernel"]
     step_frac = Bstep * value / HBM_PEAK
@@ -308,20 +352,20 @@ def main():
         "value": value, "unit": "GLL-node updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if mode == "fp64" else "f32",
         "data": "synthetic (seeded layered visco-elastic model, GRF vs perturbation)",
         "config": {"workload": "layered visco-elastic section 40x40 deg x 1400 km, "
-                               "%dx%dx%d elements, GLL order 4, 3 SLS mechanisms, FP32 with "
-                               "compensated velocity integration" % (g.nx, g.ny, g.nz),
+                               "%dx%dx%d elements, GLL order 4, 3 SLS mechanisms, %s"
+                               % (g.nx, g.ny, g.nz, MODE_TEXT[mode]),
                    "elements": g.n_elem, "gll_nodes": N, "unique_nodes": g.unique_nodes,
-                   "precision": "fp32+kahan_v", "parallelism": "single GPU"},
+                   "precision": mode, "parallelism": "single GPU"},
         "roofline": roof,
         "step_roofline": {"bytes_per_node": Bstep, "achieved_GBps": Bstep * value / 1e9,
                           "frac": step_frac,
                           "elem_ms": elem_ms, "node_ms": ms[1] / max(cnt[1], 1),
                           "record_ms": ms[2] / max(cnt[2], 1)},
     }
-    if not args.no_fp64:
+    if "fp64" in sims and mode != "fp64":
         wall64, ms64, cnt64, _ = time_gpu(sims["fp64"], args.steps, args.warmup)
         v64 = N * args.steps / wall64
         out["fp64_gpu"] = {"value": v64, "ms_per_step": 1e3 * wall64 / args.steps,

@@ -43,9 +43,12 @@ enum { S3D_FP64 = 0, S3D_FP32 = 1 };
 typedef struct s3d_options {
     int precision;         /* S3D_FP64 or S3D_FP32                        */
     int kahan_v;           /* FP32: compensated velocity integration      */
-    int kahan_sigma;       /* FP32: compensated stress integration        */
+    int kahan_sigma;       /* FP32: compensated stress integration,
+                              0 off, 1 FP32 compensation words,
+                              2 BF16 compensation words                   */
     int device;            /* HIP device ordinal                          */
     int xcd_remap;         /* 1: give each XCD a contiguous element range */
+    int bf16_static;       /* FP32: store the moduli, tau and dt/M in BF16 */
 } s3d_options;
 
 /* Model parameter ids (reference src/model_paras.F90:39-46). */

@@ -262,13 +262,21 @@ ElemArgs<R> elem_args(s3d_ctx *c)
     a.remap = c->o.xcd_remap;
     a.it_dev = c->d_it;
     a.nt = c->g.nt;
-    a.kahan_sigma = c->o.kahan_sigma && c->fp32;
+    a.kahan_sigma = c->o.kahan_sigma;
     a.vx = (const R *)c->v[0]; a.vy = (const R *)c->v[1]; a.vz = (const R *)c->v[2];
     a.sig = (R *)c->sig;
-    a.sig_lo = (R *)c->sig_lo;
+    if (c->o.kahan_sigma == 2)
+        a.sig_lo2 = (unsigned *)c->sig_lo;
+    else
+        a.sig_lo = (R *)c->sig_lo;
     a.mem = (R *)c->mem;
-    a.lam = (const R *)c->lam; a.mu = (const R *)c->mu; a.tau = (const R *)c->tau;
-    a.ca = (const R *)c->ca; a.cb = (const R *)c->cb; a.cc = (const R *)c->cc;
+    if (c->o.bf16_static) {
+        a.s16.lam = (const bf16 *)c->lam; a.s16.mu = (const bf16 *)c->mu; a.s16.tau = (const bf16 *)c->tau;
+        a.s16.ca = (const bf16 *)c->ca; a.s16.cb = (const bf16 *)c->cb; a.s16.cc = (const bf16 *)c->cc;
+    } else {
+        a.lam = (const R *)c->lam; a.mu = (const R *)c->mu; a.tau = (const R *)c->tau;
+        a.ca = (const R *)c->ca; a.cb = (const R *)c->cb; a.cc = (const R *)c->cc;
+    }
     a.force = (R *)c->force;
     using C = coef_t<R>;
     a.tth = (const ThTab<C> *)c->tth;
@@ -306,7 +314,10 @@ NodeArgs<R> node_args(s3d_ctx *c)
     a.vx = (R *)c->v[0]; a.vy = (R *)c->v[1]; a.vz = (R *)c->v[2];
     a.lx = (R *)c->lo[0]; a.ly = (R *)c->lo[1]; a.lz = (R *)c->lo[2];
     a.force = (const R *)c->force;
-    a.dt_inv_mm = (const R *)c->dt_inv_mm;
+    if (c->o.bf16_static)
+        a.dt_inv_mm16 = (const bf16 *)c->dt_inv_mm;
+    else
+        a.dt_inv_mm = (const R *)c->dt_inv_mm;
     a.px = c->dpx; a.py = c->dpy; a.pz = c->dpz;
     a.alpha = c->g.taper_slope;
     a.taper_on = c->g.taper_width > 0;
@@ -347,6 +358,16 @@ int launch_elem_e(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st)
 {
     using S = Shape<P, E>;
     const unsigned ge = nblocks(c->nel, S::EPB);
+    if constexpr (std::is_same<R, float>::value && E == 0) {
+        if (c->o.bf16_static) {   // default tile only
+            if (c->visco)
+                hipLaunchKernelGGL((elem_kernel<P, 0, float, true, true>), dim3(ge), dim3(S::NT), 0, st, ea);
+            else
+                hipLaunchKernelGGL((elem_kernel<P, 0, float, false, true>), dim3(ge), dim3(S::NT), 0, st, ea);
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+    }
     if (c->visco)
         hipLaunchKernelGGL((elem_kernel<P, E, R, true>), dim3(ge), dim3(S::NT), 0, st, ea);
     else
@@ -362,6 +383,7 @@ template <int P, typename R>
 int launch_elem(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st, int epb)
 {
     if constexpr (std::is_same<R, float>::value) {
+        if (c->o.bf16_static) epb = 0;   // the BF16-static kernel exists for the default tile
         if (epb == 1) return launch_elem_e<P, 1, R>(c, ea, st);
         if constexpr (tile_ok<P>(2)) if (epb == 2) return launch_elem_e<P, 2, R>(c, ea, st);
         if constexpr (tile_ok<P>(4)) if (epb == 4) return launch_elem_e<P, 4, R>(c, ea, st);
@@ -370,10 +392,10 @@ int launch_elem(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st, int epb)
 }
 
 template <int P>
-std::vector<int> tile_candidates(bool fp32)
+std::vector<int> tile_candidates(bool fp32, bool bf16_static)
 {
     std::vector<int> v = {0};
-    if (!fp32) return v;
+    if (!fp32 || bf16_static) return v;
     for (int e : {1, 2, 4})
         if (e * (P + 1) * (P + 1) * (P + 1) <= 1024 && e != default_epb<P>()) v.push_back(e);
     return v;
@@ -391,7 +413,7 @@ int autotune_p(s3d_ctx *c, int *tile_out, double *ms_out)
     HIPCHK(hipEventCreate(&e1));
     double best = 1e300;
     int best_tile = c->epb, best_remap = c->o.xcd_remap;
-    for (int tile : tile_candidates<P>(c->fp32)) {
+    for (int tile : tile_candidates<P>(c->fp32, c->o.bf16_static)) {
         for (int remap = 0; remap < 2; ++remap) {
             c->o.xcd_remap = remap;
             ElemArgs<R> ea = elem_args<P, R>(c);
@@ -518,10 +540,14 @@ template <int P, typename R>
 struct MassRun {
     static int run(s3d_ctx *c, const double *sin_t, const double *rad, const double *w)
     {
-        hipLaunchKernelGGL((mass_kernel<P, R>), dim3(nblocks(c->nuniq, 256)), dim3(256), 0, c->stream,
-                           c->stage[S3D_RHOINV], sin_t, rad, w, c->jac, c->g.dt, c->mm64,
-                           (R *)c->dt_inv_mm,
-                           c->g.nx, c->g.ny, c->g.nz, c->nblk);
+        if (c->o.bf16_static)
+            hipLaunchKernelGGL((mass_kernel<P, bf16>), dim3(nblocks(c->nuniq, 256)), dim3(256), 0, c->stream,
+                               c->stage[S3D_RHOINV], sin_t, rad, w, c->jac, c->g.dt, c->mm64,
+                               (bf16 *)c->dt_inv_mm, c->g.nx, c->g.ny, c->g.nz, c->nblk);
+        else
+            hipLaunchKernelGGL((mass_kernel<P, R>), dim3(nblocks(c->nuniq, 256)), dim3(256), 0, c->stream,
+                               c->stage[S3D_RHOINV], sin_t, rad, w, c->jac, c->g.dt, c->mm64,
+                               (R *)c->dt_inv_mm, c->g.nx, c->g.ny, c->g.nz, c->nblk);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -959,7 +985,10 @@ int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out)
     d.precision = S3D_FP64;
     c->o = opt ? *opt : d;
     c->fp32 = c->o.precision == S3D_FP32;
-    if (!c->fp32) { c->o.kahan_v = 0; c->o.kahan_sigma = 0; }
+    if (c->o.kahan_sigma < 0 || c->o.kahan_sigma > 2) { delete c; return fail("kahan_sigma must be 0, 1 or 2"); }
+    if (!c->fp32) { c->o.kahan_v = 0; c->o.kahan_sigma = 0; c->o.bf16_static = 0; }
+    c->o.kahan_v = c->o.kahan_v != 0;
+    c->o.bf16_static = c->o.bf16_static != 0;
     c->dev = c->o.device;
     if (c->dev < 0 || c->dev >= ndev) { delete c; return fail("bad device ordinal"); }
     hipError_t e = hipSetDevice(c->dev);
@@ -1139,30 +1168,38 @@ int s3d_init(s3d_ctx *c)
         if (dalloc(c, &c->v[q], rs * c->nuniq)) return 1;
         if (c->fp32 && c->o.kahan_v && dalloc(c, &c->lo[q], rs * c->nuniq)) return 1;
     }
-    if (dalloc(c, &c->dt_inv_mm, rs * c->nuniq)) return 1;
+    const size_t ss = c->o.bf16_static ? 2 : rs;   // static fields
+    const size_t ls = c->o.kahan_sigma == 2 ? 2 : rs; // stress compensation (BF16 in pairs)
+    if (dalloc(c, &c->dt_inv_mm, ss * c->nuniq)) return 1;
     if (dalloc(c, (void **)&c->mm64, sizeof(double) * c->nuniq)) return 1;
     if (dalloc(c, &c->sig, rs * 6 * c->nn)) return 1;
-    if (c->fp32 && c->o.kahan_sigma && dalloc(c, &c->sig_lo, rs * 6 * c->nn)) return 1;
+    if (c->o.kahan_sigma && dalloc(c, &c->sig_lo, ls * 6 * c->nn)) return 1;
     if (c->visco && dalloc(c, &c->mem, rs * 18 * c->nn)) return 1;
     if (dalloc(c, &c->force, rs * 3 * c->nn, false)) return 1;
-    if (dalloc(c, &c->lam, rs * c->nn, false)) return 1;
-    if (dalloc(c, &c->mu, rs * c->nn, false)) return 1;
-    if (c->visco && dalloc(c, &c->tau, rs * c->nn, false)) return 1;
+    if (dalloc(c, &c->lam, ss * c->nn, false)) return 1;
+    if (dalloc(c, &c->mu, ss * c->nn, false)) return 1;
+    if (c->visco && dalloc(c, &c->tau, ss * c->nn, false)) return 1;
     if (dalloc(c, &c->rho, rs * c->nn, false)) return 1;
     if (c->aniso) {
-        if (dalloc(c, &c->ca, rs * c->nn, false) || dalloc(c, &c->cb, rs * c->nn, false) ||
-            dalloc(c, &c->cc, rs * c->nn, false))
+        if (dalloc(c, &c->ca, ss * c->nn, false) || dalloc(c, &c->cb, ss * c->nn, false) ||
+            dalloc(c, &c->cc, ss * c->nn, false))
             return 1;
     }
     const unsigned gm = nblocks(c->nn, 256);
-    if (c->fp32)
-        hipLaunchKernelGGL(material_kernel<float>, dim3(gm), dim3(256), 0, c->stream, c->stage[S3D_RHOINV],
-                           c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q], c->stage[S3D_A],
-                           c->stage[S3D_B], c->stage[S3D_C], (float *)c->lam, (float *)c->mu,
-                           (float *)c->tau, (float *)c->rho, (float *)c->ca, (float *)c->cb,
+    if (c->fp32 && c->o.bf16_static)
+        hipLaunchKernelGGL((material_kernel<float, bf16>), dim3(gm), dim3(256), 0, c->stream,
+                           c->stage[S3D_RHOINV], c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q],
+                           c->stage[S3D_A], c->stage[S3D_B], c->stage[S3D_C], (bf16 *)c->lam,
+                           (bf16 *)c->mu, (bf16 *)c->tau, (float *)c->rho, (bf16 *)c->ca, (bf16 *)c->cb,
+                           (bf16 *)c->cc, c->nel, c->NP, c->g.dt);
+    else if (c->fp32)
+        hipLaunchKernelGGL((material_kernel<float, float>), dim3(gm), dim3(256), 0, c->stream,
+                           c->stage[S3D_RHOINV], c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q],
+                           c->stage[S3D_A], c->stage[S3D_B], c->stage[S3D_C], (float *)c->lam,
+                           (float *)c->mu, (float *)c->tau, (float *)c->rho, (float *)c->ca, (float *)c->cb,
                            (float *)c->cc, c->nel, c->NP, c->g.dt);
     else
-        hipLaunchKernelGGL(material_kernel<double>, dim3(gm), dim3(256), 0, c->stream, c->stage[S3D_RHOINV],
+        hipLaunchKernelGGL((material_kernel<double, double>), dim3(gm), dim3(256), 0, c->stream, c->stage[S3D_RHOINV],
                            c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q], c->stage[S3D_A],
                            c->stage[S3D_B], c->stage[S3D_C], (double *)c->lam, (double *)c->mu,
                            (double *)c->tau, (double *)c->rho, (double *)c->ca, (double *)c->cb,
@@ -1407,7 +1444,8 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
         const void *lo = nullptr;
         if (field <= S3D_SZZ) {
             src = (const char *)c->sig + rsize(c) * nn * (field - S3D_SXX);
-            if (c->sig_lo) lo = (const char *)c->sig_lo + rsize(c) * nn * (field - S3D_SXX);
+            if (c->sig_lo && c->o.kahan_sigma == 1) lo = (const char *)c->sig_lo + rsize(c) * nn * (field - S3D_SXX);
+            if (c->sig_lo && c->o.kahan_sigma == 2) lo = (const char *)c->sig_lo + 4 * nn * ((field - S3D_SXX) / 2);
         } else if (c->mem) {
             src = (const char *)c->mem + rsize(c) * nn * (field - S3D_MXX);
         }
@@ -1420,8 +1458,12 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
                 // value = hi - lo
                 double *tmp = nullptr;
                 HIPCHK(hipMalloc(&tmp, sizeof(double) * nn));
-                hipLaunchKernelGGL(from_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                                   (const float *)lo, tmp, c->nel, c->NP);
+                if (c->o.kahan_sigma == 2)
+                    hipLaunchKernelGGL(lo_pair_from_device, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                                       (const unsigned *)lo, (field - S3D_SXX) % 2, tmp, c->nel, c->NP);
+                else
+                    hipLaunchKernelGGL(from_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                                       (const float *)lo, tmp, c->nel, c->NP);
                 std::vector<double> a(nn), b(nn);
                 HIPCHK(hipMemcpyAsync(a.data(), buf, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(hipMemcpyAsync(b.data(), tmp, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
@@ -1464,9 +1506,13 @@ int s3d_set_field(s3d_ctx *c, int field, const double *in)
         void *dst = nullptr;
         if (field <= S3D_SZZ) {
             dst = (char *)c->sig + rsize(c) * nn * (field - S3D_SXX);
-            if (c->sig_lo)
+            if (c->sig_lo && c->o.kahan_sigma == 1)
                 HIPCHK(hipMemsetAsync((char *)c->sig_lo + rsize(c) * nn * (field - S3D_SXX), 0, rsize(c) * nn,
                                       c->stream));
+            if (c->sig_lo && c->o.kahan_sigma == 2)
+                hipLaunchKernelGGL(lo_pair_clear, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                                   (unsigned *)c->sig_lo + nn * ((field - S3D_SXX) / 2), (field - S3D_SXX) % 2,
+                                   nn);
         } else if (c->mem) {
             dst = (char *)c->mem + rsize(c) * nn * (field - S3D_MXX);
         } else {
@@ -1554,6 +1600,7 @@ int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out)
     for (int q = 0; q < n; ++q) {
         if (!sims[q] || !sims[q]->inited) return fail("batch members must be initialised (s3d_init)");
         if (sims[q]->dev != sims[0]->dev) return fail("batch members must share one device");
+        if (sims[q]->o.bf16_static) return fail("BF16 static fields are not supported in a batch");
         for (int r = 0; r < q; ++r)
             if (sims[r] == sims[q]) return fail("a simulation appears twice in the batch");
     }

@@ -48,6 +48,24 @@ struct RTab {                  // per (ez, k): radius dependent factors
     C r, inv_r, r2, pad;
 };
 
+using bf16 = __bf16;
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// two floats -> one dword of BF16 (round to nearest even), a in the low half
+__device__ inline unsigned pack_bf16(float a, float b)
+{
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((floatx2){a, b}, bf16x2));
+}
+__device__ inline float bf16_lo(unsigned w) { return __uint_as_float(w << 16); }
+__device__ inline float bf16_hi(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+
+// Static per-node fields stored in BF16 (all null: stored in the step
+// precision).
+struct Static16 {
+    const bf16 *lam, *mu, *tau, *ca, *cb, *cc;
+};
+
 template <typename R>
 struct ElemArgs {
     int nx, ny, nz, nel;
@@ -55,15 +73,18 @@ struct ElemArgs {
     const int *it_dev;         // iterations completed so far (device counter);
                                // this launch computes iteration *it_dev + 1
     int nt;
-    int kahan_sigma;
+    int kahan_sigma;           // 0 off, 1 FP32 compensation, 2 BF16 compensation
     const R *vx, *vy, *vz;     // unique-node velocity
     R *sig;                    // [6][nel][NP]
-    R *sig_lo;                 // [6][nel][NP] compensation or null
+    R *sig_lo;                 // [6][nel][NP] compensation (kahan_sigma 1) or null
+    unsigned *sig_lo2;         // [3][nel][NP] compensation (kahan_sigma 2): BF16 pairs
+                               // (xx,xy) (xz,yy) (yz,zz), first in the low half
     R *mem;                    // [18][nel][NP] (visco) or null
     // [nel][NP] moduli times dt: elastic lam, mu; visco-elastic
     // lam - 2/3 tau mu and mu (1 + tau) (relaxed-to-unrelaxed form)
     const R *lam, *mu, *tau;
     const R *ca, *cb, *cc;     // anisotropy dt*A, dt*B, dt*C or null
+    Static16 s16;              // the same fields in BF16 when lam is null
     R *force;                  // [3][nel][NP]
     const ThTab<coef_t<R>> *tth;   // [nx*P1]
     const RTab<coef_t<R>> *tr;     // [nz*P1]
@@ -91,6 +112,7 @@ struct NodeArgs {
     R *lx, *ly, *lz;           // compensation terms or null
     const R *force;            // [3][nel][NP]
     const R *dt_inv_mm;        // unique, dt / M
+    const bf16 *dt_inv_mm16;   // the same in BF16 when dt_inv_mm is null
     const double *px, *py, *pz;
     double alpha;
     int taper_on;

@@ -70,7 +70,8 @@ __device__ inline void kahan_add(R &s, R &c, R inc)
 // ---------------------------------------------------------------------------
 // element kernel
 // ---------------------------------------------------------------------------
-template <int P, int E, typename R, bool VISCO>
+// S16: the static per-node fields are stored in BF16 (ElemArgs::s16)
+template <int P, int E, typename R, bool VISCO, bool S16 = false>
 __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
     using S = Shape<P, E>;
@@ -114,14 +115,33 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     R sg[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) sg[c] = a.sig[c * fs + eb];
-    R lam = a.lam[eb];
-    R mu = a.mu[eb];
-    R tau = R(0);
+    R lam, mu, tau = R(0);
+    if constexpr (S16) {
+        lam = (R)(float)a.s16.lam[eb];
+        mu = (R)(float)a.s16.mu[eb];
+        if constexpr (VISCO) tau = (R)(float)a.s16.tau[eb];
+    } else {
+        lam = a.lam[eb];
+        mu = a.mu[eb];
+        if constexpr (VISCO) tau = a.tau[eb];
+    }
     R M[VISCO ? 18 : 1];
     if constexpr (VISCO) {
-        tau = a.tau[eb];
 #pragma unroll
         for (int c = 0; c < 18; ++c) M[c] = a.mem[c * fs + eb];
+    }
+    // stress compensation words (kahan_sigma 1: FP32, 2: BF16 pairs)
+    R lo[6] = {0, 0, 0, 0, 0, 0};
+    if (a.kahan_sigma == 2) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const unsigned w = a.sig_lo2[q * fs + eb];
+            lo[2 * q] = (R)bf16_lo(w);
+            lo[2 * q + 1] = (R)bf16_hi(w);
+        }
+    } else if (a.kahan_sigma == 1) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) lo[c] = a.sig_lo[c * fs + eb];
     }
     __syncthreads();
 
@@ -197,10 +217,17 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         rt[XZ] = tmut * exz + tm * Md[XZ];
         rt[YZ] = tmut * eyz + tm * Md[YZ];
     }
-    if (a.ca) {   // anisotropy (src/evolution.f90:290-296)
-        const R pa = a.ca[eb];
-        const R pb = a.cb[eb];
-        const R pc = a.cc[eb];
+    if (S16 ? a.s16.ca != nullptr : a.ca != nullptr) {   // anisotropy (src/evolution.f90:290-296)
+        R pa, pb, pc;
+        if constexpr (S16) {
+            pa = (R)(float)a.s16.ca[eb];
+            pb = (R)(float)a.s16.cb[eb];
+            pc = (R)(float)a.s16.cc[eb];
+        } else {
+            pa = a.ca[eb];
+            pb = a.cb[eb];
+            pc = a.cc[eb];
+        }
         rt[XX] += pc * ezz + pa * (eyy + exx);
         rt[YY] += pc * ezz + pa * (eyy + exx);
         rt[ZZ] += pc * (eyy + exx);
@@ -209,13 +236,9 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     }
 
     // ---- K6: integrate the stress ----
-    R lo[6];
     if (a.kahan_sigma) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            lo[c] = a.sig_lo[c * fs + eb];
-            kahan_add(sg[c], lo[c], rt[c]);
-        }
+        for (int c = 0; c < 6; ++c) kahan_add(sg[c], lo[c], rt[c]);
     } else {
 #pragma unroll
         for (int c = 0; c < 6; ++c) sg[c] = sg[c] + rt[c];
@@ -310,12 +333,21 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         a.force[c * fs + eb] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
 
     // ---- store tapered stress (K13) and memory variables ----
-    if (a.kahan_sigma) {
+    if (a.kahan_sigma == 1) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
             a.sig_lo[c * fs + eb] = lo[c] - lo[c] * dmp;
         }
+    } else if (a.kahan_sigma == 2) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            a.sig_lo2[q * fs + eb] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
+                                               (float)(lo[2 * q + 1] - lo[2 * q + 1] * dmp));
     } else {
 #pragma unroll
         for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
@@ -326,11 +358,11 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     }
 }
 
-template <int P, int E, typename R, bool VISCO>
+template <int P, int E, typename R, bool VISCO, bool S16 = false>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel(const ElemArgs<R> a)
 {
-    elem_body<P, E, R, VISCO>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
+    elem_body<P, E, R, VISCO, S16>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
 }
 
 // Index of the simulation that owns workgroup `wg` of a batched launch:
@@ -410,7 +442,11 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
         acc[c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
     }
     const long u = tid;
-    const R im = a.dt_inv_mm[u];
+    R im;
+    if (a.dt_inv_mm)
+        im = a.dt_inv_mm[u];
+    else
+        im = (R)(float)a.dt_inv_mm16[u];
     R dmp = R(0);   // 1 - taper
     if (a.taper_on) {
         const int ex = bx < a.nx ? bx : a.nx - 1, i = bx < a.nx ? li : P;
@@ -564,6 +600,24 @@ __global__ void from_device_layout(const R *src, double *dst, long nel, int NP)
     dst[t] = (double)src[e * NP + n];
 }
 
+// one half of the packed BF16 stress compensation -> reference layout
+__global__ void lo_pair_from_device(const unsigned *src, int half, double *dst, long nel, int NP)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nel * NP) return;
+    long n = t / nel;
+    long e = t - n * nel;
+    const unsigned w = src[e * NP + n];
+    dst[t] = (double)(half ? bf16_hi(w) : bf16_lo(w));
+}
+
+// clear one half of the packed BF16 stress compensation
+__global__ void lo_pair_clear(unsigned *p, int half, long n)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] &= half ? 0x0000ffffu : 0xffff0000u;
+}
+
 // unique-node field -> reference layout (every copy)
 template <int P, typename R>
 __global__ void expand_unique(const R *src, const R *lo, double *dst, int nx, int ny, int nz)
@@ -606,6 +660,16 @@ __global__ void gather_unique(const double *src, R *dst, R *lo, int nx, int ny, 
     if (lo) lo[t] = (R)(v - (double)(R)v) * R(-1);
 }
 
+// FP64 -> storage precision (BF16 through the nearest float)
+template <typename S>
+__device__ inline S narrow(double x)
+{
+    if constexpr (sizeof(S) == 2)
+        return (S)(float)x;
+    else
+        return (S)x;
+}
+
 // Material from the FP64 staging arrays in reference layout: tau
 // (= 6.0814 Q^-1.015, src/model_paras.F90:407), rho, dt*A, dt*B, dt*C and the
 // moduli times dt, elastic: lambda, mu; visco-elastic (dtau non-null):
@@ -613,10 +677,10 @@ __global__ void gather_unique(const double *src, R *dst, R *lo, int nx, int ny, 
 // (src/evolution.f90:216-284). All formed in FP64 and rounded once: a
 // constant tau makes 1 + tau the same number in every node of a region, and
 // an FP32 rounding of it would be a systematic modulus error.
-template <typename R>
+template <typename R, typename S>
 __global__ void material_kernel(const double *rhoinv, const double *lam, const double *mu,
                                 const double *q, const double *A, const double *B, const double *C,
-                                R *dlam, R *dmu, R *dtau, R *drho, R *dA, R *dB, R *dC,
+                                S *dlam, S *dmu, S *dtau, R *drho, S *dA, S *dB, S *dC,
                                 long nel, int NP, double dt)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -628,28 +692,28 @@ __global__ void material_kernel(const double *rhoinv, const double *lam, const d
     if (dtau) {
         double qq = q ? q[s] : 1.79769313486231570815e+308;
         const double tau = (double)6.0814f * pow(qq, (double)-1.015f);
-        dtau[t] = (R)tau;
-        dlam[t] = (R)(dt * (l - (2.0 / 3.0) * tau * m));
-        dmu[t] = (R)(dt * (m * (1.0 + tau)));
+        dtau[t] = narrow<S>(tau);
+        dlam[t] = narrow<S>(dt * (l - (2.0 / 3.0) * tau * m));
+        dmu[t] = narrow<S>(dt * (m * (1.0 + tau)));
     } else {
-        dlam[t] = (R)(dt * l);
-        dmu[t] = (R)(dt * m);
+        dlam[t] = narrow<S>(dt * l);
+        dmu[t] = narrow<S>(dt * m);
     }
     double ri = rhoinv ? rhoinv[s] : 0.0;
     drho[t] = (R)(ri > 0.0 ? 1.0 / ri : 0.0);
     if (dA) {
-        dA[t] = A ? (R)(dt * A[s]) : R(0);
-        dB[t] = B ? (R)(dt * B[s]) : R(0);
-        dC[t] = C ? (R)(dt * C[s]) : R(0);
+        dA[t] = A ? narrow<S>(dt * A[s]) : narrow<S>(0.0);
+        dB[t] = B ? narrow<S>(dt * B[s]) : narrow<S>(0.0);
+        dC[t] = C ? narrow<S>(dt * C[s]) : narrow<S>(0.0);
     }
 }
 
 // Mass matrix -J r^2 sin(theta) rho W per copy (src/model_paras.F90:411),
 // assembled with the same theta -> phi -> r summation tree as the force.
-// Stores M (FP64) and dt/M (step precision).
-template <int P, typename R>
+// Stores M (FP64) and dt/M (storage precision S).
+template <int P, typename S>
 __global__ void mass_kernel(const double *rhoinv, const double *sin_t, const double *rad,
-                            const double *w, double jac, double dt, double *mm64, R *dt_inv_mm,
+                            const double *w, double jac, double dt, double *mm64, S *dt_inv_mm,
                             int nx, int ny, int nz, long nblk)
 {
     constexpr int P1 = P + 1, PB = P * P * P;
@@ -661,7 +725,7 @@ __global__ void mass_kernel(const double *rhoinv, const double *sin_t, const dou
     int nbx = nx + 1, nby = ny + 1;
     int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
     if ((bx == nx && li) || (by == ny && lj) || (bz == nz && lk)) {
-        dt_inv_mm[t] = R(0);
+        dt_inv_mm[t] = narrow<S>(0.0);
         mm64[t] = 0.0;
         return;
     }
@@ -700,7 +764,7 @@ __global__ void mass_kernel(const double *rhoinv, const double *sin_t, const dou
         zhas = true;
     }
     mm64[t] = zsum;
-    dt_inv_mm[t] = (R)(zsum != 0.0 ? dt / zsum : 0.0);
+    dt_inv_mm[t] = narrow<S>(zsum != 0.0 ? dt / zsum : 0.0);
 }
 
 // Courant number partial maxima (src/model_paras.F90:529-561), FP64.

@@ -57,7 +57,8 @@ CONTAINS
                      cf%dt, cf%nt)
         o%precision = MERGE(S3D_FP32, S3D_FP64, cf%precision == 'fp32')
         o%kahan_v = MERGE(1, 0, cf%kahan_v)
-        o%kahan_sigma = MERGE(1, 0, cf%kahan_sigma)
+        o%kahan_sigma = MERGE(MERGE(2, 1, cf%kahan_sigma_bf16), 0, cf%kahan_sigma)
+        o%bf16_static = MERGE(1, 0, cf%bf16_static)
         o%device = cf%device
         CALL check(s3d_create(g, o, ctx), 'create')
         n = INT(cf%nx, 8) * cf%ny * cf%nz * (cf%lpd + 1)**3

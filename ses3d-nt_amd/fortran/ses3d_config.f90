@@ -5,7 +5,8 @@
 !! Groups: &grid, &model, &time, &general (exactly one each), &source_mt,
 !! &source_sf, &receiver (any number, read in file order), and the optional
 !! &gpu group of this build: precision ('fp32' | 'fp64'), kahan_v,
-!! kahan_sigma, device.
+!! kahan_sigma, kahan_sigma_bf16 (stress compensation words in BF16),
+!! bf16_static (moduli, tau and dt/M stored in BF16), device.
 MODULE configuration_mod
     USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY : ERROR_UNIT, IOSTAT_END
     USE, INTRINSIC :: ISO_C_BINDING, ONLY : c_double, c_int
@@ -56,7 +57,7 @@ MODULE configuration_mod
         CHARACTER(LEN=fnl) :: log_file_dir
         ! &gpu
         CHARACTER(LEN=8) :: precision
-        LOGICAL :: kahan_v, kahan_sigma
+        LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
         INTEGER :: device
         TYPE(source_entry), ALLOCATABLE :: sources(:)
         TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
@@ -105,9 +106,9 @@ CONTAINS
         NAMELIST /general/ log_file_dir, event_name, workflow
         ! &gpu
         CHARACTER(LEN=8) :: precision
-        LOGICAL :: kahan_v, kahan_sigma
+        LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
         INTEGER :: device
-        NAMELIST /gpu/ precision, kahan_v, kahan_sigma, device
+        NAMELIST /gpu/ precision, kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static, device
         INTEGER :: lun, ios, n
         TYPE(configuration_typ) :: cf
 
@@ -175,7 +176,8 @@ CONTAINS
         cf%nt = nt; cf%dt = dt; cf%date_time = date_time
 
         ! ---- &gpu (optional) ----
-        precision = 'fp32'; kahan_v = .TRUE.; kahan_sigma = .FALSE.; device = 0
+        precision = 'fp32'; kahan_v = .TRUE.; kahan_sigma = .TRUE.; device = 0
+        kahan_sigma_bf16 = .TRUE.; bf16_static = .FALSE.
         n = count_groups(lun, 'gpu')
         IF (n > 1) CALL abort('Found ' // i2c(n) // ' &gpu-groups in config-file!')
         IF (n == 1) THEN
@@ -185,6 +187,7 @@ CONTAINS
         IF (precision /= 'fp32' .AND. precision /= 'fp64') &
             CALL abort("precision must be 'fp32' or 'fp64'")
         cf%precision = precision; cf%kahan_v = kahan_v; cf%kahan_sigma = kahan_sigma
+        cf%kahan_sigma_bf16 = kahan_sigma_bf16; cf%bf16_static = bf16_static
         cf%device = device
 
         CALL parse_sources(lun, cf)

@@ -55,7 +55,7 @@ class S3dGrid(C.Structure):
 class S3dOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("kahan_v", C.c_int),
                 ("kahan_sigma", C.c_int), ("device", C.c_int),
-                ("xcd_remap", C.c_int)]
+                ("xcd_remap", C.c_int), ("bf16_static", C.c_int)]
 
 
 class S3dError(RuntimeError):
@@ -135,13 +135,17 @@ class Simulation:
     """Device state of one simulation (one mesh, one model)."""
 
     def __init__(self, grid, precision="fp32", kahan_v=True, kahan_sigma=False,
-                 device=0, xcd_remap=True):
+                 device=0, xcd_remap=True, bf16_static=False):
+        """kahan_sigma: False/0 off, True/1 FP32 compensation words,
+        2 or "bf16" BF16 compensation words. bf16_static: store the moduli,
+        tau and dt/M in BF16 (FP32 only)."""
         g = S3dGrid(grid.nx, grid.ny, grid.nz, grid.lpd, grid.lat_min,
                     grid.lat_max, grid.lon_min, grid.lon_max, grid.rad_min,
                     grid.rad_max, grid.taper_width, grid.taper_slope, grid.dt,
                     grid.nt)
+        ks = 2 if kahan_sigma == "bf16" else int(kahan_sigma)
         o = S3dOptions(FP32 if precision == "fp32" else FP64, int(kahan_v),
-                       int(kahan_sigma), int(device), int(xcd_remap))
+                       ks, int(device), int(xcd_remap), int(bf16_static))
         h = C.c_void_p()
         _chk(lib().s3d_create(C.byref(g), C.byref(o), C.byref(h)))
         self._h = h
@@ -298,12 +302,13 @@ class Batch:
 
 
 def build_case(case, precision="fp32", kahan_v=True, kahan_sigma=False,
-               slab=None, xcd_remap=True, autotune=False):
+               slab=None, xcd_remap=True, autotune=False, bf16_static=False):
     """Simulation loaded with a synth.Case (model, sources, receivers).
     The model is uploaded `slab` element layers at a time (all at once when
     None). Returns (simulation, list of receivers that were placed)."""
     g = case.grid
-    sim = Simulation(g, precision, kahan_v, kahan_sigma, xcd_remap=xcd_remap)
+    sim = Simulation(g, precision, kahan_v, kahan_sigma, xcd_remap=xcd_remap,
+                     bf16_static=bf16_static)
     nz = g.nz
     step = slab or nz
     for ez0 in range(0, nz, step):

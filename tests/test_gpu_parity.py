@@ -163,3 +163,64 @@ def test_autotuned_tiles_give_identical_results():
             assert np.array_equal(a.trace(r, ch), b.trace(r, ch))
     with pytest.raises(api.S3dError, match="at rest"):
         b.autotune()
+
+
+def _traces(sim, recs):
+    return np.array([[sim.trace(r, ch) for ch in range(3)] for r in range(len(recs))])
+
+
+def test_precision_modes_against_fp64():
+    """FP32 modes against the FP64 path on a small layered visco-elastic
+    mesh: the compensated modes stay within 1e-5 per trace; BF16-stored
+    moduli / 1/M perturb the model itself (2^-9 relative), which must show
+    as a misfit far above the FP32 ones (so the BF16 storage is in use)."""
+    case = synth.layered(seed=7, nt=400, nx=10, ny=10, nz=5, lpd=4, n_rec=8)
+    ref, recs = api.build_case(case, precision="fp64")
+    ref.step(400)
+    t64 = _traces(ref, recs)
+    norms = np.linalg.norm(t64, axis=2)
+    keep = norms > 1e-3 * norms.max()
+    out = {}
+    for name, kw in [("kv", dict(kahan_v=True)),
+                     ("kv+ks", dict(kahan_v=True, kahan_sigma=True)),
+                     ("kv+ks_bf16", dict(kahan_v=True, kahan_sigma="bf16")),
+                     ("kv+bf16_static", dict(kahan_v=True, bf16_static=True))]:
+        sim, _ = api.build_case(case, precision="fp32", **kw)
+        sim.step(400)
+        t = _traces(sim, recs)
+        err = np.linalg.norm(t - t64, axis=2)[keep] / norms[keep]
+        out[name] = (float(np.median(err)), float(err.max()))
+    print(out)
+    for name in ("kv", "kv+ks", "kv+ks_bf16"):
+        assert out[name][1] < 1e-5, (name, out[name])
+    assert out["kv+ks_bf16"][0] < 2 * out["kv+ks"][0] + 1e-8
+    assert 1e-5 < out["kv+bf16_static"][0] < 5e-2
+
+
+def test_bf16_stress_compensation_field_io():
+    """With BF16 compensation words a stress field reads back as the FP32
+    value that was set (the compensation is cleared on set), and a run
+    keeps hi - lo readable."""
+    grid = small_grid(lpd=4, nx=3, ny=3, nz=2)
+    rng = np.random.default_rng(3)
+    model = random_model(grid, rng)
+    s = api.Simulation(grid, "fp32", kahan_v=True, kahan_sigma="bf16")
+    for k, arr in model.items():
+        s.set_model(PARAMS[k][1], arr)
+    s.init()
+    P1 = grid.lpd + 1
+    sg = 1e5 * rng.standard_normal((P1, P1, P1, grid.nz, grid.ny, grid.nx))
+    s.set(api.SYY, sg)
+    assert np.array_equal(s.get(api.SYY), sg.astype(np.float32).astype(np.float64))
+    s.step(3)
+    assert np.all(np.isfinite(s.get(api.SYY)))
+
+
+def test_bf16_static_restrictions():
+    """BF16 static fields use the default element tile only and are refused
+    in a batch."""
+    case = synth.homogeneous(nt=20, nx=4, ny=4, nz=3, n_rec=2)
+    a, _ = api.build_case(case, precision="fp32", bf16_static=True, autotune=True)
+    b, _ = api.build_case(case, precision="fp32")
+    with pytest.raises(api.S3dError, match="BF16"):
+        api.Batch([a, b])
