@@ -109,6 +109,7 @@ def load_sims(case, modes, slab, remap=True):
     for name, sim in sims:
         sim.init()
         tile, tms = sim.autotune()
+        sim.tile = tile
         log("[bench] %s autotune: %d elements per workgroup, element kernel %.3f ms" % (name, tile, tms))
         nrec = 0
         for s_ in case.sources:
@@ -324,6 +325,7 @@ def main():
     mode = args.mode
     modes = [mode] + ([] if args.no_fp64 or mode == "fp64" else ["fp64"])
     sims = dict(load_sims(case, modes, args.slab, remap=not args.no_remap))
+    tile = getattr(sims[mode], "tile", 0)
     wall, ms, cnt, vr = time_gpu(sims[mode], args.steps, args.warmup)
     value = N * args.steps / wall
     log("[bench] %s %.3f ms/step, elem %.3f ms, node %.3f ms, v range %s"
@@ -336,7 +338,7 @@ def main():
     rtype = "double" if mode == "fp64" else "float"
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": None,
-            "kernel": "elem_kernel<4,%s,visco>" % rtype,
+            "kernel": "elem_kernel<4,%s,visco>, %d elements per workgroup" % (rtype, tile),
             "algorithmic_bytes_per_launch": eb}
     roof["achieved"] = achieved / 1e9
     roof["peak"] = HBM_PEAK / 1e9

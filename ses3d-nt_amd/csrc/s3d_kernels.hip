@@ -79,7 +79,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
     __shared__ R lds[6][EPB * NP];
     __shared__ C cf[4][P1 * P1];     // Dx, Dy, Dz, dl*w
-    __shared__ C geo[EPB][2 * P1];   // sin(theta), r^2 at the element's GLL points
+    // per element: sin(theta_q), r_q^2 and the three taper profiles at its GLL points
+    __shared__ C geo[EPB][5 * P1];
 
     const int t = threadIdx.x;
     const int slot = t / NP;
@@ -100,17 +101,35 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     // loads issue back to back; such lanes store nothing.
     const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
     const R vx = a.vx[u], vy = a.vy[u], vz = a.vz[u];
-    for (int x = t; x < 4 * P1 * P1; x += S::NT) {
-        const int m = x / (P1 * P1), y = x - m * P1 * P1;
-        cf[m][y] = m == 0 ? a.Dx[y] : m == 1 ? a.Dy[y] : m == 2 ? a.Dz[y] : a.dlw[y];
+    // each filling lane issues all its table loads at once, then waits once
+    static_assert(P1 * P1 <= S::NT, "coefficient fill needs (p+1)^2 lanes");
+    if (t < P1 * P1) {
+        const C d0 = a.Dx[t], d1 = a.Dy[t], d2 = a.Dz[t], d3 = a.dlw[t];
+        cf[0][t] = d0;
+        cf[1][t] = d1;
+        cf[2][t] = d2;
+        cf[3][t] = d3;
     }
-    if (inside && node < 2 * P1)
-        geo[slot][node] = node < P1 ? a.tth[ex * P1 + node].sin_t : a.tr[ez * P1 + node - P1].r2;
+    if (inside && node < P1) {
+        const C s0 = a.tth[ex * P1 + node].sin_t, s1 = a.tr[ez * P1 + node].r2;
+        C s2 = C(0), s3 = C(0), s4 = C(0);
+        if (a.taper_on) {
+            s2 = (C)a.px[ex * P1 + node];
+            s3 = (C)a.py[ey * P1 + node];
+            s4 = (C)a.pz[ez * P1 + node];
+        }
+        geo[slot][node] = s0;
+        geo[slot][P1 + node] = s1;
+        geo[slot][2 * P1 + node] = s2;
+        geo[slot][3 * P1 + node] = s3;
+        geo[slot][4 * P1 + node] = s4;
+    }
     if (inside) {
         lds[0][L + node] = vx;
         lds[1][L + node] = vy;
         lds[2][L + node] = vz;
     }
+    const int sslot = a.src_slot ? a.src_slot[ee] : -1;   // moment-tensor source slot
     // ---- state loads (issued before the barrier to hide latency) ----
     R sg[6];
 #pragma unroll
@@ -248,16 +267,13 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     // iteration it = *it_dev + 1 uses the STF sample it-1 (zero for it = 1)
     R src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int it = *a.it_dev + 1;
-    if (a.src_slot && valid && it > 1 && it <= a.nt) {
-        const int sl = a.src_slot[ee];
-        if (sl >= 0) {
+    if (valid && sslot >= 0 && it > 1 && it <= a.nt) {
 #pragma unroll
-            for (int c = 0; c < 9; ++c) {
-                const int ch = a.src_chan[sl * 9 + c];
-                if (ch >= 0) {
-                    const double d = a.stf[(long)ch * a.nt + (it - 2)];
-                    src[c] = (R)(-(d * a.src_coef[((long)sl * 9 + c) * NP + node]));
-                }
+        for (int c = 0; c < 9; ++c) {
+            const int ch = a.src_chan[sslot * 9 + c];
+            if (ch >= 0) {
+                const double d = a.stf[(long)ch * a.nt + (it - 2)];
+                src[c] = (R)(-(d * a.src_coef[((long)sslot * 9 + c) * NP + node]));
             }
         }
     }
@@ -284,8 +300,10 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 
     // ---- taper factor of this node (K13) ----
     R dmp = R(0);   // 1 - taper
-    if (a.taper_on) {
-        dmp = (R)taper_damp(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
+    {
+        const C *gs = geo[inside ? slot : 0];
+        const C tx = gs[2 * P1 + i], ty = gs[3 * P1 + j], tz = gs[4 * P1 + k];
+        if (tx + ty + tz != C(0)) dmp = (R)taper_damp(tx, ty, tz, a.alpha);
     }
 
     __syncthreads();
