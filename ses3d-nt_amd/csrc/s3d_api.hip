@@ -391,6 +391,9 @@ int launch_elem(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st, int epb)
     return launch_elem_e<P, 0, R>(c, ea, st);
 }
 
+// Element tiles offered to the autotuner: FP32 kernels exist for 1, 2 and 4
+// elements per workgroup (where that fits 1024 lanes) besides the default;
+// FP64 and BF16-static use the default tile only.
 template <int P>
 std::vector<int> tile_candidates(bool fp32, bool bf16_static)
 {

@@ -70,38 +70,128 @@ __device__ inline void kahan_add(R &s, R &c, R inc)
 // ---------------------------------------------------------------------------
 // element kernel
 // ---------------------------------------------------------------------------
-// S16: the static per-node fields are stored in BF16 (ElemArgs::s16)
-template <int P, int E, typename R, bool VISCO, bool S16 = false>
+// Direct HBM -> LDS load of one dword per lane (global_load_lds_dword):
+// lane l writes LDS byte address lds + 4 l. Written as inline assembly so the
+// compiler's wait accounting never sees it: callers issue these loads before
+// any compiler-issued load (vector memory operations complete in issue
+// order, so the compiler's own waits stay exact) and wait with an explicit
+// s_waitcnt vmcnt(0) before reading the LDS words.
+__device__ __forceinline__ void lds_dma_dword(const void *g, unsigned lds)
+{
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
+                 :: "v"(g), "s"(__builtin_amdgcn_readfirstlane(lds)) : "memory");
+}
+
+// Per-lane state of one element tile (EPB elements, one lane per
+// element-local node).
+template <typename R, bool VISCO>
+struct TileState {
+    R v[3];             // velocity gathered from the unique nodes
+    R sg[6];            // stress
+    R lam, mu, tau;     // dt-scaled moduli, tau
+    R M[VISCO ? 18 : 1];
+    R lo[6];            // stress compensation, kahan_sigma 1
+    unsigned lo2[3];    // stress compensation, kahan_sigma 2 (packed BF16 pairs)
+};
+
+// Issue the loads of tile `tile` into `s` (no waits). Lanes without an
+// element (the tail of the last tile) read element 0's addresses, so every
+// load is unconditional and in bounds; such lanes store nothing.
+template <int P, int E, typename R, bool VISCO, bool S16, bool LOADM = true>
+__device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileState<R, VISCO> &s)
+{
+    using S = Shape<P, E>;
+    constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
+    const int t = threadIdx.x;
+    const int slot = t / NP;
+    const int node = t - slot * NP;
+    const int e = tile * EPB + slot;
+    const bool valid = slot < EPB && e < a.nel;
+    const int ee = valid ? e : 0;
+    const int ex = ee % a.nx, ey = (ee / a.nx) % a.ny, ez = ee / (a.nx * a.ny);
+    const int i = node % P1, j = (node / P1) % P1, k = node / (P1 * P1);
+    const long fs = (long)a.nel * NP;
+    const long eb = (long)ee * NP + node;
+    const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
+    s.v[0] = a.vx[u];
+    s.v[1] = a.vy[u];
+    s.v[2] = a.vz[u];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s.sg[c] = a.sig[c * fs + eb];
+    s.tau = R(0);
+    if constexpr (S16) {
+        s.lam = (R)(float)a.s16.lam[eb];
+        s.mu = (R)(float)a.s16.mu[eb];
+        if constexpr (VISCO) s.tau = (R)(float)a.s16.tau[eb];
+    } else {
+        s.lam = a.lam[eb];
+        s.mu = a.mu[eb];
+        if constexpr (VISCO) s.tau = a.tau[eb];
+    }
+    if constexpr (VISCO && LOADM) {
+#pragma unroll
+        for (int c = 0; c < 18; ++c) s.M[c] = a.mem[c * fs + eb];
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) s.lo[c] = R(0);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) s.lo2[q] = 0u;
+    if (a.kahan_sigma == 2) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) s.lo2[q] = a.sig_lo2[q * fs + eb];
+    } else if (a.kahan_sigma == 1) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) s.lo[c] = a.sig_lo[c * fs + eb];
+    }
+}
+
+// Geometry rows per element (FP64): sin, 1/sin, cot of theta_q; r, 1/r, r^2
+// of r_q; the three taper profiles.
+enum { G_SIN = 0, G_ISIN, G_COT, G_R, G_IR, G_R2, G_PX, G_PY, G_PZ, G_N };
+
+// Element kernel body: the workgroup advances tile `blk` (EPB elements, one
+// lane per element-local node). S16: the static per-node fields are stored
+// in BF16 (ElemArgs::s16).
+template <int P, int E, typename R, bool VISCO, bool S16>
 __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
     using S = Shape<P, E>;
     using C = coef_t<R>;
-    constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
+    constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB, NE = EPB;
     __shared__ R lds[6][EPB * NP];
     __shared__ C cf[4][P1 * P1];     // Dx, Dy, Dz, dl*w
-    // per element: sin(theta_q), r_q^2 and the three taper profiles at its GLL points
-    __shared__ C geo[EPB][5 * P1];
+    __shared__ C cw[P1];             // GLL weights
+    __shared__ C gtab[NE][G_N * P1]; // geometry rows of the tile's elements
+    __shared__ int sst[NE];          // moment-tensor source slot per element
 
     const int t = threadIdx.x;
     const int slot = t / NP;
     const int node = t - slot * NP;
-    const int e = blk * EPB + slot;
     const bool inside = slot < EPB;
-    const bool valid = inside && e < a.nel;
     const int i = node % P1, j = (node / P1) % P1, k = node / (P1 * P1);
-    const int ee = valid ? e : 0;
-    const int ex = ee % a.nx, ey = (ee / a.nx) % a.ny, ez = ee / (a.nx * a.ny);
     const int L = (inside ? slot : 0) * NP;
     const long fs = (long)a.nel * NP;      // field stride
-    const long eb = (long)ee * NP + node;  // element-local address
+    const int tile0 = blk;
 
-    // ---- velocity gather (unique nodes -> LDS) ----
-    // Lanes without an element (the tail of the last workgroup) read element
-    // 0's addresses: every load below is unconditional and in bounds, so the
-    // loads issue back to back; such lanes store nothing.
-    const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
-    const R vx = a.vx[u], vy = a.vy[u], vz = a.vz[u];
-    // each filling lane issues all its table loads at once, then waits once
+    // ---- memory variables of the FP32 kernels: streamed straight into LDS
+    // (lane t's words at mbuf[c][t]), so they hold no registers until the
+    // stress rate; issued before every other load ----
+    constexpr bool MDMA = VISCO && sizeof(R) == 4;
+    __shared__ R mbuf[MDMA ? 18 : 1][MDMA ? S::NT : 1];
+    if constexpr (MDMA) {
+        const int e = tile0 * EPB + slot;
+        const long eb = (long)((inside && e < a.nel) ? e : 0) * NP + node;
+        const unsigned base = (unsigned)(size_t)&mbuf[0][(t >> 6) << 6];
+#pragma unroll
+        for (int c = 0; c < 18; ++c)
+            lds_dma_dword(a.mem + c * fs + eb, base + (unsigned)(c * S::NT * sizeof(R)));
+    }
+
+    // ---- state of the first tile: issued first, consumed after the tables ----
+    TileState<R, VISCO> cur;
+    load_tile<P, E, R, VISCO, S16, !MDMA>(a, tile0, cur);
+
+    // ---- tables -> LDS (every filling lane issues its loads, then waits once) ----
     static_assert(P1 * P1 <= S::NT, "coefficient fill needs (p+1)^2 lanes");
     if (t < P1 * P1) {
         const C d0 = a.Dx[t], d1 = a.Dy[t], d2 = a.Dz[t], d3 = a.dlw[t];
@@ -110,269 +200,281 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         cf[2][t] = d2;
         cf[3][t] = d3;
     }
-    if (inside && node < P1) {
-        const C s0 = a.tth[ex * P1 + node].sin_t, s1 = a.tr[ez * P1 + node].r2;
-        C s2 = C(0), s3 = C(0), s4 = C(0);
+    if (t < P1) cw[t] = a.w[t];
+    for (int x = t; x < NE * P1; x += S::NT) {
+        const int es = x / P1, q = x - es * P1;
+        int e = tile0 * EPB + es;
+        e = e < a.nel ? e : a.nel - 1;
+        const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
+        const ThTab<C> th = a.tth[ex * P1 + q];
+        const RTab<C> rr = a.tr[ez * P1 + q];
+        C px = C(0), py = C(0), pz = C(0);
         if (a.taper_on) {
-            s2 = (C)a.px[ex * P1 + node];
-            s3 = (C)a.py[ey * P1 + node];
-            s4 = (C)a.pz[ez * P1 + node];
+            px = (C)a.px[ex * P1 + q];
+            py = (C)a.py[ey * P1 + q];
+            pz = (C)a.pz[ez * P1 + q];
         }
-        geo[slot][node] = s0;
-        geo[slot][P1 + node] = s1;
-        geo[slot][2 * P1 + node] = s2;
-        geo[slot][3 * P1 + node] = s3;
-        geo[slot][4 * P1 + node] = s4;
+        C *g = gtab[es];
+        g[G_SIN * P1 + q] = th.sin_t;
+        g[G_ISIN * P1 + q] = th.inv_sin;
+        g[G_COT * P1 + q] = th.cot;
+        g[G_R * P1 + q] = rr.r;
+        g[G_IR * P1 + q] = rr.inv_r;
+        g[G_R2 * P1 + q] = rr.r2;
+        g[G_PX * P1 + q] = px;
+        g[G_PY * P1 + q] = py;
+        g[G_PZ * P1 + q] = pz;
     }
-    if (inside) {
-        lds[0][L + node] = vx;
-        lds[1][L + node] = vy;
-        lds[2][L + node] = vz;
+    for (int x = t; x < NE; x += S::NT) {
+        const int e = tile0 * EPB + x;
+        sst[x] = (a.src_slot && e < a.nel) ? a.src_slot[e] : -1;
     }
-    const int sslot = a.src_slot ? a.src_slot[ee] : -1;   // moment-tensor source slot
-    // ---- state loads (issued before the barrier to hide latency) ----
-    R sg[6];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) sg[c] = a.sig[c * fs + eb];
-    R lam, mu, tau = R(0);
-    if constexpr (S16) {
-        lam = (R)(float)a.s16.lam[eb];
-        mu = (R)(float)a.s16.mu[eb];
-        if constexpr (VISCO) tau = (R)(float)a.s16.tau[eb];
-    } else {
-        lam = a.lam[eb];
-        mu = a.mu[eb];
-        if constexpr (VISCO) tau = a.tau[eb];
-    }
-    R M[VISCO ? 18 : 1];
-    if constexpr (VISCO) {
-#pragma unroll
-        for (int c = 0; c < 18; ++c) M[c] = a.mem[c * fs + eb];
-    }
-    // stress compensation words (kahan_sigma 1: FP32, 2: BF16 pairs)
-    R lo[6] = {0, 0, 0, 0, 0, 0};
-    if (a.kahan_sigma == 2) {
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            const unsigned w = a.sig_lo2[q * fs + eb];
-            lo[2 * q] = (R)bf16_lo(w);
-            lo[2 * q + 1] = (R)bf16_hi(w);
-        }
-    } else if (a.kahan_sigma == 1) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) lo[c] = a.sig_lo[c * fs + eb];
-    }
+    // iteration it = *it_dev + 1 uses the STF sample it-1 (zero for it = 1)
+    const int it = *a.it_dev + 1;
     __syncthreads();
 
-    // ---- K1: tensor-product derivatives along theta, phi, r ----
-    C dxv[3] = {0, 0, 0}, dyv[3] = {0, 0, 0}, dzv[3] = {0, 0, 0};
     {
-        const C *Dxi = cf[0] + i * P1;
-        const C *Dyj = cf[1] + j * P1;
-        const C *Dzk = cf[2] + k * P1;
+        const int tile = tile0;
+        const int es = inside ? slot : 0;
+        const int e = tile * EPB + slot;
+        const bool valid = inside && e < a.nel;
+        const long eb = (long)(valid ? e : 0) * NP + node;    // element-local address
+        const C *gs = gtab[es];
+
+        if (inside) {
+            lds[0][L + node] = cur.v[0];
+            lds[1][L + node] = cur.v[1];
+            lds[2][L + node] = cur.v[2];
+        }
+        __syncthreads();
+
+        // ---- K1: tensor-product derivatives along theta, phi, r ----
+        C dxv[3] = {0, 0, 0}, dyv[3] = {0, 0, 0}, dzv[3] = {0, 0, 0};
+        {
+            const C *Dxi = cf[0] + i * P1;
+            const C *Dyj = cf[1] + j * P1;
+            const C *Dzk = cf[2] + k * P1;
+#pragma unroll
+            for (int q = 0; q < P1; ++q) {
+                const int nxq = L + q + P1 * (j + P1 * k);
+                const int nyq = L + i + P1 * (q + P1 * k);
+                const int nzq = L + i + P1 * (j + P1 * q);
+                const C cx = Dxi[q], cy = Dyj[q], cz = Dzk[q];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    dxv[c] += (C)lds[c][nxq] * cx;
+                    dyv[c] += (C)lds[c][nyq] * cy;
+                    dzv[c] += (C)lds[c][nzq] * cz;
+                }
+            }
+        }
+
+        // ---- K2: spherical completion, K4: symmetrisation ----
+        R exx, eyy, ezz, exy, exz, eyz;
+        {
+            const C ux = cur.v[0], uy = cur.v[1], uz = cur.v[2];
+            const C inv_r = gs[G_IR * P1 + k];
+            const C cot = gs[G_COT * P1 + i];
+            const C inv_rs = inv_r * gs[G_ISIN * P1 + i];
+            const C ezx = -dzv[0];
+            const C exz_c = (dxv[2] - ux) * inv_r;
+            const C ezy = -dzv[1];
+            const C eyz_c = -uy * inv_r + dyv[2] * inv_rs;
+            const C exy_c = dxv[1] * inv_r;
+            const C eyx = -uy * cot * inv_r + dyv[0] * inv_rs;
+            exx = (R)((uz + dxv[0]) * inv_r);
+            eyy = (R)(dyv[1] * inv_rs + (uz + ux * cot) * inv_r);
+            ezz = (R)(-dzv[2]);
+            exy = (R)((exy_c + eyx) * 0.5);
+            exz = (R)((exz_c + ezx) * 0.5);
+            eyz = (R)((eyz_c + ezy) * 0.5);
+        }
+
+        // ---- K5: stress rate times dt (lam, mu, A, B, C carry the dt) ----
+        R rt[6];
+        const R tmu = R(2) * cur.mu;
+        if constexpr (!VISCO) {
+            const R Lr = cur.lam * (exx + eyy + ezz);
+            rt[XX] = Lr + tmu * exx;
+            rt[YY] = Lr + tmu * eyy;
+            rt[ZZ] = Lr + tmu * ezz;
+            rt[XY] = tmu * exy;
+            rt[XZ] = tmu * exz;
+            rt[YZ] = tmu * eyz;
+        } else {
+            // build-defined SLS (loop form of src/evolution.f90:216-284)
+            if constexpr (MDMA) {
+                // this lane's words were written by its own wave's loads
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#pragma unroll
+                for (int c = 0; c < 18; ++c) cur.M[c] = mbuf[c][t];
+            }
+            R Md[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) Md[c] = cur.M[c] + cur.M[6 + c] + cur.M[12 + c];
+            const R Msum = Md[XX] + Md[YY] + Md[ZZ];
+            // lam holds kappa - 2 mu_tau / 3 and mu holds mu_tau = mu (1 + tau),
+            // both formed in FP64; 2 mu itself only scales the memory variables
+            const R tmut = tmu;                                    // 2 mu_tau
+            const R tm = tmut / (R(1) + cur.tau);                  // 2 mu
+            const R Lr = cur.lam * (exx + eyy + ezz) - tm * Msum * R(1.0 / 3.0);
+            rt[XX] = Lr + tmut * exx + tm * Md[XX];
+            rt[YY] = Lr + tmut * eyy + tm * Md[YY];
+            rt[ZZ] = Lr + tmut * ezz + tm * Md[ZZ];
+            rt[XY] = tmut * exy + tm * Md[XY];
+            rt[XZ] = tmut * exz + tm * Md[XZ];
+            rt[YZ] = tmut * eyz + tm * Md[YZ];
+        }
+        if (S16 ? a.s16.ca != nullptr : a.ca != nullptr) {   // anisotropy (src/evolution.f90:290-296)
+            R pa, pb, pc;
+            if constexpr (S16) {
+                pa = (R)(float)a.s16.ca[eb];
+                pb = (R)(float)a.s16.cb[eb];
+                pc = (R)(float)a.s16.cc[eb];
+            } else {
+                pa = a.ca[eb];
+                pb = a.cb[eb];
+                pc = a.cc[eb];
+            }
+            rt[XX] += pc * ezz + pa * (eyy + exx);
+            rt[YY] += pc * ezz + pa * (eyy + exx);
+            rt[ZZ] += pc * (eyy + exx);
+            rt[XZ] += R(2) * pb * exz;
+            rt[YZ] += R(2) * pb * eyz;
+        }
+
+        // ---- K6: integrate the stress ----
+        R sg[6], lo[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            sg[c] = cur.sg[c];
+            lo[c] = cur.lo[c];
+        }
+        if (a.kahan_sigma) {
+            if (a.kahan_sigma == 2) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    lo[2 * q] = (R)bf16_lo(cur.lo2[q]);
+                    lo[2 * q + 1] = (R)bf16_hi(cur.lo2[q]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 6; ++c) kahan_add(sg[c], lo[c], rt[c]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sg[c] = sg[c] + rt[c];
+        }
+
+        // ---- K7: moment-tensor glut and single force of the previous add() ----
+        R src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const int sslot = sst[es];
+        if (valid && sslot >= 0 && it > 1 && it <= a.nt) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) {
+                const int ch = a.src_chan[sslot * 9 + c];
+                if (ch >= 0) {
+                    const double d = a.stf[(long)ch * a.nt + (it - 2)];
+                    src[c] = (R)(-(d * a.src_coef[((long)sslot * 9 + c) * NP + node]));
+                }
+            }
+        }
+
+        __syncthreads();   // LDS velocity no longer needed
+        if (inside) {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) lds[c][L + node] = sg[c] + src[c];
+        }
+
+        // ---- memory variables (loop form of src/evolution.f90:415-442) ----
+        R M[VISCO ? 18 : 1];
+        if constexpr (VISCO) {
+            const R ev[6] = {exx, exy, exz, eyy, eyz, ezz};
+            // M (1 - dt/tau_p) - dt tau / (3 tau_p) e  ==  M - dt/tau_p (M + tau/3 e):
+            // the decrement form keeps the FP32 relaxation rate exact to one
+            // rounding of dt/tau_p (1 - dt/tau_p would lose its low bits)
+            const R t3 = cur.tau * R(1.0 / 3.0);
+#pragma unroll
+            for (int m = 0; m < NRDISS; ++m) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    M[6 * m + c] = cur.M[6 * m + c] - a.rate[m] * (cur.M[6 * m + c] + t3 * ev[c]);
+            }
+        }
+
+        // ---- taper factor of this node (K13) ----
+        R dmp = R(0);   // 1 - taper
+        {
+            const C tx = gs[G_PX * P1 + i], ty = gs[G_PY * P1 + j], tz = gs[G_PZ * P1 + k];
+            if (tx + ty + tz != C(0)) dmp = (R)taper_damp(tx, ty, tz, a.alpha);
+        }
+
+        __syncthreads();
+
+        // ---- K8: weak divergence ----
+        C ax[P1], ay[P1], az[P1];
+        {
+            const C *dli = cf[3] + i * P1;
+            const C *dlj = cf[3] + j * P1;
+            const C *dlk = cf[3] + k * P1;
+#pragma unroll
+            for (int q = 0; q < P1; ++q) {
+                ax[q] = dli[q] * gs[G_SIN * P1 + q];
+                ay[q] = dlj[q];
+                az[q] = dlk[q] * gs[G_R2 * P1 + q];
+            }
+        }
+        C Sx[3] = {0, 0, 0}, Sy[3] = {0, 0, 0}, Sz[3] = {0, 0, 0};
 #pragma unroll
         for (int q = 0; q < P1; ++q) {
             const int nxq = L + q + P1 * (j + P1 * k);
             const int nyq = L + i + P1 * (q + P1 * k);
             const int nzq = L + i + P1 * (j + P1 * q);
-            const C cx = Dxi[q], cy = Dyj[q], cz = Dzk[q];
+            // x contraction: sxx, sxy, sxz feed sx, sy, sz
+            Sx[0] += ax[q] * (C)lds[XX][nxq];
+            Sx[1] += ax[q] * (C)lds[XY][nxq];
+            Sx[2] += ax[q] * (C)lds[XZ][nxq];
+            // y contraction: sxy, syy, syz
+            Sy[0] += ay[q] * (C)lds[XY][nyq];
+            Sy[1] += ay[q] * (C)lds[YY][nyq];
+            Sy[2] += ay[q] * (C)lds[YZ][nyq];
+            // z contraction: sxz, syz, szz
+            Sz[0] += az[q] * (C)lds[XZ][nzq];
+            Sz[1] += az[q] * (C)lds[YZ][nzq];
+            Sz[2] += az[q] * (C)lds[ZZ][nzq];
+        }
+        const C rk = gs[G_R * P1 + k];
+        const C Cx = cw[j] * cw[k] * rk * a.d1;
+        const C Cy = cw[i] * cw[k] * rk * a.d2;
+        const C Cz = cw[i] * cw[j] * gs[G_SIN * P1 + i] * a.d3;
+
+        if (valid) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                dxv[c] += (C)lds[c][nxq] * cx;
-                dyv[c] += (C)lds[c][nyq] * cy;
-                dzv[c] += (C)lds[c][nzq] * cz;
+            for (int c = 0; c < 3; ++c)
+                a.force[c * fs + eb] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
+
+            // ---- store tapered stress (K13) and memory variables ----
+            if (a.kahan_sigma == 1) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
+                    a.sig_lo[c * fs + eb] = lo[c] - lo[c] * dmp;
+                }
+            } else if (a.kahan_sigma == 2) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    a.sig_lo2[q * fs + eb] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
+                                                       (float)(lo[2 * q + 1] - lo[2 * q + 1] * dmp));
+            } else {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
+            }
+            if constexpr (VISCO) {
+#pragma unroll
+                for (int c = 0; c < 18; ++c) a.mem[c * fs + eb] = M[c];
             }
         }
-    }
-
-    // ---- K2: spherical completion, K4: symmetrisation ----
-    const ThTab<C> th = a.tth[ex * P1 + i];
-    const RTab<C> rr = a.tr[ez * P1 + k];
-    R exx, eyy, ezz, exy, exz, eyz;
-    {
-        const C ux = vx, uy = vy, uz = vz;
-        const C inv_r = rr.inv_r;
-        const C inv_rs = inv_r * th.inv_sin;
-        const C ezx = -dzv[0];
-        const C exz_c = (dxv[2] - ux) * inv_r;
-        const C ezy = -dzv[1];
-        const C eyz_c = -uy * inv_r + dyv[2] * inv_rs;
-        const C exy_c = dxv[1] * inv_r;
-        const C eyx = -uy * th.cot * inv_r + dyv[0] * inv_rs;
-        exx = (R)((uz + dxv[0]) * inv_r);
-        eyy = (R)(dyv[1] * inv_rs + (uz + ux * th.cot) * inv_r);
-        ezz = (R)(-dzv[2]);
-        exy = (R)((exy_c + eyx) * 0.5);
-        exz = (R)((exz_c + ezx) * 0.5);
-        eyz = (R)((eyz_c + ezy) * 0.5);
-    }
-
-    // ---- K5: stress rate times dt (lam, mu, A, B, C carry the dt) ----
-    R rt[6];
-    const R tmu = R(2) * mu;
-    if constexpr (!VISCO) {
-        const R Lr = lam * (exx + eyy + ezz);
-        rt[XX] = Lr + tmu * exx;
-        rt[YY] = Lr + tmu * eyy;
-        rt[ZZ] = Lr + tmu * ezz;
-        rt[XY] = tmu * exy;
-        rt[XZ] = tmu * exz;
-        rt[YZ] = tmu * eyz;
-    } else {
-        // build-defined SLS (loop form of src/evolution.f90:216-284)
-        R Md[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) Md[c] = M[c] + M[6 + c] + M[12 + c];
-        const R Msum = Md[XX] + Md[YY] + Md[ZZ];
-        // lam holds kappa - 2 mu_tau / 3 and mu holds mu_tau = mu (1 + tau),
-        // both formed in FP64; 2 mu itself only scales the memory variables
-        const R tmut = tmu;                                    // 2 mu_tau
-        const R tm = tmut / (R(1) + tau);                      // 2 mu
-        const R Lr = lam * (exx + eyy + ezz) - tm * Msum * R(1.0 / 3.0);
-        rt[XX] = Lr + tmut * exx + tm * Md[XX];
-        rt[YY] = Lr + tmut * eyy + tm * Md[YY];
-        rt[ZZ] = Lr + tmut * ezz + tm * Md[ZZ];
-        rt[XY] = tmut * exy + tm * Md[XY];
-        rt[XZ] = tmut * exz + tm * Md[XZ];
-        rt[YZ] = tmut * eyz + tm * Md[YZ];
-    }
-    if (S16 ? a.s16.ca != nullptr : a.ca != nullptr) {   // anisotropy (src/evolution.f90:290-296)
-        R pa, pb, pc;
-        if constexpr (S16) {
-            pa = (R)(float)a.s16.ca[eb];
-            pb = (R)(float)a.s16.cb[eb];
-            pc = (R)(float)a.s16.cc[eb];
-        } else {
-            pa = a.ca[eb];
-            pb = a.cb[eb];
-            pc = a.cc[eb];
-        }
-        rt[XX] += pc * ezz + pa * (eyy + exx);
-        rt[YY] += pc * ezz + pa * (eyy + exx);
-        rt[ZZ] += pc * (eyy + exx);
-        rt[XZ] += R(2) * pb * exz;
-        rt[YZ] += R(2) * pb * eyz;
-    }
-
-    // ---- K6: integrate the stress ----
-    if (a.kahan_sigma) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) kahan_add(sg[c], lo[c], rt[c]);
-    } else {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) sg[c] = sg[c] + rt[c];
-    }
-
-    // ---- K7: moment-tensor glut and single force of the previous add() ----
-    // iteration it = *it_dev + 1 uses the STF sample it-1 (zero for it = 1)
-    R src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int it = *a.it_dev + 1;
-    if (valid && sslot >= 0 && it > 1 && it <= a.nt) {
-#pragma unroll
-        for (int c = 0; c < 9; ++c) {
-            const int ch = a.src_chan[sslot * 9 + c];
-            if (ch >= 0) {
-                const double d = a.stf[(long)ch * a.nt + (it - 2)];
-                src[c] = (R)(-(d * a.src_coef[((long)sslot * 9 + c) * NP + node]));
-            }
-        }
-    }
-
-    __syncthreads();   // LDS velocity no longer needed
-    if (inside) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) lds[c][L + node] = sg[c] + src[c];
-    }
-
-    // ---- memory variables (loop form of src/evolution.f90:415-442) ----
-    if constexpr (VISCO) {
-        const R ev[6] = {exx, exy, exz, eyy, eyz, ezz};
-        // M (1 - dt/tau_p) - dt tau / (3 tau_p) e  ==  M - dt/tau_p (M + tau/3 e):
-        // the decrement form keeps the FP32 relaxation rate exact to one
-        // rounding of dt/tau_p (1 - dt/tau_p would lose its low bits)
-        const R t3 = tau * R(1.0 / 3.0);
-#pragma unroll
-        for (int m = 0; m < NRDISS; ++m) {
-#pragma unroll
-            for (int c = 0; c < 6; ++c) M[6 * m + c] -= a.rate[m] * (M[6 * m + c] + t3 * ev[c]);
-        }
-    }
-
-    // ---- taper factor of this node (K13) ----
-    R dmp = R(0);   // 1 - taper
-    {
-        const C *gs = geo[inside ? slot : 0];
-        const C tx = gs[2 * P1 + i], ty = gs[3 * P1 + j], tz = gs[4 * P1 + k];
-        if (tx + ty + tz != C(0)) dmp = (R)taper_damp(tx, ty, tz, a.alpha);
-    }
-
-    __syncthreads();
-
-    // ---- K8: weak divergence ----
-    C ax[P1], ay[P1], az[P1];
-    {
-        const C *dli = cf[3] + i * P1;
-        const C *dlj = cf[3] + j * P1;
-        const C *dlk = cf[3] + k * P1;
-        const C *g = geo[inside ? slot : 0];
-#pragma unroll
-        for (int q = 0; q < P1; ++q) {
-            ax[q] = dli[q] * g[q];
-            ay[q] = dlj[q];
-            az[q] = dlk[q] * g[P1 + q];
-        }
-    }
-    C Sx[3] = {0, 0, 0}, Sy[3] = {0, 0, 0}, Sz[3] = {0, 0, 0};
-#pragma unroll
-    for (int q = 0; q < P1; ++q) {
-        const int nxq = L + q + P1 * (j + P1 * k);
-        const int nyq = L + i + P1 * (q + P1 * k);
-        const int nzq = L + i + P1 * (j + P1 * q);
-        // x contraction: sxx, sxy, sxz feed sx, sy, sz
-        Sx[0] += ax[q] * (C)lds[XX][nxq];
-        Sx[1] += ax[q] * (C)lds[XY][nxq];
-        Sx[2] += ax[q] * (C)lds[XZ][nxq];
-        // y contraction: sxy, syy, syz
-        Sy[0] += ay[q] * (C)lds[XY][nyq];
-        Sy[1] += ay[q] * (C)lds[YY][nyq];
-        Sy[2] += ay[q] * (C)lds[YZ][nyq];
-        // z contraction: sxz, syz, szz
-        Sz[0] += az[q] * (C)lds[XZ][nzq];
-        Sz[1] += az[q] * (C)lds[YZ][nzq];
-        Sz[2] += az[q] * (C)lds[ZZ][nzq];
-    }
-    const C Cx = a.w[j] * a.w[k] * rr.r * a.d1;
-    const C Cy = a.w[i] * a.w[k] * rr.r * a.d2;
-    const C Cz = a.w[i] * a.w[j] * th.sin_t * a.d3;
-
-    if (!valid) return;
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        a.force[c * fs + eb] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
-
-    // ---- store tapered stress (K13) and memory variables ----
-    if (a.kahan_sigma == 1) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
-            a.sig_lo[c * fs + eb] = lo[c] - lo[c] * dmp;
-        }
-    } else if (a.kahan_sigma == 2) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) {
-            a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-            a.sig_lo2[q * fs + eb] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
-                                               (float)(lo[2 * q + 1] - lo[2 * q + 1] * dmp));
-    } else {
-#pragma unroll
-        for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
-    }
-    if constexpr (VISCO) {
-#pragma unroll
-        for (int c = 0; c < 18; ++c) a.mem[c * fs + eb] = M[c];
     }
 }
 
@@ -402,7 +504,7 @@ __global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel_batch(const ElemArgs<R> *args, const int *start, int n)
 {
     const int s = batch_owner(start, n, blockIdx.x);
-    elem_body<P, E, R, VISCO>(args[s], blockIdx.x - start[s]);
+    elem_body<P, E, R, VISCO, false>(args[s], blockIdx.x - start[s]);
 }
 
 // ---------------------------------------------------------------------------
