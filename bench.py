@@ -241,7 +241,7 @@ def batch_bench(args):
     """Uneven batch: independent regional simulations (synth.uneven_batch)
     advanced together by s3d_batch_step (batched launches, graph-replayed
     chunks), against the same simulations stepped one after another."""
-    cases = synth.uneven_batch(seed=4, n_sims=args.batch_sims, nt=args.warmup + 2 * args.steps + 1)
+    cases = synth.uneven_batch(seed=4, n_sims=args.batch_sims, nt=args.warmup + 3 * args.steps + 1)
     N = sum(c.grid.nodes for c in cases)
     t0 = time.time()
     sims = [api.build_case(c, precision="fp32", autotune=True)[0] for c in cases]
@@ -255,20 +255,32 @@ def batch_bench(args):
     b.synchronize()
     wall_b = time.perf_counter() - t0
     b.close()
+    # one simulation at a time, all its steps (graph-replayed chunks): a small
+    # simulation's state stays in the 256 MB Infinity Cache between its steps
     t0 = time.perf_counter()
     for s_ in sims:
         s_.step(args.steps)
     for s_ in sims:
         s_.synchronize()
     wall_s = time.perf_counter() - t0
+    # round robin: every simulation one step per pass, as the batch advances
+    # them (all states stream from HBM every step), launched one by one
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for s_ in sims:
+            s_.step(1)
+    for s_ in sims:
+        s_.synchronize()
+    wall_r = time.perf_counter() - t0
     out = {"metric": "GLL-node updates/s", "workload": "uneven batch of %d regional simulations "
            "(8x8x8 .. 48x48x24 elements, GLL order 4 or 6, layered visco-elastic, FP32 + Kahan v)"
            % len(sims), "gll_nodes": N, "steps": args.steps,
            "batched": {"value": N * args.steps / wall_b, "ms_per_step": 1e3 * wall_b / args.steps},
-           "one_by_one": {"value": N * args.steps / wall_s, "ms_per_step": 1e3 * wall_s / args.steps},
-           "speedup_batched": wall_s / wall_b}
-    log("[batch] batched %.3f ms/step, one by one %.3f ms/step" % (1e3 * wall_b / args.steps,
-                                                                   1e3 * wall_s / args.steps))
+           "round_robin": {"value": N * args.steps / wall_r, "ms_per_step": 1e3 * wall_r / args.steps},
+           "sequential": {"value": N * args.steps / wall_s, "ms_per_step": 1e3 * wall_s / args.steps},
+           "speedup_batched_vs_round_robin": wall_r / wall_b}
+    log("[batch] batched %.3f ms/step, round robin %.3f ms/step, sequential %.3f ms/step"
+        % (1e3 * wall_b / args.steps, 1e3 * wall_r / args.steps, 1e3 * wall_s / args.steps))
     print(json.dumps(out), flush=True)
 
 

@@ -722,7 +722,8 @@ struct BatchGroup {
     bool fp32 = false, visco = false, kahan = false;
     std::vector<s3d_ctx *> sims;
     void *d_eargs = nullptr, *d_nargs = nullptr, *d_rargs = nullptr;
-    int *d_estart = nullptr, *d_nstart = nullptr, *d_rstart = nullptr;
+    // per workgroup of each batched launch: (simulation, its workgroup index)
+    int2 *d_eown = nullptr, *d_nown = nullptr, *d_rown = nullptr;
     int ewg = 0, nwg = 0, rwg = 0;
 };
 
@@ -763,18 +764,29 @@ int batch_build_group_e(BatchGroup &G)
     G.ewg = es[n];
     G.nwg = ns[n];
     G.rwg = rs[n];
+    // the element launch is padded to a multiple of 8 workgroups (idle ones
+    // own simulation -1) so its XCD-aware block order applies
+    auto owners = [n](const std::vector<int> &st, int pad8) {
+        const int total = pad8 ? (st[n] + 7) / 8 * 8 : st[n];
+        std::vector<int2> o(total, make_int2(-1, 0));
+        for (int q = 0; q < n; ++q)
+            for (int w = st[q]; w < st[q + 1]; ++w) o[w] = make_int2(q, w - st[q]);
+        return o;
+    };
+    const std::vector<int2> eo = owners(es, 1), no = owners(ns, 0), ro = owners(rs, 0);
+    G.ewg = (int)eo.size();
     HIPCHK(hipMalloc(&G.d_eargs, sizeof(ElemArgs<R>) * n));
     HIPCHK(hipMalloc(&G.d_nargs, sizeof(NodeArgs<R>) * n));
     HIPCHK(hipMalloc(&G.d_rargs, sizeof(RecArgs<R>) * n));
-    HIPCHK(hipMalloc(&G.d_estart, sizeof(int) * (n + 1)));
-    HIPCHK(hipMalloc(&G.d_nstart, sizeof(int) * (n + 1)));
-    HIPCHK(hipMalloc(&G.d_rstart, sizeof(int) * (n + 1)));
+    HIPCHK(hipMalloc(&G.d_eown, sizeof(int2) * (eo.size() + 1)));
+    HIPCHK(hipMalloc(&G.d_nown, sizeof(int2) * (no.size() + 1)));
+    HIPCHK(hipMalloc(&G.d_rown, sizeof(int2) * (ro.size() + 1)));
     HIPCHK(hipMemcpy(G.d_eargs, ea.data(), sizeof(ElemArgs<R>) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(G.d_nargs, na.data(), sizeof(NodeArgs<R>) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(G.d_rargs, ra.data(), sizeof(RecArgs<R>) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(G.d_estart, es.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(G.d_nstart, ns.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(G.d_rstart, rs.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_eown, eo.data(), sizeof(int2) * eo.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_nown, no.data(), sizeof(int2) * no.size(), hipMemcpyHostToDevice));
+    if (!ro.empty()) HIPCHK(hipMemcpy(G.d_rown, ro.data(), sizeof(int2) * ro.size(), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -782,13 +794,12 @@ template <int P, int E, typename R>
 int batch_launch_group_e(const BatchGroup &G, hipStream_t st)
 {
     using S = Shape<P, E>;
-    const int n = (int)G.sims.size();
     if (G.visco)
         hipLaunchKernelGGL((elem_kernel_batch<P, E, R, true>), dim3(G.ewg), dim3(S::NT), 0, st,
-                           (const ElemArgs<R> *)G.d_eargs, G.d_estart, n);
+                           (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
     else
         hipLaunchKernelGGL((elem_kernel_batch<P, E, R, false>), dim3(G.ewg), dim3(S::NT), 0, st,
-                           (const ElemArgs<R> *)G.d_eargs, G.d_estart, n);
+                           (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -818,17 +829,16 @@ int batch_launch_group(const BatchGroup &G, hipStream_t st)
 template <int P, typename R>
 int batch_launch_node(const BatchGroup &G, hipStream_t st)
 {
-    const int n = (int)G.sims.size();
     if (G.kahan)
         hipLaunchKernelGGL((node_kernel_batch<P, R, true>), dim3(G.nwg), dim3(256), 0, st,
-                           (const NodeArgs<R> *)G.d_nargs, G.d_nstart, n);
+                           (const NodeArgs<R> *)G.d_nargs, (const int2 *)G.d_nown);
     else
         hipLaunchKernelGGL((node_kernel_batch<P, R, false>), dim3(G.nwg), dim3(256), 0, st,
-                           (const NodeArgs<R> *)G.d_nargs, G.d_nstart, n);
+                           (const NodeArgs<R> *)G.d_nargs, (const int2 *)G.d_nown);
     HIPCHK(hipGetLastError());
     if (G.rwg > 0) {
         hipLaunchKernelGGL((record_kernel_batch<P, R>), dim3(G.rwg), dim3(64), 0, st,
-                           (const RecArgs<R> *)G.d_rargs, G.d_rstart, n);
+                           (const RecArgs<R> *)G.d_rargs, (const int2 *)G.d_rown);
         HIPCHK(hipGetLastError());
     }
     return 0;
@@ -856,7 +866,7 @@ int batch_group_call(int what, BatchGroup &G, hipStream_t st)
 void batch_free_groups(s3d_batch *b)
 {
     for (auto &G : b->groups) {
-        void *bufs[] = {G.d_eargs, G.d_nargs, G.d_rargs, G.d_estart, G.d_nstart, G.d_rstart};
+        void *bufs[] = {G.d_eargs, G.d_nargs, G.d_rargs, G.d_eown, G.d_nown, G.d_rown};
         for (void *p : bufs) dfree(p);
     }
     b->groups.clear();

@@ -74,12 +74,23 @@ __device__ inline void kahan_add(R &s, R &c, R inc)
 // lane l writes LDS byte address lds + 4 l. Written as inline assembly so the
 // compiler's wait accounting never sees it: callers issue these loads before
 // any compiler-issued load (vector memory operations complete in issue
-// order, so the compiler's own waits stay exact) and wait with an explicit
-// s_waitcnt vmcnt(0) before reading the LDS words.
+// order, so the compiler's own waits stay exact) and read the LDS words
+// only after wait_vmem_zero.
 __device__ __forceinline__ void lds_dma_dword(const void *g, unsigned lds)
 {
     asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
                  :: "v"(g), "s"(__builtin_amdgcn_readfirstlane(lds)) : "memory");
+}
+
+// s_waitcnt vmcnt(0), returning a zero that the reads of LDS words written by
+// lds_dma_dword must add to their index: the data dependency orders those
+// reads after the wait without a memory clobber (which would make the
+// compiler reload every argument read through a pointer).
+__device__ __forceinline__ int wait_vmem_zero()
+{
+    int z;
+    asm volatile("s_waitcnt vmcnt(0)\n\tv_mov_b32 %0, 0" : "=v"(z));
+    return z;
 }
 
 // Per-lane state of one element tile (EPB elements, one lane per
@@ -179,12 +190,15 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     constexpr bool MDMA = VISCO && sizeof(R) == 4;
     __shared__ R mbuf[MDMA ? 18 : 1][MDMA ? S::NT : 1];
     if constexpr (MDMA) {
+        // operands in registers first: each asm statement clobbers memory,
+        // and argument blocks read through a pointer would be reloaded
         const int e = tile0 * EPB + slot;
         const long eb = (long)((inside && e < a.nel) ? e : 0) * NP + node;
+        const R *mem = a.mem + eb;
         const unsigned base = (unsigned)(size_t)&mbuf[0][(t >> 6) << 6];
 #pragma unroll
         for (int c = 0; c < 18; ++c)
-            lds_dma_dword(a.mem + c * fs + eb, base + (unsigned)(c * S::NT * sizeof(R)));
+            lds_dma_dword(mem + c * fs, base + (unsigned)(c * S::NT * sizeof(R)));
     }
 
     // ---- state of the first tile: issued first, consumed after the tables ----
@@ -305,9 +319,9 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             // build-defined SLS (loop form of src/evolution.f90:216-284)
             if constexpr (MDMA) {
                 // this lane's words were written by its own wave's loads
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                const int z = wait_vmem_zero();
 #pragma unroll
-                for (int c = 0; c < 18; ++c) cur.M[c] = mbuf[c][t];
+                for (int c = 0; c < 18; ++c) cur.M[c] = mbuf[c][t + z];
             }
             R Md[6];
 #pragma unroll
@@ -485,26 +499,15 @@ elem_kernel(const ElemArgs<R> a)
     elem_body<P, E, R, VISCO, S16>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
 }
 
-// Index of the simulation that owns workgroup `wg` of a batched launch:
-// start[s] is the first workgroup of simulation s, start[n] the total.
-__device__ inline int batch_owner(const int *start, int n, int wg)
-{
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (start[mid] <= wg) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Element kernel over a batch of simulations with the same order and
-// precision: one launch, workgroups partitioned by `start`.
 template <int P, int E, typename R, bool VISCO>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
-elem_kernel_batch(const ElemArgs<R> *args, const int *start, int n)
+elem_kernel_batch(const ElemArgs<R> *args, const int2 *own)
 {
-    const int s = batch_owner(start, n, blockIdx.x);
-    elem_body<P, E, R, VISCO, false>(args[s], blockIdx.x - start[s]);
+    // (simulation, its workgroup index), in XCD-aware order over the whole
+    // batch: each XCD works through a contiguous range of simulations
+    const int2 o = own[remap_block(blockIdx.x, gridDim.x, 1)];
+    if (o.x < 0) return;   // padding workgroup
+    elem_body<P, E, R, VISCO, false>(args[o.x], o.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -602,10 +605,10 @@ node_kernel(const NodeArgs<R> a)
 
 template <int P, typename R, bool KV>
 __global__ void __launch_bounds__(256)
-node_kernel_batch(const NodeArgs<R> *args, const int *start, int n)
+node_kernel_batch(const NodeArgs<R> *args, const int2 *own)
 {
-    const int s = batch_owner(start, n, blockIdx.x);
-    node_body<P, R, KV>(args[s], (long)(blockIdx.x - start[s]) * blockDim.x + threadIdx.x);
+    const int2 o = own[blockIdx.x];
+    node_body<P, R, KV>(args[o.x], (long)o.y * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -675,10 +678,10 @@ __global__ void __launch_bounds__(64) record_kernel(const RecArgs<R> a)
 }
 
 template <int P, typename R>
-__global__ void __launch_bounds__(64) record_kernel_batch(const RecArgs<R> *args, const int *start, int n)
+__global__ void __launch_bounds__(64) record_kernel_batch(const RecArgs<R> *args, const int2 *own)
 {
-    const int s = batch_owner(start, n, blockIdx.x);
-    record_body<P, R>(args[s], blockIdx.x - start[s]);
+    const int2 o = own[blockIdx.x];
+    record_body<P, R>(args[o.x], o.y);
 }
 
 // End of one iteration: advance the device step counter. A single lane
