@@ -4,10 +4,12 @@
 ! the reference sources where they lie: src/gll_mod.f90,
 ! src/interpolation_mod.f90, src/coordinates_mod.f90,
 ! src/misc/source_signature_mod.f90, src/misc/coordinate_utilities_mod.f90,
-! with the reference's own test doubles src/examples/dummy_parameters.f90 and
-! src/examples/dummy_error.f90 (the recipe of src/examples/Makefile:40-75).
-! Those doubles fix real_kind = REAL32, so every value printed here is the
-! reference's single-precision result.
+! src/models/ak135_f_mod.F90 with the reference's own test doubles
+! src/examples/dummy_parameters.f90 and src/examples/dummy_error.f90 (the
+! recipe of src/examples/Makefile:40-75). Those doubles fix real_kind =
+! REAL32, so every value printed from these modules is the reference's
+! single-precision result. src/models/prem_mod.F90 is built in its own
+! STAND_ALONE mode (real_kind = 8) and printed with 17 digits.
 !
 ! Reads commands from stdin, one per line, and prints results with
 ! 9 significant digits (enough to represent any REAL32 exactly):
@@ -17,6 +19,9 @@
 !   vandermonde P X Y Z        -> create_3d_vandermonde_matrix, i fastest
 !   wavelet NAME T ONSET WIDTH -> ricker / gauss / sin_3 / brune / delta
 !   colat LAT                  -> deg2rad(lat2colat(LAT))
+!   prem R NOCRUST NOOCEAN     -> prem_iso(R, ...): rho, vp, vs (REAL64)
+!   premti R                   -> prem(R, ...): rho, vpv, vph, vsv, vsh, eta
+!   ak135 X                    -> ak135_f(X, ...): rho, vp, vs
 !   end
 PROGRAM ref_harness
     USE parameters_mod, ONLY : real_kind
@@ -25,11 +30,15 @@ PROGRAM ref_harness
     USE interp_mod, ONLY : calc_lagrange_polynomial, create_3d_vandermonde_matrix
     USE source_signature_mod, ONLY : ricker, gauss, sin_3, brune, delta
     USE coordinate_utilities_mod, ONLY : deg2rad, lat2colat
+    USE prem_mod, ONLY : prem, prem_iso
+    USE ak135_f_mod, ONLY : ak135_f
     IMPLICIT NONE
     CHARACTER(LEN=256) :: line
     CHARACTER(LEN=16) :: cmd, name
     INTEGER :: ios, p, elms, i, j
     REAL(real_kind) :: a, b, x, y, z, t, onset, width
+    REAL(8) :: r8, m8(6)
+    LOGICAL :: nocrust, noocean
     REAL(real_kind), ALLOCATABLE :: c2(:,:), l(:), lx(:), ly(:), lz(:), vm(:,:,:)
 
     DO
@@ -97,6 +106,21 @@ PROGRAM ref_harness
             READ(line, *) cmd, a
             WRITE(*, '(A)') 'colat'
             WRITE(*, '(ES17.9E3)') deg2rad(lat2colat(a))
+        CASE ('prem')
+            READ(line, *) cmd, r8, nocrust, noocean
+            CALL prem_iso(r8, rho=m8(1), vp=m8(2), vs=m8(3), nocrust=nocrust, noocean=noocean)
+            WRITE(*, '(A)') 'prem'
+            WRITE(*, '(ES25.17E3)') m8(1:3)
+        CASE ('premti')
+            READ(line, *) cmd, r8
+            CALL prem(r8, rho=m8(1), vpv=m8(2), vph=m8(3), vsv=m8(4), vsh=m8(5), eta=m8(6))
+            WRITE(*, '(A)') 'premti'
+            WRITE(*, '(ES25.17E3)') m8
+        CASE ('ak135')
+            READ(line, *) cmd, x
+            CALL ak135_f(x, rho=a, vp=b, vs=y)
+            WRITE(*, '(A)') 'ak135'
+            WRITE(*, '(ES17.9E3)') a, b, y
         CASE ('end')
             EXIT
         END SELECT

@@ -6,13 +6,14 @@
 MODULE evolution_mod
     USE, INTRINSIC :: ISO_C_BINDING
     USE ses3d_hip_c
-    USE configuration_mod, ONLY : rk, configuration, abort
+    USE configuration_mod, ONLY : rk, configuration, configuration_typ, abort
     IMPLICIT NONE
     PRIVATE
 
     TYPE(c_ptr), PUBLIC :: ctx = c_null_ptr
 
-    PUBLIC :: init_temp, ses3d_evolution, v_ptp_loc, finalize_evolution
+    PUBLIC :: init_temp, ses3d_evolution, v_ptp_loc, finalize_evolution, &
+              write_model_parameters
 
 CONTAINS
 
@@ -41,6 +42,106 @@ CONTAINS
         CLOSE(lun)
     END SUBROUTINE
 
+    !> Built-in 1-D model at the radii of the GLL points
+    !! (make_model_parameters, src/model_paras.F90:206-343): v(:, ez, k) =
+    !! rhoinv, lambda, mu, a, b, c. r runs from rad_max at the top element
+    !! layer (ez = 0) to rad_min (src/geometric_paras.f90:153-161) and is
+    !! computed here in FP64.
+    SUBROUTINE builtin_profile(cf, v)
+        USE gll_mod, ONLY : get_knots
+        USE earth_models_mod, ONLY : builtin_model
+        TYPE(configuration_typ), INTENT(IN) :: cf
+        REAL(rk), INTENT(OUT) :: v(6, 0:cf%nz - 1, 0:cf%lpd)
+        REAL(rk) :: knots(cf%lpd + 1), r
+        INTEGER :: ez, k
+        LOGICAL :: ok
+
+        knots = 0.5_rk * (1.0_rk + get_knots(cf%lpd))
+        DO k = 0, cf%lpd
+            DO ez = 0, cf%nz - 1
+                r = cf%rad_max + (ez + knots(k + 1)) * ((cf%rad_min - cf%rad_max) / cf%nz)
+                IF (ez == cf%nz - 1 .AND. k == cf%lpd) r = cf%rad_min
+                CALL builtin_model(cf%model_type, r, ez, v(1, ez, k), v(2, ez, k), &
+                                   v(3, ez, k), v(4, ez, k), v(5, ez, k), v(6, ez, k), ok)
+                IF (.NOT. ok) CALL abort('Unknown model_type')
+                v(1, ez, k) = 1.0_rk / v(1, ez, k)
+            END DO
+        END DO
+    END SUBROUTINE
+
+    !> Expands a profile over (ez, k) to the reference's global layout
+    !! (x, y, z elements, then i, j, k GLL indices; x fastest).
+    SUBROUTINE expand_profile(cf, prof, buf)
+        TYPE(configuration_typ), INTENT(IN) :: cf
+        REAL(rk), INTENT(IN) :: prof(0:cf%nz - 1, 0:cf%lpd)
+        REAL(c_double), INTENT(OUT) :: buf(:)
+        INTEGER(8) :: plane, o
+        INTEGER :: ez, k, q
+        plane = INT(cf%nx, 8) * cf%ny
+        o = 0
+        DO k = 0, cf%lpd
+            DO q = 1, (cf%lpd + 1)**2
+                DO ez = 0, cf%nz - 1
+                    buf(o + 1:o + plane) = prof(ez, k)
+                    o = o + plane
+                END DO
+            END DO
+        END DO
+    END SUBROUTINE
+
+    !> Uploads the built-in model selected by model_type.
+    SUBROUTINE set_builtin_model(cf, buf)
+        TYPE(configuration_typ), INTENT(IN) :: cf
+        REAL(c_double), INTENT(INOUT) :: buf(:)
+        INTEGER(c_int), PARAMETER :: ids(6) = [S3D_RHOINV, S3D_LAMBDA, S3D_MU, S3D_A, &
+                                               S3D_B, S3D_C]
+        REAL(rk) :: v(6, 0:cf%nz - 1, 0:cf%lpd)
+        INTEGER :: p
+        CALL builtin_profile(cf, v)
+        DO p = 1, 6
+            CALL expand_profile(cf, v(p, :, :), buf)
+            CALL check(s3d_set_model(ctx, ids(p), buf), 'set_model')
+        END DO
+    END SUBROUTINE
+
+    !> Workflow 'model' (src/ses3d-nt.F90:326-329, write_model_parameters at
+    !! src/model_paras.F90:140-202): writes the built-in model to the
+    !! parameter files named in the &model group, as raw native-endian
+    !! REAL32 in the global layout. Q is written as HUGE (elastic), the
+    !! value the built-in models leave it at. An existing file is replaced
+    !! only with override = .TRUE.
+    SUBROUTINE write_model_parameters()
+        REAL(rk), ALLOCATABLE :: v(:, :, :)
+        REAL(c_double), ALLOCATABLE :: buf(:)
+        INTEGER(8) :: n
+        INTEGER :: p, lun, ios
+        LOGICAL :: there
+
+        ASSOCIATE (cf => configuration)
+        n = INT(cf%nx, 8) * cf%ny * cf%nz * (cf%lpd + 1)**3
+        ALLOCATE(v(6, 0:cf%nz - 1, 0:cf%lpd), buf(n))
+        CALL builtin_profile(cf, v)
+        ! cf%file: rhoinv, lambda, mu, a, b, c, q -- the order of v
+        DO p = 1, 7
+            IF (cf%file(p) == 'N/A') CYCLE
+            INQUIRE(FILE=cf%file(p), EXIST=there)
+            IF (there .AND. .NOT. cf%override) &
+                CALL abort('File "' // TRIM(cf%file(p)) // '" exists; set override = .TRUE.')
+            IF (p <= 6) THEN
+                CALL expand_profile(cf, v(p, :, :), buf)
+            ELSE
+                buf = REAL(HUGE(0.0), c_double)
+            END IF
+            OPEN(NEWUNIT=lun, FILE=cf%file(p), ACCESS='stream', FORM='unformatted', &
+                 STATUS='replace', ACTION='write', IOSTAT=ios)
+            IF (ios /= 0) CALL abort('Cannot write "' // TRIM(cf%file(p)) // '"')
+            WRITE(lun) REAL(buf, 4)
+            CLOSE(lun)
+            WRITE(*, '(" writing: ", A)') TRIM(cf%file(p))
+        END DO
+        END ASSOCIATE
+    END SUBROUTINE
+
     SUBROUTINE init_temp()
         TYPE(s3d_grid) :: g
         TYPE(s3d_options) :: o
@@ -63,15 +164,8 @@ CONTAINS
         CALL check(s3d_create(g, o, ctx), 'create')
         n = INT(cf%nx, 8) * cf%ny * cf%nz * (cf%lpd + 1)**3
         ALLOCATE(buf(n))
-        IF (cf%model_type == 1) THEN
-            ! homogeneous model (src/models/homogeneous.f90): rho 3000,
-            ! mu 7e10, lambda 1.1e11 (single-precision literals)
-            buf = 1.0_c_double / REAL(3000.0, c_double)
-            CALL check(s3d_set_model(ctx, S3D_RHOINV, buf), 'set_model')
-            buf = REAL(1.1e11, c_double)
-            CALL check(s3d_set_model(ctx, S3D_LAMBDA, buf), 'set_model')
-            buf = REAL(7.0e10, c_double)
-            CALL check(s3d_set_model(ctx, S3D_MU, buf), 'set_model')
+        IF (cf%model_type /= -1) THEN
+            CALL set_builtin_model(cf, buf)
         ELSE
             DO p = 1, 7
                 IF (cf%file(p) == 'N/A') CYCLE

@@ -1,6 +1,8 @@
 !> Configuration file reader: the reference's NAMELIST format
 !! (src/parser/parser.f90:148-638) with its consistency checks
 !! (src/config/config.F90:97-232), for one GPU holding the whole mesh.
+!! Workflows 'forward' and 'model'; model_type -1 (parameter files) or one
+!! of the built-in models 1, 2, 3, 11, 12, 13, 14, 15, 21.
 !!
 !! Groups: &grid, &model, &time, &general (exactly one each), &source_mt,
 !! &source_sf, &receiver (any number, read in file order), and the optional
@@ -49,6 +51,7 @@ MODULE configuration_mod
         REAL(rk) :: lat_min, lat_max, lon_min, lon_max, rad_min, rad_max
         INTEGER :: model_type
         CHARACTER(LEN=fnl) :: file(7)   ! rhoinv, lambda, mu, a, b, c, q
+        LOGICAL :: override
         ! &time
         REAL(rk) :: dt
         INTEGER :: nt, date_time(8)
@@ -133,7 +136,8 @@ CONTAINS
         IF (n /= 1) CALL abort('Found ' // i2c(n) // ' &general-groups in config-file!')
         READ(lun, NML=general, IOSTAT=ios)
         IF (ios /= 0) CALL abort('Error while parsing &general-group!')
-        IF (TRIM(workflow) /= 'forward') CALL abort('Unknown workflow ' // TRIM(workflow))
+        IF (TRIM(workflow) /= 'forward' .AND. TRIM(workflow) /= 'model') &
+            CALL abort('Unknown workflow ' // TRIM(workflow))
         cf%workflow = workflow; cf%event_name = event_name; cf%log_file_dir = log_file_dir
 
         ! ---- &model (src/config/config.F90:160-189) ----
@@ -153,16 +157,31 @@ CONTAINS
         IF (ANY([lon_min, lon_max] < -180.0_rk) .OR. ANY([lon_min, lon_max] > 360.0_rk)) &
             CALL abort('Invalid longitude')
         IF (ANY([rad_min, rad_max] < 1.0_rk)) CALL abort('Invalid Radius')
-        IF (model_type == -1) THEN
+        IF (TRIM(workflow) == 'model') THEN
+            ! src/config/model.f90:447-458
+            IF (model_type == -1) &
+                CALL abort("In workflow 'model' a model_type different from -1 must be specified!")
+            IF (ALL([rhoinv, lambda, mu, a, b, c, q] == 'N/A')) &
+                CALL abort("In workflow 'model' at least one model-parameter-filename must be specified!")
+            IF (ALL(model_type /= [1, 2, 3, 11, 12, 13, 14, 15, 21])) &
+                CALL abort('Unknown model_type ' // i2c(model_type))
+        ELSE IF (model_type == -1) THEN
             IF (rhoinv == 'N/A') CALL abort('"rhoinv" model-file must be given!')
             IF (lambda == 'N/A') CALL abort('"lambda" model-file must be given!')
             IF (mu == 'N/A') CALL abort('"mu" model-file must be given!')
-        ELSE IF (model_type /= 1) THEN
-            CALL abort('Only model files and model_type 1 (homogeneous) are supported')
+        ELSE
+            ! src/config/model.f90:469-483
+            IF (rhoinv /= 'N/A') CALL abort("'rhoinv' and a 'model_type' can't be specified at the same time")
+            IF (lambda /= 'N/A') CALL abort("'lambda' and a 'model_type' can't be specified at the same time")
+            IF (mu /= 'N/A') CALL abort("'mu' and a 'model_type' can't be specified at the same time")
+            ! built-in models of src/model_paras.F90:219-336
+            IF (ALL(model_type /= [1, 2, 3, 11, 12, 13, 14, 15, 21])) &
+                CALL abort('Unknown model_type ' // i2c(model_type))
         END IF
         cf%lat_min = lat_min; cf%lat_max = lat_max; cf%lon_min = lon_min; cf%lon_max = lon_max
         cf%rad_min = rad_min; cf%rad_max = rad_max; cf%model_type = model_type
         cf%file = [CHARACTER(LEN=fnl) :: rhoinv, lambda, mu, a, b, c, q]
+        cf%override = override
 
         ! ---- &time (src/config/config.F90:192-210) ----
         nt = -1; date_time = -1; dt = -1.0_rk

@@ -1,14 +1,17 @@
-!> Forward workflow of Ses3d-NT on one MI355X (reference main program,
-!! src/ses3d-nt.F90:30-359, workflow 'forward').
+!> Ses3d-NT on one MI355X (reference main program, src/ses3d-nt.F90:30-359),
+!! workflows 'forward' and 'model'.
 !!
 !!   ses3d_nt <config-file>
 !!
-!! Reads the reference's NAMELIST config, uploads the model, runs nt steps on
-!! the GPU and writes one SAC file per receiver channel.
+!! forward: reads the reference's NAMELIST config, uploads the model, runs nt
+!! steps on the GPU and writes one SAC file per receiver channel.
+!! model: writes the built-in model of model_type to the parameter files
+!! named in the &model group (no GPU needed).
 PROGRAM ses3d_nt
     USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY : OUTPUT_UNIT
     USE configuration_mod, ONLY : configuration, parse_conf_file, abort, rk
-    USE evolution_mod, ONLY : ctx, init_temp, ses3d_evolution, v_ptp_loc, finalize_evolution
+    USE evolution_mod, ONLY : ctx, init_temp, ses3d_evolution, v_ptp_loc, finalize_evolution, &
+                              write_model_parameters
     USE channels_mod, ONLY : source_cls, receiver_cls, parse_source_file, parse_receiver_file
     USE snapshots_mod, ONLY : write_snapshots, check_snapshot_attributes
     IMPLICIT NONE
@@ -22,35 +25,41 @@ PROGRAM ses3d_nt
     IF (COMMAND_ARGUMENT_COUNT() < 1) CALL abort('Config-file missing!')
     CALL GET_COMMAND_ARGUMENT(1, conf_file_name)
     CALL parse_conf_file(TRIM(conf_file_name))
-    CALL check_snapshot_attributes()
-    CALL init_temp()
-    CALL parse_receiver_file(ctx, receivers)
-    WRITE(OUTPUT_UNIT, '("Found ", I0, " receivers.")') SIZE(receivers)
-    CALL parse_source_file(ctx, sources)
-    WRITE(OUTPUT_UNIT, '("Found ", I0, " sources.")') SIZE(sources)
-    WRITE(OUTPUT_UNIT, '(A, /, "Processing workflow ''", A, "'' of event ''", A, "''", /, A)') &
-        REPEAT('=', 50), TRIM(configuration%workflow), TRIM(configuration%event_name), REPEAT('=', 50)
+    IF (TRIM(configuration%workflow) == 'model') THEN
+        CALL write_model_parameters()
+        WRITE(OUTPUT_UNIT, '(A, /, "Workflow ''model'' successfully completed! ", /, A)') &
+            REPEAT('=', 50), REPEAT('=', 50)
+    ELSE
+        CALL check_snapshot_attributes()
+        CALL init_temp()
+        CALL parse_receiver_file(ctx, receivers)
+        WRITE(OUTPUT_UNIT, '("Found ", I0, " receivers.")') SIZE(receivers)
+        CALL parse_source_file(ctx, sources)
+        WRITE(OUTPUT_UNIT, '("Found ", I0, " sources.")') SIZE(sources)
+        WRITE(OUTPUT_UNIT, '(A, /, "Processing workflow ''", A, "'' of event ''", A, "''", /, A)') &
+            REPEAT('=', 50), TRIM(configuration%workflow), TRIM(configuration%event_name), REPEAT('=', 50)
 
-    every = MAX(1, configuration%nt / 10)
-    DO it = 1, configuration%nt
-        CALL ses3d_evolution()
-        DO i = 1, SIZE(sources)
-            CALL sources(i)%add(it)
+        every = MAX(1, configuration%nt / 10)
+        DO it = 1, configuration%nt
+            CALL ses3d_evolution()
+            DO i = 1, SIZE(sources)
+                CALL sources(i)%add(it)
+            END DO
+            DO i = 1, SIZE(receivers)
+                CALL receivers(i)%record(it)
+            END DO
+            CALL write_snapshots(ctx, it)
+            IF (MOD(it, every) == 0 .OR. it == configuration%nt) THEN
+                ptp = v_ptp_loc()
+                WRITE(OUTPUT_UNIT, fmt) it, 100 * it / configuration%nt, ptp
+            END IF
         END DO
+
         DO i = 1, SIZE(receivers)
-            CALL receivers(i)%record(it)
+            CALL receivers(i)%write(ctx)
         END DO
-        CALL write_snapshots(ctx, it)
-        IF (MOD(it, every) == 0 .OR. it == configuration%nt) THEN
-            ptp = v_ptp_loc()
-            WRITE(OUTPUT_UNIT, fmt) it, 100 * it / configuration%nt, ptp
-        END IF
-    END DO
-
-    DO i = 1, SIZE(receivers)
-        CALL receivers(i)%write(ctx)
-    END DO
-    CALL finalize_evolution()
-    WRITE(OUTPUT_UNIT, '(A, /, "Workflow ''forward'' successfully completed! ", /, A)') &
-        REPEAT('=', 50), REPEAT('=', 50)
+        CALL finalize_evolution()
+        WRITE(OUTPUT_UNIT, '(A, /, "Workflow ''forward'' successfully completed! ", /, A)') &
+            REPEAT('=', 50), REPEAT('=', 50)
+    END IF
 END PROGRAM ses3d_nt

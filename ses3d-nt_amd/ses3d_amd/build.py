@@ -21,7 +21,7 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-FORTRAN_SOURCES = ["s3d_c_binding.f90", "gll_mod.f90", "ses3d_config.f90",
+FORTRAN_SOURCES = ["s3d_c_binding.f90", "gll_mod.f90", "ses3d_config.f90", "earth_models.f90",
                    "sac_io.f90", "channels.f90", "evolution_mod.f90",
                    "snapshots.f90", "ses3d_nt.f90"]
 
@@ -71,7 +71,9 @@ def build_fortran():
     objs = []
     for s in srcs:
         o = os.path.join(bdir, os.path.basename(s).replace(".f90", ".o"))
-        _run([FLANG, "-O2", "-module-dir", bdir, "-I", bdir, "-c", s, "-o", o])
+        # the built-in models keep default-kind literals on purpose
+        _run([FLANG, "-O2", "-Wno-real-constant-widening", "-module-dir", bdir, "-I", bdir,
+              "-c", s, "-o", o])
         objs.append(o)
     _run([FLANG, "-O2"] + objs + ["-L" + PKG, "-lses3d_hip", "-Wl,-rpath," + PKG,
                                   "-o", exe])

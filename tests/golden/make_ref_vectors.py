@@ -2,18 +2,40 @@
 """Write tests/golden/ref_vectors.json from the reference's own code.
 
 Runs oracle/_ref/ref_harness, which is the reference's gll_mod,
-interpolation_mod, coordinates_mod, coordinate_utilities_mod and
-source_signature_mod compiled unmodified from /root/reference with the
-reference's own test doubles (real_kind = REAL32). Build it first with
+interpolation_mod, coordinates_mod, coordinate_utilities_mod,
+source_signature_mod and models/ak135_f_mod compiled unmodified from
+/root/reference with the reference's own test doubles (real_kind = REAL32),
+and models/prem_mod in its STAND_ALONE mode (REAL64). Build it first with
 `make -C oracle ref`. The JSON holds only inputs and the harness's outputs.
+
+The Earth-model records are taken at the GLL radii of MODEL_GRIDS (computed
+as the Fortran host computes them) and at the PREM region boundaries.
 """
 import json
 import os
 import subprocess
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+sys.path.insert(0, os.path.join(ROOT, "ses3d-nt_amd"))
+from ses3d_amd import synth  # noqa: E402
+
+# (nz, lpd, rad_max, rad_min) [m]: a crust/upper-mantle box, the whole
+# mantle and core, and a box whose top and bottom are PREM discontinuities
+MODEL_GRIDS = [(16, 4, 6371000.0, 5371000.0), (10, 3, 6371000.0, 1000000.0),
+               (4, 2, 6368000.0, 5701000.0)]
+PREM_BOUNDARIES = [0.0, 1221500.0, 3480000.0, 3630000.0, 5600000.0, 5701000.0,
+                   5771000.0, 5971000.0, 6151000.0, 6291000.0, 6346600.0,
+                   6356000.0, 6368000.0, 6371000.0]
+
+
+def model_radii():
+    rs = set(PREM_BOUNDARIES)
+    for nz, p, top, bottom in MODEL_GRIDS:
+        rs.update(float(v) for v in synth.coords_1d(p, nz, top, bottom).ravel())
+    return sorted(rs)
 
 
 def commands():
@@ -39,6 +61,13 @@ def commands():
                          {"name": name, "t": t, "onset": 40.0, "width": 20.0}))
     for lat in (-20.0, 5.0, 67.0):
         cmds.append(("colat", "colat %r" % lat, {"lat": lat}))
+    for r in model_radii():
+        for nocrust, noocean in ((False, True), (False, False), (True, True)):
+            cmds.append(("prem", "prem %r %s %s" % (r, "TF"[not nocrust], "TF"[not noocean]),
+                         {"r": r, "nocrust": nocrust, "noocean": noocean}))
+        cmds.append(("premti", "premti %r" % r, {"r": r}))
+        if r > 0.0:
+            cmds.append(("ak135", "ak135 %r" % (r / 1000.0), {"x": r / 1000.0}))
     return cmds
 
 
@@ -62,7 +91,7 @@ def main():
         n = {"coords": args.get("elms", 0) * (args.get("p", 0) + 1),
              "lagrange": args.get("p", 0) + 1,
              "vandermonde": (args.get("p", 0) + 1) ** 3,
-             "wavelet": 1, "colat": 1}[kind]
+             "wavelet": 1, "colat": 1, "prem": 3, "premti": 6, "ak135": 3}[kind]
         assert lines[pos].strip() == kind, (lines[pos], kind)
         vals = [float(v) for v in lines[pos + 1:pos + 1 + n]]
         pos += 1 + n
@@ -70,8 +99,10 @@ def main():
     with open(os.path.join(HERE, "ref_vectors.json"), "w") as f:
         json.dump({"source": "reference src/gll_mod.f90, src/interpolation_mod.f90, "
                              "src/coordinates_mod.f90, src/misc/coordinate_utilities_mod.f90, "
-                             "src/misc/source_signature_mod.f90 built with "
-                             "src/examples/dummy_parameters.f90 (REAL32)",
+                             "src/misc/source_signature_mod.f90, "
+                             "src/models/ak135_f_mod.F90 built with "
+                             "src/examples/dummy_parameters.f90 (REAL32); "
+                             "src/models/prem_mod.F90 built with -DSTAND_ALONE (REAL64)",
                    "records": records}, f, indent=0)
     print("wrote %d records" % len(records))
 

@@ -72,29 +72,35 @@ def node_coords(o):
     return o.get(oracle.THETA), o.get(oracle.PHI), o.get(oracle.R)
 
 
-def write_config(case, d, precision="fp32", nt=None, extra=""):
+def write_config(case, d, precision="fp32", nt=None, extra="", model=None, workflow="forward"):
     """Write the reference's NAMELIST config and raw float64 model files for a
-    synth.Case into directory d; returns the config path."""
+    synth.Case into directory d; returns the config path. With `model` (a
+    dict of &model entries such as model_type and output file names) no
+    model files are written and those entries take their place."""
     import os
     g = case.grid
-    m = case.model_full()
     files = {}
-    for k in ("rhoinv", "lambda", "mu", "q"):
-        if k in m:
-            p = os.path.join(d, k)
-            np.ascontiguousarray(m[k], dtype="<f8").tofile(p)
-            files[k] = p
+    if model is None:
+        m = case.model_full()
+        for k in ("rhoinv", "lambda", "mu", "q"):
+            if k in m:
+                p = os.path.join(d, k)
+                np.ascontiguousarray(m[k], dtype="<f8").tofile(p)
+                files[k] = "'%s'" % p
+    else:
+        files = {k: (v if isinstance(v, str) and v.startswith(".") else repr(v))
+                 for k, v in model.items()}
     lines = ["&grid", " nx = %d" % g.nx, " ny = %d" % g.ny, " nz = %d" % g.nz,
              " lpd = %d" % g.lpd, " taper_width = %d" % g.taper_width,
              " taper_slope = %r" % g.taper_slope, "/", "",
              "&model", " lat_min = %r, lat_max = %r," % (g.lat_min, g.lat_max),
              " lon_min = %r, lon_max = %r," % (g.lon_min, g.lon_max),
              " rad_min = %r, rad_max = %r," % (g.rad_min, g.rad_max)]
-    for k, p in files.items():
-        lines.append(" %s = '%s'" % (k, p))
+    for k, v in files.items():
+        lines.append(" %s = %s" % (k, v))
     lines += ["/", "", "&time", " dt = %r" % float(g.dt), " nt = %d" % (nt or g.nt),
               " date_time = 2009, 8, 20, 0, 6, 35, 0, 0", "/", "",
-              "&general", " workflow = 'forward'", " event_name = 'synthetic'",
+              "&general", " workflow = '%s'" % workflow, " event_name = 'synthetic'",
               " log_file_dir = './'", "/", "",
               "&gpu", " precision = '%s'" % precision, "/", ""]
     for s in case.sources:

@@ -76,3 +76,37 @@ def test_forward_run_matches_oracle(tmp_path, precision, tol):
             assert i[9] == nt and abs(f[0] - case.grid.dt) < 1e-6 * case.grid.dt
             # SAC stores float32: the floor is single-precision rounding
             assert rel_l2(data, o.trace(ri, ci)) < tol, (rec.station, ch)
+
+
+@pytest.mark.gpu
+def test_forward_builtin_prem_matches_oracle(tmp_path):
+    """model_type 3 (PREM, crust without ocean) built by the host and uploaded
+    in FP64, against the oracle run on the model that workflow 'model' writes
+    for the same grid (REAL32 files: agreement to single precision)."""
+    import oracle
+    nt = 200
+    case = synth.homogeneous(nt=nt, nx=8, ny=8, nz=5, n_rec=5)
+    case.grid.dt = synth.cfl_dt(case.grid, lambda r: np.full_like(r, 10000.0), 0.3)
+    names = ("rhoinv", "lambda", "mu")
+    mdir = tmp_path / "model"
+    mdir.mkdir()
+    cfg = write_config(case, str(mdir), workflow="model",
+                       model=dict([("model_type", 3)] +
+                                  [(k, os.path.join(str(mdir), k)) for k in names]))
+    r = run(cfg)
+    assert r.returncode == 0, r.stderr
+    g = case.grid
+    shp = (g.lpd + 1,) * 3 + (g.nz, g.ny, g.nx)
+    m = {k: np.fromfile(os.path.join(str(mdir), k), dtype="<f4").astype(np.float64).reshape(shp)
+         for k in names}
+    case.model_full = lambda: m
+    cfg = write_config(case, str(tmp_path), precision="fp64", model={"model_type": 3})
+    r = run(cfg)
+    print(r.stdout[-600:], r.stderr[-600:])
+    assert r.returncode == 0
+    o, recs = oracle.build_case(case)
+    o.step(nt)
+    for ri, rec in enumerate(recs):
+        for ci, ch in enumerate(rec.channels):
+            _, _, data = read_sac(os.path.join(str(tmp_path), "XX.00.%s.%s.sac" % (rec.station, ch)))
+            assert rel_l2(data, o.trace(ri, ci)) < 1e-5, (rec.station, ch)
