@@ -10,7 +10,7 @@ MODULE channels_mod
     USE, INTRINSIC :: ISO_C_BINDING
     USE ses3d_hip_c
     USE configuration_mod, ONLY : rk, configuration, source_entry, receiver_entry, abort
-    USE sac_io_mod, ONLY : write_sac, sac_file_name
+    USE sac_io_mod, ONLY : write_sac, read_sac, sac_file_name
     IMPLICIT NONE
     PRIVATE
 
@@ -63,14 +63,58 @@ CONTAINS
         CLOSE(lun)
     END SUBROUTINE
 
+    !> A &source_sac trace (create_from_sac, src/config/source.f90:142-203):
+    !! location from STLA/STLO/STDP, one channel named by KCMPNM, signature
+    !! data * SCALE. On return stf6 (moment tensor, xx xy xz yy yz zz) or stf3
+    !! (single force, x y z) holds the signature in its channel and zeros
+    !! elsewhere; kind = 1 (moment), 2 (force) or 0 (unknown channel).
+    SUBROUTINE sac_source(s, nt, stf6, stf3, kind)
+        TYPE(source_entry), INTENT(INOUT) :: s
+        INTEGER, INTENT(IN) :: nt
+        REAL(c_double), INTENT(OUT) :: stf6(6 * nt), stf3(3 * nt)
+        INTEGER, INTENT(OUT) :: kind
+        REAL(8), ALLOCATABLE :: data(:)
+        REAL(8) :: scale
+        CHARACTER(LEN=8) :: ch
+        INTEGER :: ios, c
+
+        CALL read_sac(TRIM(s%file), data, s%lat, s%lon, s%depth, scale, ch, ios)
+        IF (ios /= 0) CALL abort('Cannot read SAC-file "' // TRIM(s%file) // '"')
+        IF (SIZE(data) < nt) CALL abort('SAC-trace too short!')
+        stf6 = 0.0_c_double
+        stf3 = 0.0_c_double
+        ! channel names of src/config/channel_mod.f90:300-330
+        SELECT CASE (TRIM(ch))
+        CASE ('mxx'); kind = 1; c = 1
+        CASE ('mxy', 'myx'); kind = 1; c = 2
+        CASE ('mxz', 'mzx'); kind = 1; c = 3
+        CASE ('myy'); kind = 1; c = 4
+        CASE ('myz', 'mzy'); kind = 1; c = 5
+        CASE ('mzz'); kind = 1; c = 6
+        CASE ('fx'); kind = 2; c = 1
+        CASE ('fy'); kind = 2; c = 2
+        CASE ('fz'); kind = 2; c = 3
+        CASE DEFAULT
+            WRITE(*, '("WARNING: Unknown attribute ''", A, "'' in SAC source.")') TRIM(ch)
+            kind = 0
+            RETURN
+        END SELECT
+        IF (kind == 1) THEN
+            stf6((c - 1) * nt + 1:c * nt) = data(1:nt) * scale
+        ELSE
+            stf3((c - 1) * nt + 1:c * nt) = data(1:nt) * scale
+        END IF
+    END SUBROUTINE
+
     !> Locate every source, build its signature and hand it to the device
-    !! (src/config/source.f90:274-327, 449-536).
+    !! (src/config/source.f90:274-327, 449-536). At least one source must
+    !! lie in the domain.
     SUBROUTINE parse_source_file(ctx, sources)
         TYPE(c_ptr), INTENT(IN) :: ctx
         TYPE(source_cls), ALLOCATABLE, INTENT(OUT) :: sources(:)
-        INTEGER :: i, c, nt, n
+        INTEGER :: i, c, nt, n, kind
         INTEGER(c_int) :: exyz(3), code
-        REAL(c_double), ALLOCATABLE :: vm(:), stf(:), comp(:)
+        REAL(c_double), ALLOCATABLE :: vm(:), stf(:), comp(:), comp3(:)
         TYPE(source_entry) :: s
         TYPE(source_cls), ALLOCATABLE :: tmp(:)
 
@@ -79,35 +123,55 @@ CONTAINS
         n = 0
         DO i = 1, SIZE(configuration%sources)
             s = configuration%sources(i)
+            IF (s%sac) THEN
+                ALLOCATE(comp(6 * nt), comp3(3 * nt))
+                CALL sac_source(s, nt, comp, comp3, kind)
+                IF (kind == 0) THEN
+                    DEALLOCATE(comp, comp3)
+                    CYCLE
+                END IF
+            END IF
             IF (s3d_locate(ctx, s%lat, s%lon, s%depth, exyz, vm) /= 0) THEN
                 WRITE(*, '("WARNING: source (", G0, ", ", G0, ", ", G0, ") not in computational domain!")') &
                     s%lat, s%lon, s%depth
+                IF (ALLOCATED(comp)) DEALLOCATE(comp)
+                IF (ALLOCATED(comp3)) DEALLOCATE(comp3)
                 CYCLE
             END IF
-            code = wavelet_code(s%wavelet)
-            IF (code >= 0) THEN
-                CALL check(s3d_wavelet(code, s%onset, s%width, configuration%dt, INT(nt, c_int), stf), &
-                           'wavelet')
+            IF (s%sac) THEN
+                IF (kind == 1) THEN
+                    CALL check(s3d_add_source_mt(ctx, exyz, vm, comp), 'add_source_mt')
+                ELSE
+                    CALL check(s3d_add_source_sf(ctx, exyz, vm, comp3), 'add_source_sf')
+                END IF
+                DEALLOCATE(comp, comp3)
             ELSE
-                CALL read_stf(s%wavelet, stf)
+                code = wavelet_code(s%wavelet)
+                IF (code >= 0) THEN
+                    CALL check(s3d_wavelet(code, s%onset, s%width, configuration%dt, INT(nt, c_int), stf), &
+                               'wavelet')
+                ELSE
+                    CALL read_stf(s%wavelet, stf)
+                END IF
+                IF (s%moment) THEN
+                    ALLOCATE(comp(6 * nt))
+                    DO c = 1, 6
+                        comp((c - 1) * nt + 1:c * nt) = stf * s%mt(c)
+                    END DO
+                    CALL check(s3d_add_source_mt(ctx, exyz, vm, comp), 'add_source_mt')
+                ELSE
+                    ALLOCATE(comp(3 * nt))
+                    DO c = 1, 3
+                        comp((c - 1) * nt + 1:c * nt) = stf * s%dir(c)
+                    END DO
+                    CALL check(s3d_add_source_sf(ctx, exyz, vm, comp), 'add_source_sf')
+                END IF
+                DEALLOCATE(comp)
             END IF
-            IF (s%moment) THEN
-                ALLOCATE(comp(6 * nt))
-                DO c = 1, 6
-                    comp((c - 1) * nt + 1:c * nt) = stf * s%mt(c)
-                END DO
-                CALL check(s3d_add_source_mt(ctx, exyz, vm, comp), 'add_source_mt')
-            ELSE
-                ALLOCATE(comp(3 * nt))
-                DO c = 1, 3
-                    comp((c - 1) * nt + 1:c * nt) = stf * s%dir(c)
-                END DO
-                CALL check(s3d_add_source_sf(ctx, exyz, vm, comp), 'add_source_sf')
-            END IF
-            DEALLOCATE(comp)
             n = n + 1
             tmp(n) = source_cls(exyz(1), exyz(2), exyz(3))
         END DO
+        IF (n < 1) CALL abort('No sources defined in config file. At least one source must be specified.')
         sources = tmp(1:n)
     END SUBROUTINE
 

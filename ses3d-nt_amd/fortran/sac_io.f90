@@ -1,12 +1,14 @@
-!> Binary SAC trace writer (float32 data, 158-word header) with the fields
-!! the reference's receiver output sets (src/sac_io/sac_io_mod.f90:178-222,
-!! 453-502) and its file name NETWORK.LOCATION.STATION.CHANNEL.sac
-!! (auto_file_name, src/sac_io/sac_io_mod.f90:407-416).
+!> Binary SAC traces (float32 data, 158-word header, native endian): the
+!! writer with the fields the reference's receiver output sets
+!! (src/sac_io/sac_io_mod.f90:178-222, 453-502) and its file name
+!! NETWORK.LOCATION.STATION.CHANNEL.sac (auto_file_name,
+!! src/sac_io/sac_io_mod.f90:407-416), and the reader of the header fields a
+!! &source_sac trace carries (read_sac, src/sac_io/sac_io_mod.f90:229-288).
 MODULE sac_io_mod
     USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY : REAL32, INT32
     IMPLICIT NONE
     PRIVATE
-    PUBLIC :: write_sac, sac_file_name
+    PUBLIC :: write_sac, read_sac, sac_file_name
 
 CONTAINS
 
@@ -78,6 +80,40 @@ CONTAINS
         WRITE(lun) k
         WRITE(lun) REAL(data, REAL32)
         CLOSE(lun)
+    END SUBROUTINE
+
+    !> Reads the first data section of a SAC file and the header fields
+    !! STLA, STLO, STDP, SCALE and KCMPNM. iostat /= 0 when the file cannot
+    !! be read or is truncated.
+    SUBROUTINE read_sac(file, data, lat, lon, depth, scale, channel, iostat)
+        CHARACTER(LEN=*), INTENT(IN) :: file
+        REAL(8), ALLOCATABLE, INTENT(OUT) :: data(:)
+        REAL(8), INTENT(OUT) :: lat, lon, depth, scale
+        CHARACTER(LEN=8), INTENT(OUT) :: channel
+        INTEGER, INTENT(OUT) :: iostat
+        REAL(REAL32) :: f(70)
+        INTEGER(INT32) :: iv(35), lv(5)
+        CHARACTER(LEN=8) :: k(24)
+        REAL(REAL32), ALLOCATABLE :: d(:)
+        INTEGER :: lun
+
+        OPEN(NEWUNIT=lun, FILE=file, ACCESS='stream', FORM='unformatted', &
+             ACTION='read', STATUS='old', IOSTAT=iostat)
+        IF (iostat /= 0) RETURN
+        READ(lun, IOSTAT=iostat) f, iv, lv, k
+        IF (iostat == 0 .AND. iv(10) < 0) iostat = -1
+        IF (iostat == 0) THEN
+            ALLOCATE(d(iv(10)))                     ! NPTS
+            READ(lun, IOSTAT=iostat) d
+        END IF
+        CLOSE(lun)
+        IF (iostat /= 0) RETURN
+        data = REAL(d, 8)
+        scale = REAL(f(4), 8)                       ! SCALE
+        lat = REAL(f(32), 8)                        ! STLA
+        lon = REAL(f(33), 8)                        ! STLO
+        depth = REAL(f(35), 8)                      ! STDP
+        channel = ADJUSTL(k(21))                    ! KCMPNM
     END SUBROUTINE
 
 END MODULE sac_io_mod

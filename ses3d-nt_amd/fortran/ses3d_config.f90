@@ -5,7 +5,8 @@
 !! of the built-in models 1, 2, 3, 11, 12, 13, 14, 15, 21.
 !!
 !! Groups: &grid, &model, &time, &general (exactly one each), &source_mt,
-!! &source_sf, &receiver (any number, read in file order), and the optional
+!! &source_sf, &source_sac, &receiver (any number, read in file order), and
+!! the optional
 !! &gpu group of this build: precision ('fp32' | 'fp64'), kahan_v,
 !! kahan_sigma, kahan_sigma_bf16 (stress compensation words in BF16),
 !! bf16_static (moduli, tau and dt/M stored in BF16), device.
@@ -22,10 +23,13 @@ MODULE configuration_mod
 
     TYPE, PUBLIC :: source_entry
         LOGICAL :: moment = .TRUE.
-        REAL(rk) :: lat, lon, depth, onset, width
+        LOGICAL :: sac = .FALSE.                ! &source_sac: everything from the file
+        CHARACTER(LEN=fnl) :: file = ' '
+        REAL(rk) :: lat = 0.0_rk, lon = 0.0_rk, depth = 0.0_rk
+        REAL(rk) :: onset = 0.0_rk, width = 0.0_rk
         REAL(rk) :: mt(6) = 0.0_rk
         REAL(rk) :: dir(3) = 0.0_rk
-        CHARACTER(LEN=fnl) :: wavelet
+        CHARACTER(LEN=fnl) :: wavelet = ' '
     END TYPE
 
     TYPE, PUBLIC :: receiver_entry
@@ -216,20 +220,24 @@ CONTAINS
         configuration = cf
     END SUBROUTINE
 
-    !> &source_mt and &source_sf groups (src/parser/parser.f90:265-365)
+    !> &source_mt, &source_sf and &source_sac groups
+    !! (src/parser/parser.f90:265-411); the &source_sac entries follow the
+    !! others, as in parse_source_file (src/config/source.f90:449-530).
     SUBROUTINE parse_sources(lun, cf)
         INTEGER, INTENT(IN) :: lun
         TYPE(configuration_typ), INTENT(INOUT) :: cf
         REAL(rk) :: lat, lon, depth, onset, width, moment_tensor(6), direction(3)
-        CHARACTER(LEN=fnl) :: wavelet
+        CHARACTER(LEN=fnl) :: wavelet, file_name
         NAMELIST /source_mt/ lat, lon, depth, moment_tensor, wavelet, onset, width
         NAMELIST /source_sf/ lat, lon, depth, direction, wavelet, onset, width
-        INTEGER :: nmt, nsf, i, ios
+        NAMELIST /source_sac/ file_name
+        INTEGER :: nmt, nsf, nsac, i, ios
         TYPE(source_entry) :: s
 
         nmt = count_groups(lun, 'source_mt')
         nsf = count_groups(lun, 'source_sf')
-        ALLOCATE(cf%sources(nmt + nsf))
+        nsac = count_groups(lun, 'source_sac')
+        ALLOCATE(cf%sources(nmt + nsf + nsac))
         REWIND(lun)
         DO i = 1, nmt
             lat = default_real; lon = default_real; depth = default_real
@@ -255,6 +263,17 @@ CONTAINS
             s%lat = lat; s%lon = lon; s%depth = depth; s%onset = onset; s%width = width
             s%dir = direction; s%wavelet = wavelet
             cf%sources(nmt + i) = s
+        END DO
+        REWIND(lun)
+        DO i = 1, nsac
+            file_name = ''
+            READ(lun, NML=source_sac, IOSTAT=ios)
+            IF (ios /= 0) CALL abort('Error while parsing &source_sac-group!')
+            IF (LEN_TRIM(file_name) == 0) CALL abort('&source_sac: file_name missing!')
+            s = source_entry()
+            s%sac = .TRUE.
+            s%file = file_name
+            cf%sources(nmt + nsf + i) = s
         END DO
     END SUBROUTINE
 

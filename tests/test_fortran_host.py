@@ -110,3 +110,79 @@ def test_forward_builtin_prem_matches_oracle(tmp_path):
         for ci, ch in enumerate(rec.channels):
             _, _, data = read_sac(os.path.join(str(tmp_path), "XX.00.%s.%s.sac" % (rec.station, ch)))
             assert rel_l2(data, o.trace(ri, ci)) < 1e-5, (rec.station, ch)
+
+
+def write_sac_source(path, data, lat, lon, depth, scale, channel, dt):
+    """A binary SAC file (native little endian) with the header fields a
+    &source_sac trace is read from."""
+    f = np.full(70, -12345.0, dtype="<f4")
+    f[0], f[3], f[31], f[32], f[34] = dt, scale, lat, lon, depth   # DELTA SCALE STLA STLO STDP
+    iv = np.full(35, -12345, dtype="<i4")
+    iv[6], iv[9], iv[15] = 6, len(data), 1                           # NVHDR NPTS IFTYPE
+    lv = np.array([1, 0, 0, 0, 0], dtype="<i4")
+    k = bytearray(b"-12345  " * 24)
+    k[160:168] = channel.ljust(8).encode()                          # KCMPNM
+    with open(path, "wb") as fh:
+        fh.write(f.tobytes() + iv.tobytes() + lv.tobytes() + bytes(k) +
+                 np.asarray(data, dtype="<f4").tobytes())
+
+
+def test_source_sac_needs_file_name(tmp_path):
+    case = synth.homogeneous(nt=60, nx=6, ny=6, nz=5, n_rec=2)
+    cfg = write_config(case, str(tmp_path), extra="\n&source_sac\n/\n")
+    r = run(cfg)
+    assert r.returncode != 0 and "file_name missing" in r.stderr
+
+
+@pytest.mark.gpu
+def test_source_sac_matches_config_sources(tmp_path):
+    """&source_sac traces (mxx and fz channels) give the same seismograms,
+    bit for bit, as &source_mt / &source_sf groups with the same location,
+    signature and amplitude (all chosen exactly representable in float32,
+    so data * SCALE equals stf * amplitude in FP64). A short trace aborts;
+    an unknown channel drops the source."""
+    nt = 150
+    case = synth.homogeneous(nt=nt, nx=8, ny=8, nz=5, n_rec=4)
+    dt = case.grid.dt
+    t = np.arange(nt) * dt
+    stf = np.exp(-((t - 20.0) / 8.0) ** 2).astype(np.float32)
+    m, fz = float(np.float32(3e18)), float(np.float32(2e12))
+    a, b = (0.5, -0.25, 40000.0), (-1.0, 1.5, 30000.0)
+
+    d1 = tmp_path / "sac"
+    d1.mkdir()
+    write_sac_source(str(d1 / "m.sac"), stf, *a, m, "mxx", dt)
+    write_sac_source(str(d1 / "f.sac"), stf, *b, fz, "fz", dt)
+    case.sources = []
+    extra = "".join("\n&source_sac\n file_name = '%s'\n/\n" % str(d1 / n) for n in ("m.sac", "f.sac"))
+    r = run(write_config(case, str(d1), precision="fp64", extra=extra))
+    assert r.returncode == 0, r.stderr
+    assert "Found 2 sources." in r.stdout
+
+    d2 = tmp_path / "cfg"
+    d2.mkdir()
+    stf_file = str(d2 / "stf.txt")
+    with open(stf_file, "w") as fh:
+        fh.write("\n".join(repr(float(v)) for v in stf) + "\n")
+    case.sources = [synth.Source(*a, (m, 0.0, 0.0, 0.0, 0.0, 0.0), wavelet=stf_file)]
+    extra = ("\n&source_sf\n lat = %r\n lon = %r\n depth = %r\n direction = 0.0 0.0 %r\n"
+             " wavelet = '%s'\n onset = 0.0\n width = 0.0\n/\n" % (b + (fz, stf_file)))
+    r = run(write_config(case, str(d2), precision="fp64", extra=extra))
+    assert r.returncode == 0, r.stderr
+    for rec in case.receivers:
+        for ch in rec.channels:
+            name = "XX.00.%s.%s.sac" % (rec.station, ch)
+            _, _, x1 = read_sac(str(d1 / name))
+            _, _, x2 = read_sac(str(d2 / name))
+            assert np.max(np.abs(x2)) > 0 and np.array_equal(x1, x2), name
+
+    d3 = tmp_path / "short"
+    d3.mkdir()
+    write_sac_source(str(d3 / "m.sac"), stf[:nt - 1], *a, m, "mxx", dt)
+    write_sac_source(str(d3 / "v.sac"), stf, *a, m, "vx", dt)
+    case.sources = []
+    r = run(write_config(case, str(d3), extra="\n&source_sac\n file_name = '%s'\n/\n" % str(d3 / "m.sac")))
+    assert r.returncode != 0 and "SAC-trace too short" in r.stderr
+    r = run(write_config(case, str(d3), extra="\n&source_sac\n file_name = '%s'\n/\n" % str(d3 / "v.sac")))
+    assert r.returncode != 0 and "Unknown attribute 'vx'" in r.stdout
+    assert "No sources defined" in r.stderr
