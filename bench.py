@@ -284,6 +284,62 @@ def batch_bench(args):
     print(json.dumps(out), flush=True)
 
 
+def multi_source_bench(args):
+    """Many sources sharing one Earth model: one simulation per source on a
+    single copy of the model (Simulation.shared), advanced by s3d_batch_step
+    with the members' workgroups source-innermost, against the same
+    simulations stepped round robin (whole mesh per source, no sharing of
+    the static-field reads)."""
+    mode = args.mode
+    kw = dict(MODES[mode])
+    if kw.pop("bf16_static"):
+        raise SystemExit("BF16 static fields are not supported in a batch")
+    case = synth.multi_source(seed=5, n_src=args.sources, nt=args.warmup + 2 * args.steps + 1)
+    g = case.grid
+    N = g.nodes
+    S = len(case.sources)
+    u = g.unique_nodes / N
+    t0 = time.time()
+    sims, recs = api.build_shared_sources(case, slab=args.slab, autotune=True, **kw)
+    dev = sum(s_.device_bytes() for s_ in sims)
+    log("[multi-source] %dx%dx%d p=%d, %d sources, %d receivers, %d GLL nodes per source, "
+        "device %.1f GB, setup %.0f s" % (g.nx, g.ny, g.nz, g.lpd, S, len(recs), N, dev / 1e9,
+                                          time.time() - t0))
+    b = api.Batch(sims)
+    b.step(args.warmup)
+    b.synchronize()
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    b.synchronize()
+    wall_b = time.perf_counter() - t0
+    b.close()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for s_ in sims:
+            s_.step(1)
+    for s_ in sims:
+        s_.synchronize()
+    wall_r = time.perf_counter() - t0
+    B = step_bytes(mode, u, True, S=S)
+    value = N * S * args.steps / wall_b
+    out = {"metric": "GLL-node updates/s (node x source)",
+           "workload": "%d moment-tensor sources on one layered visco-elastic model, %dx%dx%d "
+                       "elements, GLL order %d, %s" % (S, g.nx, g.ny, g.nz, g.lpd, mode),
+           "gll_nodes_per_source": N, "sources": S, "steps": args.steps,
+           "batched_shared": {"value": value, "ms_per_step": 1e3 * wall_b / args.steps},
+           "round_robin": {"value": N * S * args.steps / wall_r,
+                           "ms_per_step": 1e3 * wall_r / args.steps},
+           "speedup_batched_vs_round_robin": wall_r / wall_b,
+           "step_roofline": {"bytes_per_node_source": B, "achieved_GBps": B * value / 1e9,
+                             "frac": B * value / HBM_PEAK},
+           "device_bytes": dev}
+    log("[multi-source] batched %.3f ms/step (%.3e node-source updates/s, %.1f %% of 8 TB/s at "
+        "%.1f B), round robin %.3f ms/step" % (1e3 * wall_b / args.steps, value,
+                                              100 * B * value / HBM_PEAK, B,
+                                              1e3 * wall_r / args.steps))
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +357,10 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="uneven batch of independent simulations instead of the single mesh")
     ap.add_argument("--batch-sims", type=int, default=64)
+    ap.add_argument("--multi-source", action="store_true",
+                    help="many sources sharing one model (128x128x48, p=4) instead of the single mesh")
+    ap.add_argument("--sources", type=int, default=16,
+                    help="sources per shared-model batch (16 fit in HBM at 128x128x48)")
     ap.add_argument("--precision-study", action="store_true",
                     help="FP32 variants vs FP64 seismogram / snapshot misfit instead of timing")
     ap.add_argument("--study-steps", type=int, default=4000)
@@ -322,6 +382,9 @@ def main():
         return
     if args.batch:
         batch_bench(args)
+        return
+    if args.multi_source:
+        multi_source_bench(args)
         return
 
     if args.copy_probe > 0:

@@ -87,8 +87,20 @@ int s3d_device_count(int *count);
  * allocate_elastic_variables (reference src/ses3d-nt.F90:128-146). */
 int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out);
 
-/* Release everything the context owns. */
+/* Release everything the context owns. Fails while simulations created with
+ * s3d_create_shared still use this context's model. */
 int s3d_destroy(s3d_ctx *ctx);
+
+/* A further simulation on the mesh and model of an initialised one: it uses
+ * `model`'s static fields (moduli, tau, A/B/C, dt/M, mass matrix, geometry
+ * tables, taper) and options and element tile, and owns only its state
+ * (velocity, stress, memory variables, compensation, force), its sources,
+ * receivers and step counter; it is ready to step (no s3d_init). The model's
+ * simulation must outlive it. No counterpart in the reference, which
+ * superposes several sources in one wavefield; this is the batch axis of
+ * many sources sharing one Earth model (batched source-innermost by
+ * s3d_batch_create). */
+int s3d_create_shared(s3d_ctx *model, s3d_ctx **out);
 
 /* Model input: element layers ez0 .. ez0+nez-1 of one parameter, float64 in
  * the reference layout with nz replaced by nez. Parameters never set keep the
@@ -198,7 +210,11 @@ int s3d_device_bytes(s3d_ctx *ctx, double *bytes);
  * and step counters, and may differ in mesh size and GLL order. No
  * counterpart in the reference, which runs one event per process group;
  * this is the batch axis of the workloads with many independent regional
- * simulations. A member may not be stepped on its own while its batch steps. */
+ * simulations. A member may not be stepped on its own while its batch steps.
+ * When every member of a group shares one model (s3d_create_shared), the
+ * group's workgroups run source-innermost: all members' workgroups of one
+ * element tile follow each other, and the shared static fields are read
+ * from HBM once per tile. */
 typedef struct s3d_batch s3d_batch;
 int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out);
 int s3d_batch_step(s3d_batch *batch, int nsteps);

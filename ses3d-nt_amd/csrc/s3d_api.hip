@@ -195,6 +195,11 @@ struct s3d_ctx {
     bool use_graph = true;
     int epb = 0;               // element tile (0 = default), see s3d_autotune
     bool fields_set = false;   // s3d_set_field was called (state not at rest)
+    // s3d_create_shared: the simulation whose static fields (moduli, tau, rho,
+    // A/B/C, dt/M, FP64 mass, 1-D tables, taper profiles) this one uses, and
+    // on the owner the number of simulations using its fields
+    s3d_ctx *model = nullptr;
+    int sharers = 0;
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev[3];   // start/stop pairs per kernel class
@@ -223,6 +228,28 @@ void dfree(void *p)
 {
     if (p) (void)hipFree(p);
 }
+
+size_t rsize(const s3d_ctx *c);
+
+// Per-simulation state: velocity (+ compensation), stress (+ compensation),
+// memory variables, element force and the device step counter, zeroed.
+int alloc_state(s3d_ctx *c)
+{
+    const size_t rs = rsize(c);
+    const size_t ls = c->o.kahan_sigma == 2 ? 2 : rs;   // BF16 compensation in pairs
+    for (int q = 0; q < 3; ++q) {
+        if (dalloc(c, &c->v[q], rs * c->nuniq)) return 1;
+        if (c->fp32 && c->o.kahan_v && dalloc(c, &c->lo[q], rs * c->nuniq)) return 1;
+    }
+    if (dalloc(c, &c->sig, rs * 6 * c->nn)) return 1;
+    if (c->o.kahan_sigma && dalloc(c, &c->sig_lo, ls * 6 * c->nn)) return 1;
+    if (c->visco && dalloc(c, &c->mem, rs * 18 * c->nn)) return 1;
+    if (dalloc(c, &c->force, rs * 3 * c->nn, false)) return 1;
+    if (dalloc(c, (void **)&c->d_it, sizeof(int))) return 1;
+    return 0;
+}
+
+s3d_ctx *model_of(s3d_ctx *c) { return c->model ? c->model : c; }
 
 // 1-D colatitude and radius tables of the element kernel, computed in FP64
 // and stored in the kernel's coefficient precision C.
@@ -764,16 +791,30 @@ int batch_build_group_e(BatchGroup &G)
     G.ewg = es[n];
     G.nwg = ns[n];
     G.rwg = rs[n];
+    // Simulations that share one model (s3d_create_shared) run
+    // source-innermost: the workgroups of one element tile (or node range)
+    // for every member follow each other, so the shared moduli, tau and dt/M
+    // come from HBM once per tile and from cache for the other members.
+    bool interleave = n > 1;
+    for (int q = 1; q < n && interleave; ++q) interleave = model_of(G.sims[q]) == model_of(G.sims[0]);
     // the element launch is padded to a multiple of 8 workgroups (idle ones
     // own simulation -1) so its XCD-aware block order applies
-    auto owners = [n](const std::vector<int> &st, int pad8) {
+    auto owners = [n](const std::vector<int> &st, int pad8, bool inter) {
         const int total = pad8 ? (st[n] + 7) / 8 * 8 : st[n];
         std::vector<int2> o(total, make_int2(-1, 0));
-        for (int q = 0; q < n; ++q)
-            for (int w = st[q]; w < st[q + 1]; ++w) o[w] = make_int2(q, w - st[q]);
+        if (inter) {
+            const int per = st[1] - st[0];   // members share the mesh and tile
+            for (int w = 0; w < per; ++w)
+                for (int q = 0; q < n; ++q) o[(size_t)w * n + q] = make_int2(q, w);
+        } else {
+            for (int q = 0; q < n; ++q)
+                for (int w = st[q]; w < st[q + 1]; ++w) o[w] = make_int2(q, w - st[q]);
+        }
         return o;
     };
-    const std::vector<int2> eo = owners(es, 1), no = owners(ns, 0), ro = owners(rs, 0);
+    // receivers differ between members: their launch stays member-major
+    const std::vector<int2> eo = owners(es, 1, interleave), no = owners(ns, 0, interleave),
+                            ro = owners(rs, 0, false);
     G.ewg = (int)eo.size();
     HIPCHK(hipMalloc(&G.d_eargs, sizeof(ElemArgs<R>) * n));
     HIPCHK(hipMalloc(&G.d_nargs, sizeof(NodeArgs<R>) * n));
@@ -1023,8 +1064,20 @@ int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out)
 int s3d_destroy(s3d_ctx *c)
 {
     if (!c) return 0;
+    if (c->sharers > 0)
+        return fail(std::to_string(c->sharers) + " simulation(s) created with s3d_create_shared still use "
+                    "this model; destroy them first");
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->model) {
+        // the static fields belong to the model's simulation
+        void **shared[] = {&c->dt_inv_mm, (void **)&c->mm64, &c->lam, &c->mu, &c->tau, &c->rho, &c->ca,
+                           &c->cb, &c->cc, &c->tth, &c->tr, &c->wdev, (void **)&c->dpx,
+                           (void **)&c->dpy, (void **)&c->dpz};
+        for (void **q : shared) *q = nullptr;
+        c->model->sharers--;
+        c->model = nullptr;
+    }
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
     void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->sig,
                     c->sig_lo, c->mem, c->lam, c->mu, c->tau, c->rho, c->ca, c->cb, c->cc,
@@ -1038,6 +1091,63 @@ int s3d_destroy(s3d_ctx *c)
     dfree(c->d_it);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return 0;
+}
+
+int s3d_create_shared(s3d_ctx *base, s3d_ctx **out)
+{
+    if (!base || !out) return fail("null argument");
+    if (!base->inited) return fail("the model's simulation must be initialised (s3d_init) first");
+    s3d_ctx *m = model_of(base);
+    HIPCHK(hipSetDevice(m->dev));
+    s3d_ctx *c = new s3d_ctx();
+    // mesh, options, host geometry and element tile of the model's simulation
+    c->g = m->g;
+    c->o = m->o;
+    c->p = m->p; c->P1 = m->P1; c->NP = m->NP; c->PB = m->PB;
+    c->nel = m->nel; c->nn = m->nn; c->nblk = m->nblk; c->nuniq = m->nuniq;
+    c->fp32 = m->fp32; c->visco = m->visco; c->aniso = m->aniso;
+    c->q_finite = m->q_finite; c->abc_nonzero = m->abc_nonzero;
+    c->dev = m->dev;
+    std::copy(m->knots, m->knots + MAXP1, c->knots);
+    std::copy(m->weights, m->weights + MAXP1, c->weights);
+    std::copy(m->dl, m->dl + MAXP1 * MAXP1, c->dl);
+    c->x_glb_min = m->x_glb_min; c->x_glb_max = m->x_glb_max;
+    c->y_glb_min = m->y_glb_min; c->y_glb_max = m->y_glb_max;
+    c->z_glb_min = m->z_glb_min; c->z_glb_max = m->z_glb_max;
+    c->dx = m->dx; c->dy = m->dy; c->dz = m->dz; c->jac = m->jac;
+    c->x_loc_min = m->x_loc_min; c->x_loc_max = m->x_loc_max;
+    c->y_loc_min = m->y_loc_min; c->y_loc_max = m->y_loc_max;
+    c->z_loc_min = m->z_loc_min; c->z_loc_max = m->z_loc_max;
+    c->x1 = m->x1; c->y1 = m->y1; c->z1 = m->z1;
+    c->px1 = m->px1; c->py1 = m->py1; c->pz1 = m->pz1;
+    c->courant = m->courant;
+    c->epb = m->epb;
+    // static fields: the model's, not copied
+    c->dt_inv_mm = m->dt_inv_mm; c->mm64 = m->mm64; c->lam = m->lam; c->mu = m->mu; c->tau = m->tau;
+    c->rho = m->rho; c->ca = m->ca; c->cb = m->cb; c->cc = m->cc; c->tth = m->tth; c->tr = m->tr;
+    c->wdev = m->wdev; c->dpx = m->dpx; c->dpy = m->dpy; c->dpz = m->dpz;
+    c->model = m;
+    m->sharers++;
+    hipError_t e = hipStreamCreate(&c->stream);
+    if (e != hipSuccess) {
+        s3d_destroy(c);
+        return fail(std::string("stream: ") + hipGetErrorString(e));
+    }
+    if (alloc_state(c)) {
+        const std::string msg = s3d_last_error();
+        s3d_destroy(c);
+        return fail(msg);
+    }
+    e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        s3d_destroy(c);
+        return fail(std::string("create_shared: ") + hipGetErrorString(e));
+    }
+    c->slot_of.assign(c->nel, -1);
+    c->inited = true;
+    c->it = 0;
+    *out = c;
     return 0;
 }
 
@@ -1174,21 +1284,13 @@ int s3d_init(s3d_ctx *c)
     HIPCHK(hipMemcpy(d_th, th1.data(), sizeof(double) * th1.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(d_ph, ph1.data(), sizeof(double) * ph1.size(), hipMemcpyHostToDevice));
 
-    // ---- state ----
+    // ---- state and static fields ----
     c->visco = c->q_finite;
     c->aniso = c->abc_nonzero;
-    for (int q = 0; q < 3; ++q) {
-        if (dalloc(c, &c->v[q], rs * c->nuniq)) return 1;
-        if (c->fp32 && c->o.kahan_v && dalloc(c, &c->lo[q], rs * c->nuniq)) return 1;
-    }
+    if (alloc_state(c)) return 1;
     const size_t ss = c->o.bf16_static ? 2 : rs;   // static fields
-    const size_t ls = c->o.kahan_sigma == 2 ? 2 : rs; // stress compensation (BF16 in pairs)
     if (dalloc(c, &c->dt_inv_mm, ss * c->nuniq)) return 1;
     if (dalloc(c, (void **)&c->mm64, sizeof(double) * c->nuniq)) return 1;
-    if (dalloc(c, &c->sig, rs * 6 * c->nn)) return 1;
-    if (c->o.kahan_sigma && dalloc(c, &c->sig_lo, ls * 6 * c->nn)) return 1;
-    if (c->visco && dalloc(c, &c->mem, rs * 18 * c->nn)) return 1;
-    if (dalloc(c, &c->force, rs * 3 * c->nn, false)) return 1;
     if (dalloc(c, &c->lam, ss * c->nn, false)) return 1;
     if (dalloc(c, &c->mu, ss * c->nn, false)) return 1;
     if (c->visco && dalloc(c, &c->tau, ss * c->nn, false)) return 1;
@@ -1227,7 +1329,6 @@ int s3d_init(s3d_ctx *c)
     HIPCHK(hipStreamSynchronize(c->stream));
     (void)hipFree(d_sin); (void)hipFree(d_r); (void)hipFree(d_w); (void)hipFree(d_th); (void)hipFree(d_ph);
     for (int q = 0; q < 7; ++q) { dfree(c->stage[q]); c->stage[q] = nullptr; }
-    if (dalloc(c, (void **)&c->d_it, sizeof(int))) return 1;
     HIPCHK(hipStreamSynchronize(c->stream));
     c->slot_of.assign(c->nel, -1);
     c->inited = true;

@@ -501,7 +501,7 @@ elem_kernel(const ElemArgs<R> a)
 
 template <int P, int E, typename R, bool VISCO>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
-elem_kernel_batch(const ElemArgs<R> *args, const int2 *own)
+elem_kernel_batch(const ElemArgs<R> *__restrict__ args, const int2 *__restrict__ own)
 {
     // (simulation, its workgroup index), in XCD-aware order over the whole
     // batch: each XCD works through a contiguous range of simulations
@@ -605,7 +605,7 @@ node_kernel(const NodeArgs<R> a)
 
 template <int P, typename R, bool KV>
 __global__ void __launch_bounds__(256)
-node_kernel_batch(const NodeArgs<R> *args, const int2 *own)
+node_kernel_batch(const NodeArgs<R> *__restrict__ args, const int2 *__restrict__ own)
 {
     const int2 o = own[blockIdx.x];
     node_body<P, R, KV>(args[o.x], (long)o.y * blockDim.x + threadIdx.x);
@@ -678,7 +678,7 @@ __global__ void __launch_bounds__(64) record_kernel(const RecArgs<R> a)
 }
 
 template <int P, typename R>
-__global__ void __launch_bounds__(64) record_kernel_batch(const RecArgs<R> *args, const int2 *own)
+__global__ void __launch_bounds__(64) record_kernel_batch(const RecArgs<R> *__restrict__ args, const int2 *__restrict__ own)
 {
     const int2 o = own[blockIdx.x];
     record_body<P, R>(args[o.x], o.y);

@@ -31,7 +31,7 @@ CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6}
 WAVELET = {"RICKER": 0, "GAUSS": 1, "SIN**3": 2, "BRUNE": 3, "DELTA": 4}
 
 EXPORTS = [
-    "s3d_last_error", "s3d_device_count", "s3d_create", "s3d_destroy",
+    "s3d_last_error", "s3d_device_count", "s3d_create", "s3d_destroy", "s3d_create_shared",
     "s3d_set_model_slab", "s3d_set_model", "s3d_init", "s3d_cfl", "s3d_gll",
     "s3d_autotune", "s3d_locate", "s3d_wavelet", "s3d_add_source_mt", "s3d_add_source_sf",
     "s3d_add_receiver", "s3d_step", "s3d_current_step", "s3d_get_trace",
@@ -77,6 +77,7 @@ def lib():
         L.s3d_device_count.argtypes = [C.POINTER(C.c_int)]
         L.s3d_create.argtypes = [C.POINTER(S3dGrid), C.POINTER(S3dOptions), C.POINTER(vp)]
         L.s3d_destroy.argtypes = [vp]
+        L.s3d_create_shared.argtypes = [vp, C.POINTER(vp)]
         L.s3d_set_model_slab.argtypes = [vp, C.c_int, C.c_int, C.c_int, dp]
         L.s3d_set_model.argtypes = [vp, C.c_int, dp]
         L.s3d_init.argtypes = [vp]
@@ -154,9 +155,25 @@ class Simulation:
         self.precision = precision
         self.shape = (grid.lpd + 1,) * 3 + (grid.nz, grid.ny, grid.nx)
 
+    @classmethod
+    def shared(cls, model):
+        """A further simulation on `model`'s mesh and Earth model
+        (s3d_create_shared): own state, sources and receivers; the static
+        fields are `model`'s. Ready to step."""
+        self = cls.__new__(cls)
+        h = C.c_void_p()
+        _chk(lib().s3d_create_shared(model._h, C.byref(h)))
+        self._h = h
+        self._model = model          # the model's simulation must outlive this one
+        self.grid = model.grid
+        self.p = model.p
+        self.precision = model.precision
+        self.shape = model.shape
+        return self
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
-            lib().s3d_destroy(self._h)
+            _chk(lib().s3d_destroy(self._h))
             self._h = C.c_void_p()
 
     def __del__(self):
@@ -323,7 +340,14 @@ def build_case(case, precision="fp32", kahan_v=True, kahan_sigma=False,
     sim.init()
     if autotune:
         sim.autotune()
-    for s in case.sources:
+    return sim, _add_channels(sim, case, case.sources)
+
+
+def _add_channels(sim, case, sources):
+    """Locate and add `sources` and the case's receivers; returns the
+    receivers that were placed."""
+    g = case.grid
+    for s in sources:
         loc = sim.locate(s.lat, s.lon, s.depth)
         if loc is None:
             continue
@@ -336,4 +360,20 @@ def build_case(case, precision="fp32", kahan_v=True, kahan_sigma=False,
             continue
         sim.add_receiver(loc[0], loc[1], r.channels)
         placed.append(r)
-    return sim, placed
+    return placed
+
+
+def build_shared_sources(case, precision="fp32", kahan_v=True, kahan_sigma=False,
+                         slab=None, autotune=False):
+    """One simulation per source of `case`, all on one copy of the case's
+    model (the first owns it, the others are Simulation.shared): the
+    many-sources-one-model workload. Returns (simulations, placed receivers)."""
+    from dataclasses import replace
+    first, placed = build_case(replace(case, sources=case.sources[:1]), precision, kahan_v,
+                               kahan_sigma, slab=slab, autotune=autotune)
+    sims = [first]
+    for s in case.sources[1:]:
+        sim = Simulation.shared(first)
+        _add_channels(sim, case, [s])
+        sims.append(sim)
+    return sims, placed

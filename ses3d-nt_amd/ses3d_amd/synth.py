@@ -375,6 +375,13 @@ def scaled(case, **kw):
     return c
 
 
+def multi_source(seed=5, n_src=32, nt=4000, nx=128, ny=128, nz=48, lpd=4, n_rec=256):
+    """Many seeded moment-tensor sources on one layered visco-elastic
+    model (40 x 40 deg x 1400 km): each source is its own simulation sharing
+    the model."""
+    return layered(seed=seed, nt=nt, nx=nx, ny=ny, nz=nz, lpd=lpd, n_rec=n_rec, n_src=n_src)
+
+
 def uneven_batch(seed=4, n_sims=64, nt=2000, n_rec=32, min_e=(8, 8, 8), max_e=(48, 48, 24),
                  orders=(4, 6)):
     """Independent regional simulations of uneven size: element counts drawn

@@ -224,3 +224,42 @@ def test_bf16_static_restrictions():
     b, _ = api.build_case(case, precision="fp32")
     with pytest.raises(api.S3dError, match="BF16"):
         api.Batch([a, b])
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_shared_model_sources_match_independent_runs(precision):
+    """Sources on one shared copy of the model (s3d_create_shared), batched
+    source-innermost, give bit for bit the traces of independent simulations
+    that each own the model; a sharer stepped on its own does too; the
+    model's simulation cannot be destroyed while sharers use it."""
+    from dataclasses import replace
+    case = synth.layered(seed=7, nt=40, nx=6, ny=5, nz=4, lpd=4, n_rec=4, n_src=3)
+    solo = [api.build_case(replace(case, sources=[s]), precision=precision)
+            for s in case.sources]
+    for s, _ in solo:
+        s.step(35)
+    sims, recs = api.build_shared_sources(case, precision=precision)
+    assert len(sims) == 3 and len(recs) == len(solo[0][1])
+    b = api.Batch(sims)
+    b.step(3)
+    b.step(16)
+    b.step(16)
+    b.synchronize()
+    for (s1, _), s2 in zip(solo, sims):
+        assert s2.it == 35
+        for r in range(len(recs)):
+            for ch in range(3):
+                assert np.array_equal(s2.trace(r, ch), s1.trace(r, ch)), (r, ch)
+        assert np.array_equal(s2.get(api.VZ), s1.get(api.VZ))
+    b.close()
+    lone = api.Simulation.shared(sims[0])
+    api._add_channels(lone, case, case.sources[1:2])
+    lone.step(35)
+    for r in range(len(recs)):
+        assert np.array_equal(lone.trace(r, 2), solo[1][0].trace(r, 2))
+    with pytest.raises(api.S3dError, match="still use"):
+        sims[0].close()
+    lone.close()
+    for s in sims[1:]:
+        s.close()
+    sims[0].close()
