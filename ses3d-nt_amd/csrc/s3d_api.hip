@@ -287,34 +287,34 @@ ElemArgs<R> elem_args(s3d_ctx *c)
     const int P1 = P + 1;
     a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz; a.nel = (int)c->nel;
     a.remap = c->o.xcd_remap;
-    a.it_dev = c->d_it;
+    a.it_dev = (decltype(a.it_dev))(c->d_it);
     a.nt = c->g.nt;
     a.kahan_sigma = c->o.kahan_sigma;
-    a.vx = (const R *)c->v[0]; a.vy = (const R *)c->v[1]; a.vz = (const R *)c->v[2];
-    a.sig = (R *)c->sig;
+    a.vx = (decltype(a.vx))((const R *)c->v[0]); a.vy = (decltype(a.vy))((const R *)c->v[1]); a.vz = (decltype(a.vz))((const R *)c->v[2]);
+    a.sig = (decltype(a.sig))((R *)c->sig);
     if (c->o.kahan_sigma == 2)
-        a.sig_lo2 = (unsigned *)c->sig_lo;
+        a.sig_lo2 = (decltype(a.sig_lo2))((unsigned *)c->sig_lo);
     else
-        a.sig_lo = (R *)c->sig_lo;
-    a.mem = (R *)c->mem;
+        a.sig_lo = (decltype(a.sig_lo))((R *)c->sig_lo);
+    a.mem = (decltype(a.mem))((R *)c->mem);
     if (c->o.bf16_static) {
-        a.s16.lam = (const bf16 *)c->lam; a.s16.mu = (const bf16 *)c->mu; a.s16.tau = (const bf16 *)c->tau;
-        a.s16.ca = (const bf16 *)c->ca; a.s16.cb = (const bf16 *)c->cb; a.s16.cc = (const bf16 *)c->cc;
+        a.s16.lam = (decltype(a.s16.lam))((const bf16 *)c->lam); a.s16.mu = (decltype(a.s16.mu))((const bf16 *)c->mu); a.s16.tau = (decltype(a.s16.tau))((const bf16 *)c->tau);
+        a.s16.ca = (decltype(a.s16.ca))((const bf16 *)c->ca); a.s16.cb = (decltype(a.s16.cb))((const bf16 *)c->cb); a.s16.cc = (decltype(a.s16.cc))((const bf16 *)c->cc);
     } else {
-        a.lam = (const R *)c->lam; a.mu = (const R *)c->mu; a.tau = (const R *)c->tau;
-        a.ca = (const R *)c->ca; a.cb = (const R *)c->cb; a.cc = (const R *)c->cc;
+        a.lam = (decltype(a.lam))((const R *)c->lam); a.mu = (decltype(a.mu))((const R *)c->mu); a.tau = (decltype(a.tau))((const R *)c->tau);
+        a.ca = (decltype(a.ca))((const R *)c->ca); a.cb = (decltype(a.cb))((const R *)c->cb); a.cc = (decltype(a.cc))((const R *)c->cc);
     }
-    a.force = (R *)c->force;
+    a.force = (decltype(a.force))((R *)c->force);
     using C = coef_t<R>;
-    a.tth = (const ThTab<C> *)c->tth;
-    a.tr = (const RTab<C> *)c->tr;
-    a.px = c->dpx; a.py = c->dpy; a.pz = c->dpz;
+    a.tth = (decltype(a.tth))((const ThTab<C> *)c->tth);
+    a.tr = (decltype(a.tr))((const RTab<C> *)c->tr);
+    a.px = (decltype(a.px))(c->dpx); a.py = (decltype(a.py))(c->dpy); a.pz = (decltype(a.pz))(c->dpz);
     a.alpha = c->g.taper_slope;
     a.taper_on = c->g.taper_width > 0;
-    a.src_slot = c->d_slot;
-    a.src_chan = c->d_chan;
-    a.src_coef = c->d_coef;
-    a.stf = c->d_stf;
+    a.src_slot = (decltype(a.src_slot))(c->d_slot);
+    a.src_chan = (decltype(a.src_chan))(c->d_chan);
+    a.src_coef = (decltype(a.src_coef))(c->d_coef);
+    a.stf = (decltype(a.stf))(c->d_stf);
     a.d1 = (C)((2.0 / c->dx) * c->jac);
     a.d2 = (C)((2.0 / c->dy) * c->jac);
     a.d3 = (C)(-((2.0 / c->dz) * c->jac));
@@ -338,14 +338,14 @@ NodeArgs<R> node_args(s3d_ctx *c)
     std::memset(&a, 0, sizeof(a));
     a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz; a.nel = (int)c->nel;
     a.nblk = c->nblk;
-    a.vx = (R *)c->v[0]; a.vy = (R *)c->v[1]; a.vz = (R *)c->v[2];
-    a.lx = (R *)c->lo[0]; a.ly = (R *)c->lo[1]; a.lz = (R *)c->lo[2];
-    a.force = (const R *)c->force;
+    a.vx = (decltype(a.vx))((R *)c->v[0]); a.vy = (decltype(a.vy))((R *)c->v[1]); a.vz = (decltype(a.vz))((R *)c->v[2]);
+    a.lx = (decltype(a.lx))((R *)c->lo[0]); a.ly = (decltype(a.ly))((R *)c->lo[1]); a.lz = (decltype(a.lz))((R *)c->lo[2]);
+    a.force = (decltype(a.force))((const R *)c->force);
     if (c->o.bf16_static)
-        a.dt_inv_mm16 = (const bf16 *)c->dt_inv_mm;
+        a.dt_inv_mm16 = (decltype(a.dt_inv_mm16))((const bf16 *)c->dt_inv_mm);
     else
-        a.dt_inv_mm = (const R *)c->dt_inv_mm;
-    a.px = c->dpx; a.py = c->dpy; a.pz = c->dpz;
+        a.dt_inv_mm = (decltype(a.dt_inv_mm))((const R *)c->dt_inv_mm);
+    a.px = (decltype(a.px))(c->dpx); a.py = (decltype(a.py))(c->dpy); a.pz = (decltype(a.pz))(c->dpz);
     a.alpha = c->g.taper_slope;
     a.taper_on = c->g.taper_width > 0;
     return a;
@@ -356,13 +356,13 @@ RecArgs<R> rec_args(s3d_ctx *c)
 {
     RecArgs<R> a;
     a.nx = c->g.nx; a.ny = c->g.ny; a.nel = (int)c->nel; a.nt = c->g.nt;
-    a.vx = (const R *)c->v[0]; a.vy = (const R *)c->v[1]; a.vz = (const R *)c->v[2];
-    a.lx = (const R *)c->lo[0]; a.ly = (const R *)c->lo[1]; a.lz = (const R *)c->lo[2];
-    a.sig = (const R *)c->sig;
-    a.rec_e = c->d_rec_e; a.rec_nch = c->d_rec_nch; a.rec_ch = c->d_rec_ch;
-    a.rec_vm = c->d_rec_vm;
-    a.traces = c->d_traces;
-    a.it_dev = c->d_it;
+    a.vx = (decltype(a.vx))((const R *)c->v[0]); a.vy = (decltype(a.vy))((const R *)c->v[1]); a.vz = (decltype(a.vz))((const R *)c->v[2]);
+    a.lx = (decltype(a.lx))((const R *)c->lo[0]); a.ly = (decltype(a.ly))((const R *)c->lo[1]); a.lz = (decltype(a.lz))((const R *)c->lo[2]);
+    a.sig = (decltype(a.sig))((const R *)c->sig);
+    a.rec_e = (decltype(a.rec_e))(c->d_rec_e); a.rec_nch = (decltype(a.rec_nch))(c->d_rec_nch); a.rec_ch = (decltype(a.rec_ch))(c->d_rec_ch);
+    a.rec_vm = (decltype(a.rec_vm))(c->d_rec_vm);
+    a.traces = (decltype(a.traces))(c->d_traces);
+    a.it_dev = (decltype(a.it_dev))(c->d_it);
     return a;
 }
 

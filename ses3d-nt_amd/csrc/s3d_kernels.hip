@@ -617,11 +617,11 @@ node_kernel_batch(const NodeArgs<R> *__restrict__ args, const int2 *__restrict__
 template <typename R>
 struct RecArgs {
     int nx, ny, nel, nt;
-    const R *vx, *vy, *vz, *lx, *ly, *lz, *sig;
-    const int *rec_e, *rec_nch, *rec_ch;
-    const double *rec_vm;
-    double *traces;
-    const int *it_dev;
+    gptr<const R> vx, vy, vz, lx, ly, lz, sig;
+    gptr<const int> rec_e, rec_nch, rec_ch;
+    gptr<const double> rec_vm;
+    gptr<double> traces;
+    gptr<const int> it_dev;
 };
 
 template <int P, typename R>
