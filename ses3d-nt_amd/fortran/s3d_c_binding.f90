@@ -174,6 +174,32 @@ MODULE ses3d_hip_c
             IMPORT :: c_int, c_ptr
             TYPE(c_ptr), VALUE :: ctx
         END FUNCTION
+        ! further simulations on one model, and batches (no counterpart in
+        ! the reference; see include/ses3d_hip.h)
+        INTEGER(c_int) FUNCTION s3d_create_shared(model, out) BIND(C, NAME='s3d_create_shared')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: model
+            TYPE(c_ptr), INTENT(OUT) :: out
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_batch_create(n, sims, out) BIND(C, NAME='s3d_batch_create')
+            IMPORT :: c_int, c_ptr
+            INTEGER(c_int), VALUE :: n
+            TYPE(c_ptr), INTENT(IN) :: sims(*)
+            TYPE(c_ptr), INTENT(OUT) :: out
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_batch_step(batch, nsteps) BIND(C, NAME='s3d_batch_step')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: batch
+            INTEGER(c_int), VALUE :: nsteps
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_batch_synchronize(batch) BIND(C, NAME='s3d_batch_synchronize')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: batch
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_batch_destroy(batch) BIND(C, NAME='s3d_batch_destroy')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: batch
+        END FUNCTION
         FUNCTION c_strlen(s) BIND(C, NAME='strlen')
             IMPORT :: c_ptr, c_size_t
             TYPE(c_ptr), VALUE :: s
