@@ -380,27 +380,38 @@ void ev_end(s3d_ctx *c, int k) { ev_begin(c, k); }
 // Element-tile variants offered to the autotuner: FP32 kernels exist for 1,
 // 2 and 4 elements per workgroup (where that fits 1024 lanes) besides the
 // default; FP64 uses the default tile only.
+// the stress-compensation mode is a kernel template parameter (FP32 only)
+template <int P, int E, typename R, bool VISCO, bool S16>
+int launch_elem_ks(int ks, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
+{
+    using S = Shape<P, E>;
+    if constexpr (std::is_same<R, float>::value) {
+        if (ks == 1)
+            hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 1>), dim3(ge), dim3(S::NT), 0, st, ea);
+        else if (ks == 2)
+            hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 2>), dim3(ge), dim3(S::NT), 0, st, ea);
+        else
+            hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0>), dim3(ge), dim3(S::NT), 0, st, ea);
+    } else {
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0>), dim3(ge), dim3(S::NT), 0, st, ea);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 template <int P, int E, typename R>
 int launch_elem_e(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st)
 {
     using S = Shape<P, E>;
     const unsigned ge = nblocks(c->nel, S::EPB);
+    const int ks = c->o.kahan_sigma;
     if constexpr (std::is_same<R, float>::value && E == 0) {
-        if (c->o.bf16_static) {   // default tile only
-            if (c->visco)
-                hipLaunchKernelGGL((elem_kernel<P, 0, float, true, true>), dim3(ge), dim3(S::NT), 0, st, ea);
-            else
-                hipLaunchKernelGGL((elem_kernel<P, 0, float, false, true>), dim3(ge), dim3(S::NT), 0, st, ea);
-            HIPCHK(hipGetLastError());
-            return 0;
-        }
+        if (c->o.bf16_static)   // default tile only
+            return c->visco ? launch_elem_ks<P, 0, float, true, true>(ks, ea, st, ge)
+                            : launch_elem_ks<P, 0, float, false, true>(ks, ea, st, ge);
     }
-    if (c->visco)
-        hipLaunchKernelGGL((elem_kernel<P, E, R, true>), dim3(ge), dim3(S::NT), 0, st, ea);
-    else
-        hipLaunchKernelGGL((elem_kernel<P, E, R, false>), dim3(ge), dim3(S::NT), 0, st, ea);
-    HIPCHK(hipGetLastError());
-    return 0;
+    return c->visco ? launch_elem_ks<P, E, R, true, false>(ks, ea, st, ge)
+                    : launch_elem_ks<P, E, R, false, false>(ks, ea, st, ge);
 }
 
 template <int P>
@@ -746,6 +757,7 @@ int upload_receivers(s3d_ctx *c)
 struct BatchGroup {
     int p = 0;
     int epb = 0;
+    int ks = 0;                 // stress compensation mode
     bool fp32 = false, visco = false, kahan = false;
     std::vector<s3d_ctx *> sims;
     void *d_eargs = nullptr, *d_nargs = nullptr, *d_rargs = nullptr;
@@ -831,18 +843,30 @@ int batch_build_group_e(BatchGroup &G)
     return 0;
 }
 
+template <int P, int E, typename R, bool VISCO, int KS>
+int batch_launch_group_k(const BatchGroup &G, hipStream_t st)
+{
+    using S = Shape<P, E>;
+    hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS>), dim3(G.ewg), dim3(S::NT), 0, st,
+                       (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <int P, int E, typename R, bool VISCO>
+int batch_launch_group_v(const BatchGroup &G, hipStream_t st)
+{
+    if constexpr (std::is_same<R, float>::value) {
+        if (G.ks == 1) return batch_launch_group_k<P, E, R, VISCO, 1>(G, st);
+        if (G.ks == 2) return batch_launch_group_k<P, E, R, VISCO, 2>(G, st);
+    }
+    return batch_launch_group_k<P, E, R, VISCO, 0>(G, st);
+}
+
 template <int P, int E, typename R>
 int batch_launch_group_e(const BatchGroup &G, hipStream_t st)
 {
-    using S = Shape<P, E>;
-    if (G.visco)
-        hipLaunchKernelGGL((elem_kernel_batch<P, E, R, true>), dim3(G.ewg), dim3(S::NT), 0, st,
-                           (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
-    else
-        hipLaunchKernelGGL((elem_kernel_batch<P, E, R, false>), dim3(G.ewg), dim3(S::NT), 0, st,
-                           (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
-    HIPCHK(hipGetLastError());
-    return 0;
+    return G.visco ? batch_launch_group_v<P, E, R, true>(G, st) : batch_launch_group_v<P, E, R, false>(G, st);
 }
 
 template <int P, typename R>
@@ -948,7 +972,7 @@ int batch_build(s3d_batch *b)
         bool placed = false;
         for (auto &G : b->groups)
             if (G.p == c->p && G.fp32 == c->fp32 && G.visco == c->visco &&
-                G.kahan == (c->lo[0] != nullptr) && G.epb == c->epb) {
+                G.kahan == (c->lo[0] != nullptr) && G.epb == c->epb && G.ks == c->o.kahan_sigma) {
                 G.sims.push_back(c);
                 placed = true;
                 break;
@@ -960,6 +984,7 @@ int batch_build(s3d_batch *b)
             G.visco = c->visco;
             G.kahan = c->lo[0] != nullptr;
             G.epb = c->epb;
+            G.ks = c->o.kahan_sigma;
             G.sims.push_back(c);
             b->groups.push_back(G);
         }

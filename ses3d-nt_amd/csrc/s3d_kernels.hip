@@ -107,8 +107,11 @@ struct TileState {
 
 // Issue the loads of tile `tile` into `s` (no waits). Lanes without an
 // element (the tail of the last tile) read element 0's addresses, so every
-// load is unconditional and in bounds; such lanes store nothing.
-template <int P, int E, typename R, bool VISCO, bool S16, bool LOADM = true>
+// load is unconditional and in bounds; such lanes store nothing. KS: stress
+// compensation 0 off, 1 FP32 words, 2 BF16 words in pairs (a template
+// parameter: a run-time choice leaves register joins that wait for every
+// load in flight).
+template <int P, int E, typename R, bool VISCO, bool S16, bool LOADM, int KS>
 __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileState<R, VISCO> &s)
 {
     using S = Shape<P, E>;
@@ -147,10 +150,10 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     for (int c = 0; c < 6; ++c) s.lo[c] = R(0);
 #pragma unroll
     for (int q = 0; q < 3; ++q) s.lo2[q] = 0u;
-    if (a.kahan_sigma == 2) {
+    if constexpr (KS == 2) {
 #pragma unroll
         for (int q = 0; q < 3; ++q) s.lo2[q] = a.sig_lo2[q * fs + eb];
-    } else if (a.kahan_sigma == 1) {
+    } else if constexpr (KS == 1) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) s.lo[c] = a.sig_lo[c * fs + eb];
     }
@@ -162,8 +165,8 @@ enum { G_SIN = 0, G_ISIN, G_COT, G_R, G_IR, G_R2, G_PX, G_PY, G_PZ, G_N };
 
 // Element kernel body: the workgroup advances tile `blk` (EPB elements, one
 // lane per element-local node). S16: the static per-node fields are stored
-// in BF16 (ElemArgs::s16).
-template <int P, int E, typename R, bool VISCO, bool S16>
+// in BF16 (ElemArgs::s16). KS: stress compensation (see load_tile).
+template <int P, int E, typename R, bool VISCO, bool S16, int KS>
 __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
     using S = Shape<P, E>;
@@ -201,51 +204,53 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             lds_dma_dword(mem + c * fs, base + (unsigned)(c * S::NT * sizeof(R)));
     }
 
-    // ---- state of the first tile: issued first, consumed after the tables ----
-    TileState<R, VISCO> cur;
-    load_tile<P, E, R, VISCO, S16, !MDMA>(a, tile0, cur);
-
-    // ---- tables -> LDS (every filling lane issues its loads, then waits once) ----
+    // ---- tables of the tile: every lane loads from clamped (in-bounds)
+    // indices, the lanes owning an entry store it after the state loads are
+    // issued, so table and state loads share one round trip ----
     static_assert(P1 * P1 <= S::NT, "coefficient fill needs (p+1)^2 lanes");
+    static_assert(NE * P1 <= S::NT, "one geometry entry per lane");
+    const int tq = t < P1 * P1 ? t : 0;
+    const C tD0 = a.Dx[tq], tD1 = a.Dy[tq], tD2 = a.Dz[tq], tD3 = a.dlw[tq];
+    const C tW = a.w[t < P1 ? t : 0];
+    const int xg = t < NE * P1 ? t : 0;
+    const int ges = xg / P1, gq = xg - ges * P1;
+    int ge = tile0 * EPB + ges;
+    ge = ge < a.nel ? ge : a.nel - 1;
+    const int gex = ge % a.nx, gey = (ge / a.nx) % a.ny, gez = ge / (a.nx * a.ny);
+    const ThTab<C> th = a.tth[gex * P1 + gq];
+    const RTab<C> rr = a.tr[gez * P1 + gq];
+    // the profiles exist (zero) also without a taper
+    const double gpx = a.px[gex * P1 + gq], gpy = a.py[gey * P1 + gq], gpz = a.pz[gez * P1 + gq];
+    const int se = tile0 * EPB + (t < NE ? t : 0);
+    int sslot = -1;
+    if (a.src_slot) sslot = a.src_slot[se < a.nel ? se : a.nel - 1];
+
+    // ---- state of the tile ----
+    TileState<R, VISCO> cur;
+    load_tile<P, E, R, VISCO, S16, !MDMA, KS>(a, tile0, cur);
+
     if (t < P1 * P1) {
-        const C d0 = a.Dx[t], d1 = a.Dy[t], d2 = a.Dz[t], d3 = a.dlw[t];
-        cf[0][t] = d0;
-        cf[1][t] = d1;
-        cf[2][t] = d2;
-        cf[3][t] = d3;
+        cf[0][t] = tD0;
+        cf[1][t] = tD1;
+        cf[2][t] = tD2;
+        cf[3][t] = tD3;
     }
-    if (t < P1) cw[t] = a.w[t];
-    for (int x = t; x < NE * P1; x += S::NT) {
-        const int es = x / P1, q = x - es * P1;
-        int e = tile0 * EPB + es;
-        e = e < a.nel ? e : a.nel - 1;
-        const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
-        const ThTab<C> th = a.tth[ex * P1 + q];
-        const RTab<C> rr = a.tr[ez * P1 + q];
-        C px = C(0), py = C(0), pz = C(0);
-        if (a.taper_on) {
-            px = (C)a.px[ex * P1 + q];
-            py = (C)a.py[ey * P1 + q];
-            pz = (C)a.pz[ez * P1 + q];
-        }
-        C *g = gtab[es];
-        g[G_SIN * P1 + q] = th.sin_t;
-        g[G_ISIN * P1 + q] = th.inv_sin;
-        g[G_COT * P1 + q] = th.cot;
-        g[G_R * P1 + q] = rr.r;
-        g[G_IR * P1 + q] = rr.inv_r;
-        g[G_R2 * P1 + q] = rr.r2;
-        g[G_PX * P1 + q] = px;
-        g[G_PY * P1 + q] = py;
-        g[G_PZ * P1 + q] = pz;
+    if (t < P1) cw[t] = tW;
+    if (t < NE * P1) {
+        C *g = gtab[ges];
+        g[G_SIN * P1 + gq] = th.sin_t;
+        g[G_ISIN * P1 + gq] = th.inv_sin;
+        g[G_COT * P1 + gq] = th.cot;
+        g[G_R * P1 + gq] = rr.r;
+        g[G_IR * P1 + gq] = rr.inv_r;
+        g[G_R2 * P1 + gq] = rr.r2;
+        g[G_PX * P1 + gq] = a.taper_on ? (C)gpx : C(0);
+        g[G_PY * P1 + gq] = a.taper_on ? (C)gpy : C(0);
+        g[G_PZ * P1 + gq] = a.taper_on ? (C)gpz : C(0);
     }
-    for (int x = t; x < NE; x += S::NT) {
-        const int e = tile0 * EPB + x;
-        sst[x] = (a.src_slot && e < a.nel) ? a.src_slot[e] : -1;
-    }
+    if (t < NE) sst[t] = se < a.nel ? sslot : -1;
     // iteration it = *it_dev + 1 uses the STF sample it-1 (zero for it = 1)
     const int it = *a.it_dev + 1;
-    __syncthreads();
 
     {
         const int tile = tile0;
@@ -260,6 +265,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             lds[1][L + node] = cur.v[1];
             lds[2][L + node] = cur.v[2];
         }
+        // tables and velocity in LDS
         __syncthreads();
 
         // ---- K1: tensor-product derivatives along theta, phi, r ----
@@ -364,8 +370,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             sg[c] = cur.sg[c];
             lo[c] = cur.lo[c];
         }
-        if (a.kahan_sigma) {
-            if (a.kahan_sigma == 2) {
+        if constexpr (KS != 0) {
+            if constexpr (KS == 2) {
 #pragma unroll
                 for (int q = 0; q < 3; ++q) {
                     lo[2 * q] = (R)bf16_lo(cur.lo2[q]);
@@ -467,13 +473,13 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
                 a.force[c * fs + eb] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
 
             // ---- store tapered stress (K13) and memory variables ----
-            if (a.kahan_sigma == 1) {
+            if constexpr (KS == 1) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
                     a.sig_lo[c * fs + eb] = lo[c] - lo[c] * dmp;
                 }
-            } else if (a.kahan_sigma == 2) {
+            } else if constexpr (KS == 2) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
 #pragma unroll
@@ -492,14 +498,14 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     }
 }
 
-template <int P, int E, typename R, bool VISCO, bool S16 = false>
+template <int P, int E, typename R, bool VISCO, bool S16, int KS>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel(const ElemArgs<R> a)
 {
-    elem_body<P, E, R, VISCO, S16>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
+    elem_body<P, E, R, VISCO, S16, KS>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
 }
 
-template <int P, int E, typename R, bool VISCO>
+template <int P, int E, typename R, bool VISCO, int KS>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel_batch(const ElemArgs<R> *__restrict__ args, const int2 *__restrict__ own)
 {
@@ -507,7 +513,7 @@ elem_kernel_batch(const ElemArgs<R> *__restrict__ args, const int2 *__restrict__
     // batch: each XCD works through a contiguous range of simulations
     const int2 o = own[remap_block(blockIdx.x, gridDim.x, 1)];
     if (o.x < 0) return;   // padding workgroup
-    elem_body<P, E, R, VISCO, false>(args[o.x], o.y);
+    elem_body<P, E, R, VISCO, false, KS>(args[o.x], o.y);
 }
 
 // ---------------------------------------------------------------------------
