@@ -216,6 +216,8 @@ CONTAINS
         CALL parse_sources(lun, cf)
         CALL parse_receivers(lun, cf)
         CALL parse_outputs(lun, cf)
+        IF (count_groups(lun, 'output_netcdf') > 0) &
+            CALL abort('&output_netcdf: NetCDF output is not provided by this build (use &output_raw)')
         CLOSE(lun)
         configuration = cf
     END SUBROUTINE

@@ -186,3 +186,10 @@ def test_source_sac_matches_config_sources(tmp_path):
     r = run(write_config(case, str(d3), extra="\n&source_sac\n file_name = '%s'\n/\n" % str(d3 / "v.sac")))
     assert r.returncode != 0 and "Unknown attribute 'vx'" in r.stdout
     assert "No sources defined" in r.stderr
+
+
+def test_netcdf_output_aborts(tmp_path):
+    case = synth.homogeneous(nt=60, nx=6, ny=6, nz=5, n_rec=2)
+    cfg = write_config(case, str(tmp_path), extra="\n&output_netcdf\n timestep_start = 1\n/\n")
+    r = run(cfg)
+    assert r.returncode != 0 and "NetCDF" in r.stderr
