@@ -180,3 +180,49 @@ def test_forward_rejects_model_files_with_model_type(tmp_path):
                                                    "mu": os.path.join(str(tmp_path), "mu")})
     r = subprocess.run([EXE, cfg], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "can't be specified at the same time" in r.stderr
+
+
+KAT = json.load(open(os.path.join(HERE, "golden", "model_known_answers.json")))
+
+
+def _top_values(tmp_path, model_type, r_top):
+    """Moduli written by workflow 'model' at the top GLL point (r = rad_max
+    exactly) of a small grid."""
+    names = ("rhoinv", "lambda", "mu", "b", "c")
+    case, r = run_model_workflow(tmp_path, model_type, 4, 2, r_top, r_top - 50e3, params=names)
+    assert r.returncode == 0, r.stderr
+    out = {}
+    for n in names:
+        f = np.fromfile(os.path.join(str(tmp_path), n + ".out"), dtype=np.float32).astype(np.float64)
+        out[n] = f[0]   # k = 0, ez = 0: r = rad_max
+    rho = 1.0 / out["rhoinv"]
+    lam, mu = out["lambda"], out["mu"]
+    return {"rho": rho / 1000.0, "vsh": np.sqrt(mu / rho) / 1000.0,
+            "vph": np.sqrt((lam + 2.0 * mu) / rho) / 1000.0,
+            "vsv": np.sqrt(max(out["b"] + mu, 0.0) / rho) / 1000.0,
+            "eta": 1.0 + out["c"] / lam}
+
+
+@pytest.mark.parametrize("k", range(len(KAT["ak135_f"])))
+def test_ak135_reference_known_answers(tmp_path, k):
+    """The six radii of src/examples/ak135_f_test.f90 (surface, Lehmann, 410,
+    660, CMB, ICB), through model_type 21."""
+    kat = KAT["ak135_f"][k]
+    v = _top_values(tmp_path, 21, kat["x_km"] * 1000.0)
+    assert v["rho"] == pytest.approx(kat["rho"], rel=1e-6)
+    assert v["vph"] == pytest.approx(kat["vp"], rel=1e-6)
+    assert v["vsh"] == pytest.approx(kat["vs"], rel=1e-6, abs=1e-6)
+
+
+@pytest.mark.parametrize("k", [k for k, p in enumerate(KAT["prem"]) if p["r_m"] > 0.0])
+def test_prem_reference_known_answers(tmp_path, k):
+    """The radii of src/examples/prem_test.f90 10 m either side of the CMB,
+    the 670 and the Moho, and the surface, through model_type 11 (the
+    reference's transversely isotropic prem()); each value must lie in the
+    test's interval (5 decimals, so widened by 1e-5). The centre r = 0 is not
+    a radius a mesh can start at (rad_max > rad_min >= 1)."""
+    kat = KAT["prem"][k]
+    v = _top_values(tmp_path, 11, kat["r_m"])
+    for q in ("rho", "vph", "vsv", "vsh", "eta"):
+        lo, hi = kat[q]
+        assert lo - 1e-5 <= v[q] <= hi + 1e-5, (q, v[q], kat[q])
