@@ -370,7 +370,7 @@ def main():
                     help="first run a plain device copy of this many GB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the whole mesh on a single "
+        print("bench.py measures one GPU: the library runs the whole mesh on a single "
               "MI355X and has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)

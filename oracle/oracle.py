@@ -1,7 +1,7 @@
 """ctypes front end of the FP64 oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker of the HIP path. The product
+cpu_baseline leg of bench.py, as the reference the HIP path is compared with. The product
 never imports it.
 
 All 6-D arrays are numpy float64 arrays of shape (p+1, p+1, p+1, nz, ny, nx)

@@ -1,7 +1,7 @@
 /*
  * s3d_oracle.c -- FP64 CPU restatement of the Ses3d-NT forward time step.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never the product.
+ * TEST INFRASTRUCTURE ONLY: what the HIP path is compared with, never the product.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
  *
  * Every routine restates one piece of the reference (cited per function) in

@@ -1,7 +1,7 @@
 /*
  * s3d_oracle.h -- FP64 CPU restatement of the Ses3d-NT forward time step.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP path:
+ * TEST INFRASTRUCTURE ONLY. The HIP path is compared with this library:
  * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
  * may load it. The product (ses3d-nt_amd/) never links or calls it.
  *

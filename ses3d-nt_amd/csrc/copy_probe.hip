@@ -1,6 +1,6 @@
 // copy_probe.hip -- plain device streams of a known size, for calibrating the
 // rocprofv3 FETCH_SIZE / WRITE_SIZE counters against HBM bytes and for
-// measuring the attainable HBM bandwidth on the box.
+// measuring the attainable HBM bandwidth of the device.
 //
 //   copy_probe <GB> [repeats] [mode]
 //

@@ -322,7 +322,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             rt[XZ] = tmu * exz;
             rt[YZ] = tmu * eyz;
         } else {
-            // build-defined SLS (loop form of src/evolution.f90:216-284)
+            // SLS in the loop form of the commented-out src/evolution.f90:216-284
             if constexpr (MDMA) {
                 // this lane's words were written by its own wave's loads
                 const int z = wait_vmem_zero();

@@ -6,8 +6,7 @@
 !!
 !! Groups: &grid, &model, &time, &general (exactly one each), &source_mt,
 !! &source_sf, &source_sac, &receiver (any number, read in file order), and
-!! the optional
-!! &gpu group of this build: precision ('fp32' | 'fp64'), kahan_v,
+!! the optional &gpu group of the GPU library: precision ('fp32' | 'fp64'), kahan_v,
 !! kahan_sigma, kahan_sigma_bf16 (stress compensation words in BF16),
 !! bf16_static (moduli, tau and dt/M stored in BF16), device.
 MODULE configuration_mod
@@ -217,7 +216,7 @@ CONTAINS
         CALL parse_receivers(lun, cf)
         CALL parse_outputs(lun, cf)
         IF (count_groups(lun, 'output_netcdf') > 0) &
-            CALL abort('&output_netcdf: NetCDF output is not provided by this build (use &output_raw)')
+            CALL abort('&output_netcdf: NetCDF output is not provided (use &output_raw)')
         CLOSE(lun)
         configuration = cf
     END SUBROUTINE
