@@ -130,6 +130,11 @@ int s3d_cfl(s3d_ctx *ctx, double *courant);
  * counterpart in the reference. */
 int s3d_autotune(s3d_ctx *ctx, int *tile, double *ms);
 
+/* Element tile of the FP32 step without timing: 0 default, or 1, 2 or 4
+ * elements per workgroup (where that fits 1024 lanes). No counterpart in the
+ * reference. */
+int s3d_set_tile(s3d_ctx *ctx, int tile);
+
 /* GLL points, weights and derivative matrix dl[i + (p+1)*j] = l_i'(xi_j)
  * (reference get_knots / get_weights / get_dlgll, src/gll_mod.f90:178-275). */
 int s3d_gll(int lpd, double *knots, double *weights, double *dl);

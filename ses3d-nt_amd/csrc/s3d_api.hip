@@ -1378,6 +1378,19 @@ int s3d_autotune(s3d_ctx *c, int *tile, double *ms)
     return rc;
 }
 
+int s3d_set_tile(s3d_ctx *c, int tile)
+{
+    if (!c) return fail("null argument");
+    if (tile != 0 && tile != 1 && tile != 2 && tile != 4) return fail("tile must be 0, 1, 2 or 4");
+    if (tile != 0 && (!c->fp32 || c->o.bf16_static))
+        return fail("FP64 and BF16-static simulations use the default tile");
+    const int P1 = c->p + 1;
+    if (tile * P1 * P1 * P1 > 1024) return fail("tile does not fit 1024 lanes at this order");
+    c->epb = tile;
+    drop_graph(c);
+    return 0;
+}
+
 int s3d_cfl(s3d_ctx *c, double *courant)
 {
     if (!c || !courant) return fail("null argument");

@@ -37,7 +37,7 @@ EXPORTS = [
     "s3d_add_receiver", "s3d_step", "s3d_current_step", "s3d_get_trace",
     "s3d_get_field", "s3d_set_field", "s3d_vrange", "s3d_kinetic_energy",
     "s3d_synchronize", "s3d_timing_enable", "s3d_kernel_times",
-    "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step",
+    "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step", "s3d_set_tile",
     "s3d_batch_synchronize", "s3d_batch_destroy",
 ]
 
@@ -83,6 +83,7 @@ def lib():
         L.s3d_init.argtypes = [vp]
         L.s3d_cfl.argtypes = [vp, C.POINTER(C.c_double)]
         L.s3d_autotune.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.s3d_set_tile.argtypes = [vp, C.c_int]
         L.s3d_gll.argtypes = [C.c_int, dp, dp, dp]
         L.s3d_locate.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, dp]
         L.s3d_wavelet.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp]
@@ -205,6 +206,11 @@ class Simulation:
         t, ms = C.c_int(), C.c_double()
         _chk(lib().s3d_autotune(self._h, C.byref(t), C.byref(ms)))
         return t.value, ms.value
+
+    def set_tile(self, tile):
+        """Element tile without timing (s3d_set_tile): 0 (default), 1, 2 or 4
+        elements per workgroup."""
+        _chk(lib().s3d_set_tile(self._h, int(tile)))
 
     def cfl(self):
         x = C.c_double()

@@ -263,3 +263,33 @@ def test_shared_model_sources_match_independent_runs(precision):
     for s in sims[1:]:
         s.close()
     sims[0].close()
+
+
+@pytest.mark.parametrize("tile", [1, 2, 4])
+def test_set_tile_identical(tile):
+    """A tile chosen with s3d_set_tile computes the same per-node arithmetic:
+    traces and fields equal the default tile's bit for bit."""
+    case = synth.layered(seed=6, nt=24, nx=12, ny=10, nz=6, lpd=4, n_rec=4)
+    ref, recs = api.build_case(case, precision="fp32", kahan_sigma="bf16")
+    ref.step(24)
+    s, _ = api.build_case(case, precision="fp32", kahan_sigma="bf16")
+    s.set_tile(tile)
+    s.step(24)
+    for r in range(len(recs)):
+        for ch in range(3):
+            assert np.array_equal(s.trace(r, ch), ref.trace(r, ch)), (tile, r, ch)
+    for f in (api.VX, api.SXX, api.MXX + 17):
+        assert np.array_equal(s.get(f), ref.get(f)), (tile, f)
+
+
+def test_set_tile_checks():
+    elastic = synth.homogeneous(nt=10, nx=5, ny=5, nz=4, n_rec=1)
+    s, _ = api.build_case(elastic, precision="fp32")
+    with pytest.raises(api.S3dError, match="tile must be"):
+        s.set_tile(3)
+    with pytest.raises(api.S3dError, match="tile must be"):
+        s.set_tile(104)
+    s.set_tile(2)
+    d, _ = api.build_case(elastic, precision="fp64")
+    with pytest.raises(api.S3dError, match="default tile"):
+        d.set_tile(4)
