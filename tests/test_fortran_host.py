@@ -110,6 +110,19 @@ def test_forward_builtin_prem_matches_oracle(tmp_path):
         for ci, ch in enumerate(rec.channels):
             _, _, data = read_sac(os.path.join(str(tmp_path), "XX.00.%s.%s.sac" % (rec.station, ch)))
             assert rel_l2(data, o.trace(ri, ci)) < 1e-5, (rec.station, ch)
+    # the REAL32 files of workflow 'model' read back as model files
+    fdir = tmp_path / "files"
+    fdir.mkdir()
+    cfg = write_config(case, str(fdir), precision="fp64",
+                       model=dict((k, os.path.join(str(mdir), k)) for k in names))
+    r = run(cfg)
+    assert r.returncode == 0, r.stderr[-600:]
+    for rec in recs:
+        for ch in rec.channels:
+            name = "XX.00.%s.%s.sac" % (rec.station, ch)
+            _, _, a = read_sac(os.path.join(str(tmp_path), name))
+            _, _, b = read_sac(os.path.join(str(fdir), name))
+            assert rel_l2(b, a) < 1e-5, name
 
 
 def write_sac_source(path, data, lat, lon, depth, scale, channel, dt):
