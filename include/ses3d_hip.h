@@ -224,6 +224,8 @@ typedef struct s3d_batch s3d_batch;
 int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out);
 int s3d_batch_step(s3d_batch *batch, int nsteps);
 int s3d_batch_synchronize(s3d_batch *batch);
+/* Members cannot be destroyed while a batch holds them: destroy the batch
+ * first. */
 int s3d_batch_destroy(s3d_batch *batch);
 
 #ifdef __cplusplus

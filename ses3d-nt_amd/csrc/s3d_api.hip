@@ -200,6 +200,7 @@ struct s3d_ctx {
     // on the owner the number of simulations using its fields
     s3d_ctx *model = nullptr;
     int sharers = 0;
+    int batches = 0;           // batches this simulation is a member of
     // timing
     bool timing = false;
     std::vector<hipEvent_t> ev[3];   // start/stop pairs per kernel class
@@ -1092,6 +1093,9 @@ int s3d_destroy(s3d_ctx *c)
     if (c->sharers > 0)
         return fail(std::to_string(c->sharers) + " simulation(s) created with s3d_create_shared still use "
                     "this model; destroy them first");
+    if (c->batches > 0)
+        return fail("the simulation is a member of " + std::to_string(c->batches) +
+                    " batch(es); destroy them first");
     (void)hipSetDevice(c->dev);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->model) {
@@ -1773,6 +1777,7 @@ int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out)
         delete b;
         return fail(std::string("batch: ") + hipGetErrorString(e));
     }
+    for (int q = 0; q < n; ++q) sims[q]->batches++;
     *out = b;
     return 0;
 }
@@ -1842,6 +1847,7 @@ int s3d_batch_destroy(s3d_batch *b)
     dfree(b->d_its);
     if (b->ev) (void)hipEventDestroy(b->ev);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    for (s3d_ctx *c : b->sims) c->batches--;
     delete b;
     return 0;
 }

@@ -293,3 +293,14 @@ def test_set_tile_checks():
     d, _ = api.build_case(elastic, precision="fp64")
     with pytest.raises(api.S3dError, match="default tile"):
         d.set_tile(4)
+
+
+def test_batch_member_outlives_batch():
+    case = synth.homogeneous(seed=3, nt=10, nx=4, ny=4, nz=4, n_rec=1)
+    s, _ = api.build_case(case, precision="fp32")
+    b = api.Batch([s])
+    b.step(2)
+    with pytest.raises(api.S3dError, match="member of 1 batch"):
+        s.close()
+    b.close()
+    s.close()
