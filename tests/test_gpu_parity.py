@@ -266,10 +266,13 @@ def test_shared_model_sources_match_independent_runs(precision):
 
 
 @pytest.mark.parametrize("tile", [1, 2, 4])
-def test_set_tile_identical(tile):
+@pytest.mark.parametrize("shape", [(12, 10, 6), (5, 5, 3)])
+def test_set_tile_identical(tile, shape):
     """A tile chosen with s3d_set_tile computes the same per-node arithmetic:
-    traces and fields equal the default tile's bit for bit."""
-    case = synth.layered(seed=6, nt=24, nx=12, ny=10, nz=6, lpd=4, n_rec=4)
+    traces and fields equal the default tile's bit for bit (5x5x3 = 75
+    elements leaves a partly filled last tile)."""
+    nx, ny, nz = shape
+    case = synth.layered(seed=6, nt=24, nx=nx, ny=ny, nz=nz, lpd=4, n_rec=4)
     ref, recs = api.build_case(case, precision="fp32", kahan_sigma="bf16")
     ref.step(24)
     s, _ = api.build_case(case, precision="fp32", kahan_sigma="bf16")
