@@ -150,7 +150,8 @@ CONTAINS
         INTEGER(c_int), PARAMETER :: ids(7) = [S3D_RHOINV, S3D_LAMBDA, S3D_MU, S3D_A, &
                                                S3D_B, S3D_C, S3D_Q]
         REAL(c_double), ALLOCATABLE :: buf(:)
-        REAL(c_double) :: courant
+        REAL(c_double) :: courant, tile_ms
+        INTEGER(c_int) :: tile
 
         ASSOCIATE (cf => configuration)
         g = s3d_grid(cf%nx, cf%ny, cf%nz, cf%lpd, cf%lat_min, cf%lat_max, cf%lon_min, &
@@ -175,6 +176,13 @@ CONTAINS
         END IF
         DEALLOCATE(buf)
         CALL check(s3d_init(ctx), 'init')
+        IF (cf%precision == 'fp32' .AND. .NOT. cf%bf16_static) THEN
+            IF (cf%tile == -1) THEN
+                CALL check(s3d_autotune(ctx, tile, tile_ms), 'autotune')
+            ELSE
+                CALL check(s3d_set_tile(ctx, INT(cf%tile, c_int)), 'set_tile')
+            END IF
+        END IF
         CALL check(s3d_cfl(ctx, courant), 'cfl')
         WRITE(*, '("Courant number:", F5.2)') courant
         END ASSOCIATE

@@ -47,7 +47,8 @@ MODULE ses3d_hip_c
               s3d_locate, s3d_wavelet, s3d_add_source_mt, s3d_add_source_sf, &
               s3d_add_receiver, s3d_step, s3d_current_step, s3d_get_trace, &
               s3d_get_field, s3d_set_field, s3d_vrange, s3d_kinetic_energy, &
-              s3d_synchronize
+              s3d_synchronize, s3d_autotune, s3d_set_tile, s3d_create_shared, &
+              s3d_batch_create, s3d_batch_step, s3d_batch_synchronize, s3d_batch_destroy
     PUBLIC :: check, s3d_error_message
 
     INTERFACE
@@ -173,6 +174,17 @@ MODULE ses3d_hip_c
         INTEGER(c_int) FUNCTION s3d_synchronize(ctx) BIND(C, NAME='s3d_synchronize')
             IMPORT :: c_int, c_ptr
             TYPE(c_ptr), VALUE :: ctx
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_autotune(ctx, tile, ms) BIND(C, NAME='s3d_autotune')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int), INTENT(OUT) :: tile
+            REAL(c_double), INTENT(OUT) :: ms
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_set_tile(ctx, tile) BIND(C, NAME='s3d_set_tile')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int), VALUE :: tile
         END FUNCTION
         ! further simulations on one model, and batches (no counterpart in
         ! the reference; see include/ses3d_hip.h)

@@ -8,7 +8,9 @@
 !! &source_sf, &source_sac, &receiver (any number, read in file order), and
 !! the optional &gpu group of the GPU library: precision ('fp32' | 'fp64'), kahan_v,
 !! kahan_sigma, kahan_sigma_bf16 (stress compensation words in BF16),
-!! bf16_static (moduli, tau and dt/M stored in BF16), device.
+!! bf16_static (moduli, tau and dt/M stored in BF16), device, tile (element
+!! tile of the FP32 step: -1 autotune, 0 default, 1, 2 or 4 elements per
+!! workgroup).
 MODULE configuration_mod
     USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY : ERROR_UNIT, IOSTAT_END
     USE, INTRINSIC :: ISO_C_BINDING, ONLY : c_double, c_int
@@ -64,7 +66,7 @@ MODULE configuration_mod
         ! &gpu
         CHARACTER(LEN=8) :: precision
         LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
-        INTEGER :: device
+        INTEGER :: device, tile
         TYPE(source_entry), ALLOCATABLE :: sources(:)
         TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
         TYPE(output_entry), ALLOCATABLE :: outputs(:)
@@ -113,8 +115,8 @@ CONTAINS
         ! &gpu
         CHARACTER(LEN=8) :: precision
         LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
-        INTEGER :: device
-        NAMELIST /gpu/ precision, kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static, device
+        INTEGER :: device, tile
+        NAMELIST /gpu/ precision, kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static, device, tile
         INTEGER :: lun, ios, n
         TYPE(configuration_typ) :: cf
 
@@ -199,7 +201,7 @@ CONTAINS
 
         ! ---- &gpu (optional) ----
         precision = 'fp32'; kahan_v = .TRUE.; kahan_sigma = .TRUE.; device = 0
-        kahan_sigma_bf16 = .TRUE.; bf16_static = .FALSE.
+        kahan_sigma_bf16 = .TRUE.; bf16_static = .FALSE.; tile = -1
         n = count_groups(lun, 'gpu')
         IF (n > 1) CALL abort('Found ' // i2c(n) // ' &gpu-groups in config-file!')
         IF (n == 1) THEN
@@ -211,6 +213,8 @@ CONTAINS
         cf%precision = precision; cf%kahan_v = kahan_v; cf%kahan_sigma = kahan_sigma
         cf%kahan_sigma_bf16 = kahan_sigma_bf16; cf%bf16_static = bf16_static
         cf%device = device
+        IF (ALL(tile /= [-1, 0, 1, 2, 4])) CALL abort('tile must be -1 (autotune), 0, 1, 2 or 4')
+        cf%tile = tile
 
         CALL parse_sources(lun, cf)
         CALL parse_receivers(lun, cf)

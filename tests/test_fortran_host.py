@@ -206,3 +206,12 @@ def test_netcdf_output_aborts(tmp_path):
     cfg = write_config(case, str(tmp_path), extra="\n&output_netcdf\n timestep_start = 1\n/\n")
     r = run(cfg)
     assert r.returncode != 0 and "NetCDF" in r.stderr
+
+
+def test_tile_option_checked(tmp_path):
+    case = synth.homogeneous(nt=60, nx=6, ny=6, nz=5, n_rec=2)
+    cfg = write_config(case, str(tmp_path))
+    text = open(cfg).read().replace("&gpu\n", "&gpu\n tile = 3\n", 1)
+    open(cfg, "w").write(text)
+    r = run(cfg)
+    assert r.returncode != 0 and "tile must be" in r.stderr
