@@ -1,0 +1,37 @@
+"""bench.py host logic: the one-GPU contract and the algorithmic byte counts
+the roofline figures are computed from."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+
+def test_more_than_one_gpu_exits_with_message():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_algorithmic_bytes_of_the_timed_mode():
+    u = 227673985 / 442368000          # unique / element-local nodes, 192x192x96 p=4
+    mode = bench.DEFAULT_MODE
+    # element kernel: stress 24 r+w, BF16 compensation 12 r+w, memory
+    # variables 72 r+w, moduli and tau 12 r, velocity 12u r, force 12 w
+    assert bench.elem_kernel_bytes(mode, u, True) == pytest.approx(240 + 12 * u)
+    # whole step: adds the node kernel's v, its compensation, dt/M per unique node
+    assert bench.step_bytes(mode, u, True) == pytest.approx(254.76, abs=0.01)
+    # 16 sources sharing the statics
+    assert bench.step_bytes(mode, u, True, S=16) == pytest.approx(
+        bench.step_bytes(mode, u, True) - (12 + 4 * u) * 15 / 16)
+
+
+def test_fp64_and_fp32_modes_sizes():
+    assert bench.mode_sizes("fp64") == (8, 0, 0, 8)
+    assert bench.mode_sizes("fp32+kahan_v+kahan_sigma_bf16") == (4, 4, 2, 4)
+    assert bench.mode_sizes("fp32+kahan_v+bf16_static")[3] == 2
