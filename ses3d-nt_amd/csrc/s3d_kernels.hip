@@ -533,10 +533,10 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
 
     const long fs = (long)a.nel * NP;
     // Copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P,
-    // lk+dz*P). All eight loads are issued unconditionally from clamped
-    // (always valid) addresses and absent copies are replaced by +0, so the
-    // loads overlap instead of waiting behind divergent branches. Adding +0
-    // is exact: the sum equals the reference's theta -> phi -> r sweep.
+    // lk+dz*P), from clamped (always valid) addresses. Only present copies
+    // are loaded (lane-predicated loads: an interior node, the common case,
+    // issues one load instead of eight); absent ones are +0. Adding +0 is
+    // exact: the sum equals the reference's theta -> phi -> r sweep.
     const bool px0 = bx < a.nx, px1 = li == 0 && bx > 0;
     const bool py0 = by < a.ny, py1 = lj == 0 && by > 0;
     const bool pz0 = bz < a.nz, pz1 = lk == 0 && bz > 0;
@@ -545,7 +545,6 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
     const int ez0 = bz < a.nz ? bz : a.nz - 1, ez1 = bz > 0 ? bz - 1 : 0;
     long addr[8];
     bool pres[8];
-    long fallback = -1;
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
@@ -553,21 +552,19 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
         const int n = (li + dx * P) + P1 * ((lj + dy * P) + P1 * (lk + dz * P));
         addr[m] = e * NP + (n < NP ? n : NP - 1);
         pres[m] = (dx ? px1 : px0) && (dy ? py1 : py0) && (dz ? pz1 : pz0);
-        if (pres[m] && fallback < 0) fallback = addr[m];
     }
-    // absent copies load a present copy's word: no extra cache lines
-#pragma unroll
-    for (int m = 0; m < 8; ++m)
-        if (!pres[m]) addr[m] = fallback;
     R acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const R *f = a.force + c * fs;
         R v[8];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = f[addr[m]];
+        for (int m = 0; m < 8; ++m) v[m] = R(0);
+        v[0] = f[addr[0]];   // clamped: in bounds also for a padding block
 #pragma unroll
-        for (int m = 0; m < 8; ++m) v[m] = pres[m] ? v[m] : R(0);
+        for (int m = 1; m < 8; ++m)
+            if (pres[m]) v[m] = f[addr[m]];
+        v[0] = pres[0] ? v[0] : R(0);
         acc[c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
     }
     const long u = tid;
