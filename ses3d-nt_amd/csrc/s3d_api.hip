@@ -1598,18 +1598,21 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
     } else if ((field >= S3D_SXX && field <= S3D_SZZ) || (field >= S3D_MXX && field < S3D_MXX + 18)) {
         const void *src = nullptr;
         const void *lo = nullptr;
+        int nc = 1;     // components of the element-major block
         if (field <= S3D_SZZ) {
-            src = (const char *)c->sig + rsize(c) * nn * (field - S3D_SXX);
-            if (c->sig_lo && c->o.kahan_sigma == 1) lo = (const char *)c->sig_lo + rsize(c) * nn * (field - S3D_SXX);
-            if (c->sig_lo && c->o.kahan_sigma == 2) lo = (const char *)c->sig_lo + 4 * nn * ((field - S3D_SXX) / 2);
+            nc = 6;
+            src = (const char *)c->sig + rsize(c) * c->NP * (field - S3D_SXX);
+            if (c->sig_lo && c->o.kahan_sigma == 1) lo = (const char *)c->sig_lo + rsize(c) * c->NP * (field - S3D_SXX);
+            if (c->sig_lo && c->o.kahan_sigma == 2) lo = (const char *)c->sig_lo + 4 * c->NP * ((field - S3D_SXX) / 2);
         } else if (c->mem) {
-            src = (const char *)c->mem + rsize(c) * nn * (field - S3D_MXX);
+            nc = 18;
+            src = (const char *)c->mem + rsize(c) * c->NP * (field - S3D_MXX);
         }
         if (!src) {
             HIPCHK(hipMemsetAsync(buf, 0, sizeof(double) * nn, c->stream));
         } else if (c->fp32) {
             hipLaunchKernelGGL(from_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                               (const float *)src, buf, c->nel, c->NP);
+                               (const float *)src, buf, c->nel, c->NP, nc);
             if (lo) {
                 // value = hi - lo
                 double *tmp = nullptr;
@@ -1619,7 +1622,7 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
                                        (const unsigned *)lo, (field - S3D_SXX) % 2, tmp, c->nel, c->NP);
                 else
                     hipLaunchKernelGGL(from_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                                       (const float *)lo, tmp, c->nel, c->NP);
+                                       (const float *)lo, tmp, c->nel, c->NP, nc);
                 std::vector<double> a(nn), b(nn);
                 HIPCHK(hipMemcpyAsync(a.data(), buf, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
                 HIPCHK(hipMemcpyAsync(b.data(), tmp, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
@@ -1631,7 +1634,7 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
             }
         } else {
             hipLaunchKernelGGL(from_device_layout<double>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                               (const double *)src, buf, c->nel, c->NP);
+                               (const double *)src, buf, c->nel, c->NP, nc);
         }
     } else {
         (void)hipFree(buf);
@@ -1660,27 +1663,30 @@ int s3d_set_field(s3d_ctx *c, int field, const double *in)
                      : with_p<GatherRun, double>(c->p, c, (const double *)buf, c->v[field], (void *)nullptr);
     } else if ((field >= S3D_SXX && field <= S3D_SZZ) || (field >= S3D_MXX && field < S3D_MXX + 18)) {
         void *dst = nullptr;
+        int nc = 1;     // components of the element-major block
         if (field <= S3D_SZZ) {
-            dst = (char *)c->sig + rsize(c) * nn * (field - S3D_SXX);
+            nc = 6;
+            dst = (char *)c->sig + rsize(c) * c->NP * (field - S3D_SXX);
             if (c->sig_lo && c->o.kahan_sigma == 1)
-                HIPCHK(hipMemsetAsync((char *)c->sig_lo + rsize(c) * nn * (field - S3D_SXX), 0, rsize(c) * nn,
-                                      c->stream));
+                hipLaunchKernelGGL(clear_component<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                                   (float *)c->sig_lo + c->NP * (field - S3D_SXX), c->nel, c->NP, 6);
             if (c->sig_lo && c->o.kahan_sigma == 2)
                 hipLaunchKernelGGL(lo_pair_clear, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                                   (unsigned *)c->sig_lo + nn * ((field - S3D_SXX) / 2), (field - S3D_SXX) % 2,
-                                   nn);
+                                   (unsigned *)c->sig_lo + c->NP * ((field - S3D_SXX) / 2), (field - S3D_SXX) % 2,
+                                   c->nel, c->NP);
         } else if (c->mem) {
-            dst = (char *)c->mem + rsize(c) * nn * (field - S3D_MXX);
+            nc = 18;
+            dst = (char *)c->mem + rsize(c) * c->NP * (field - S3D_MXX);
         } else {
             rc = fail("memory variables exist only for a visco-elastic model");
         }
         if (dst) {
             if (c->fp32)
                 hipLaunchKernelGGL(to_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                                   (const double *)buf, (float *)dst, c->nel, c->NP);
+                                   (const double *)buf, (float *)dst, c->nel, c->NP, nc);
             else
                 hipLaunchKernelGGL(to_device_layout<double>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                                   (const double *)buf, (double *)dst, c->nel, c->NP);
+                                   (const double *)buf, (double *)dst, c->nel, c->NP, nc);
         }
     } else {
         rc = fail("field " + std::to_string(field) + " cannot be set");

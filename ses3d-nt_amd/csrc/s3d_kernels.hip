@@ -111,6 +111,13 @@ struct TileState {
 // compensation 0 off, 1 FP32 words, 2 BF16 words in pairs (a template
 // parameter: a run-time choice leaves register joins that wait for every
 // load in flight).
+// Element-major block fields (stress, its compensation, memory variables):
+// component c of element e occupies the NP entries at (e * nc + c) * NP.
+__device__ __forceinline__ long em_index(long e, int nc, int c, int NP, int node)
+{
+    return (e * nc + c) * NP + node;
+}
+
 template <int P, int E, typename R, bool VISCO, bool S16, bool LOADM, int KS>
 __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileState<R, VISCO> &s)
 {
@@ -124,14 +131,13 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     const int ee = valid ? e : 0;
     const int ex = ee % a.nx, ey = (ee / a.nx) % a.ny, ez = ee / (a.nx * a.ny);
     const int i = node % P1, j = (node / P1) % P1, k = node / (P1 * P1);
-    const long fs = (long)a.nel * NP;
     const long eb = (long)ee * NP + node;
     const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
     s.v[0] = a.vx[u];
     s.v[1] = a.vy[u];
     s.v[2] = a.vz[u];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) s.sg[c] = a.sig[c * fs + eb];
+    for (int c = 0; c < 6; ++c) s.sg[c] = a.sig[em_index(ee, 6, c, NP, node)];
     s.tau = R(0);
     if constexpr (S16) {
         s.lam = (R)(float)a.s16.lam[eb];
@@ -144,7 +150,7 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     }
     if constexpr (VISCO && LOADM) {
 #pragma unroll
-        for (int c = 0; c < 18; ++c) s.M[c] = a.mem[c * fs + eb];
+        for (int c = 0; c < 18; ++c) s.M[c] = a.mem[em_index(ee, 18, c, NP, node)];
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) s.lo[c] = R(0);
@@ -152,10 +158,10 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     for (int q = 0; q < 3; ++q) s.lo2[q] = 0u;
     if constexpr (KS == 2) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) s.lo2[q] = a.sig_lo2[q * fs + eb];
+        for (int q = 0; q < 3; ++q) s.lo2[q] = a.sig_lo2[em_index(ee, 3, q, NP, node)];
     } else if constexpr (KS == 1) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) s.lo[c] = a.sig_lo[c * fs + eb];
+        for (int c = 0; c < 6; ++c) s.lo[c] = a.sig_lo[em_index(ee, 6, c, NP, node)];
     }
 }
 
@@ -196,12 +202,11 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         // operands in registers first: each asm statement clobbers memory,
         // and argument blocks read through a pointer would be reloaded
         const int e = tile0 * EPB + slot;
-        const long eb = (long)((inside && e < a.nel) ? e : 0) * NP + node;
-        const R *mem = a.mem + eb;
+        const R *mem = a.mem + em_index((inside && e < a.nel) ? e : 0, 18, 0, NP, node);
         const unsigned base = (unsigned)(size_t)&mbuf[0][(t >> 6) << 6];
 #pragma unroll
         for (int c = 0; c < 18; ++c)
-            lds_dma_dword(mem + c * fs, base + (unsigned)(c * S::NT * sizeof(R)));
+            lds_dma_dword(mem + c * NP, base + (unsigned)(c * S::NT * sizeof(R)));
     }
 
     // ---- tables of the tile: every lane loads from clamped (in-bounds)
@@ -258,6 +263,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         const int e = tile * EPB + slot;
         const bool valid = inside && e < a.nel;
         const long eb = (long)(valid ? e : 0) * NP + node;    // element-local address
+        const long ev = valid ? e : 0;
         const C *gs = gtab[es];
 
         if (inside) {
@@ -476,23 +482,23 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             if constexpr (KS == 1) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
-                    a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
-                    a.sig_lo[c * fs + eb] = lo[c] - lo[c] * dmp;
+                    a.sig[em_index(ev, 6, c, NP, node)] = sg[c] - sg[c] * dmp;
+                    a.sig_lo[em_index(ev, 6, c, NP, node)] = lo[c] - lo[c] * dmp;
                 }
             } else if constexpr (KS == 2) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
+                for (int c = 0; c < 6; ++c) a.sig[em_index(ev, 6, c, NP, node)] = sg[c] - sg[c] * dmp;
 #pragma unroll
                 for (int q = 0; q < 3; ++q)
-                    a.sig_lo2[q * fs + eb] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
+                    a.sig_lo2[em_index(ev, 3, q, NP, node)] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
                                                        (float)(lo[2 * q + 1] - lo[2 * q + 1] * dmp));
             } else {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) a.sig[c * fs + eb] = sg[c] - sg[c] * dmp;
+                for (int c = 0; c < 6; ++c) a.sig[em_index(ev, 6, c, NP, node)] = sg[c] - sg[c] * dmp;
             }
             if constexpr (VISCO) {
 #pragma unroll
-                for (int c = 0; c < 18; ++c) a.mem[c * fs + eb] = M[c];
+                for (int c = 0; c < 18; ++c) a.mem[em_index(ev, 18, c, NP, node)] = M[c];
             }
         }
     }
@@ -637,7 +643,6 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
     const int lane = threadIdx.x;
     const int e = a.rec_e[r];
     const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
-    const long fs = (long)a.nel * NP;
     double acc[4] = {0, 0, 0, 0};   // vx, vy, vz, pressure
     for (int n = lane; n < NP; n += 64) {
         const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
@@ -647,8 +652,8 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
         acc[0] += w * ((double)a.vx[u] - (a.lx ? (double)a.lx[u] : 0.0));
         acc[1] += w * ((double)a.vy[u] - (a.ly ? (double)a.ly[u] : 0.0));
         acc[2] += w * ((double)a.vz[u] - (a.lz ? (double)a.lz[u] : 0.0));
-        const long eb = (long)e * NP + n;
-        const double tr = (double)a.sig[XX * fs + eb] + (double)a.sig[YY * fs + eb] + (double)a.sig[ZZ * fs + eb];
+        const double tr = (double)a.sig[em_index(e, 6, XX, NP, n)] + (double)a.sig[em_index(e, 6, YY, NP, n)] +
+                          (double)a.sig[em_index(e, 6, ZZ, NP, n)];
         acc[3] += -((double)(1.0f / 3.0f) * tr) * w;
     }
 #pragma unroll
@@ -705,43 +710,59 @@ __global__ void __launch_bounds__(64) advance_many_kernel(int *const *its, int n
 // setup and I/O kernels
 // ---------------------------------------------------------------------------
 
-// reference layout [n][e] (FP64) -> device layout [e][n] (R)
+// Conversions between the reference layout [n][e] (FP64) and one component
+// of an element-major block field of nc components (dst/src point at the
+// component's first entry; element e's entries start at e * nc * NP).
 template <typename R>
-__global__ void to_device_layout(const double *src, R *dst, long nel, int NP)
+__global__ void to_device_layout(const double *src, R *dst, long nel, int NP, int nc)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nel * NP) return;
     long e = t / NP;
     int n = (int)(t - e * NP);
-    dst[t] = (R)src[(long)n * nel + e];
+    dst[e * nc * NP + n] = (R)src[(long)n * nel + e];
 }
 
 template <typename R>
-__global__ void from_device_layout(const R *src, double *dst, long nel, int NP)
+__global__ void from_device_layout(const R *src, double *dst, long nel, int NP, int nc)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nel * NP) return;
     long n = t / nel;
     long e = t - n * nel;
-    dst[t] = (double)src[e * NP + n];
+    dst[t] = (double)src[e * nc * NP + n];
 }
 
-// one half of the packed BF16 stress compensation -> reference layout
+template <typename R>
+__global__ void clear_component(R *p, long nel, int NP, int nc)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nel * NP) return;
+    long e = t / NP;
+    int n = (int)(t - e * NP);
+    p[e * nc * NP + n] = R(0);
+}
+
+// one half of the packed BF16 stress compensation (3 packed components per
+// element) -> reference layout
 __global__ void lo_pair_from_device(const unsigned *src, int half, double *dst, long nel, int NP)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nel * NP) return;
     long n = t / nel;
     long e = t - n * nel;
-    const unsigned w = src[e * NP + n];
+    const unsigned w = src[e * 3 * NP + n];
     dst[t] = (double)(half ? bf16_hi(w) : bf16_lo(w));
 }
 
-// clear one half of the packed BF16 stress compensation
-__global__ void lo_pair_clear(unsigned *p, int half, long n)
+// clear one half of one packed BF16 stress compensation component
+__global__ void lo_pair_clear(unsigned *p, int half, long nel, int NP)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) p[t] &= half ? 0x0000ffffu : 0xffff0000u;
+    if (t >= nel * NP) return;
+    long e = t / NP;
+    int n = (int)(t - e * NP);
+    p[e * 3 * NP + n] &= half ? 0x0000ffffu : 0xffff0000u;
 }
 
 // unique-node field -> reference layout (every copy)
