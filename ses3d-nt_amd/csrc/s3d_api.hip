@@ -167,7 +167,7 @@ struct s3d_ctx {
     // device state
     void *v[3] = {}, *lo[3] = {}, *dt_inv_mm = nullptr, *sig = nullptr, *sig_lo = nullptr,
          *mem = nullptr, *lam = nullptr, *mu = nullptr, *tau = nullptr, *rho = nullptr,
-         *ca = nullptr, *cb = nullptr, *cc = nullptr, *force = nullptr, *tth = nullptr,
+         *ca = nullptr, *cb = nullptr, *cc = nullptr, *force = nullptr, *tth = nullptr, *mat = nullptr,
          *tr = nullptr, *wdev = nullptr;
     double *mm64 = nullptr, *dpx = nullptr, *dpy = nullptr, *dpz = nullptr;
     double bytes = 0;
@@ -1109,7 +1109,7 @@ int s3d_destroy(s3d_ctx *c)
     }
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
     void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->sig,
-                    c->sig_lo, c->mem, c->lam, c->mu, c->tau, c->rho, c->ca, c->cb, c->cc,
+                    c->sig_lo, c->mem, c->mat, c->rho, c->ca, c->cb, c->cc,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
                     c->d_rec_ch, c->d_rec_vm, c->d_traces};
@@ -1320,9 +1320,13 @@ int s3d_init(s3d_ctx *c)
     const size_t ss = c->o.bf16_static ? 2 : rs;   // static fields
     if (dalloc(c, &c->dt_inv_mm, ss * c->nuniq)) return 1;
     if (dalloc(c, (void **)&c->mm64, sizeof(double) * c->nuniq)) return 1;
-    if (dalloc(c, &c->lam, ss * c->nn, false)) return 1;
-    if (dalloc(c, &c->mu, ss * c->nn, false)) return 1;
-    if (c->visco && dalloc(c, &c->tau, ss * c->nn, false)) return 1;
+    // lam, mu (and tau) as one element-major block; lam/mu/tau point at the
+    // components' first entries
+    const int ncm = c->visco ? 3 : 2;
+    if (dalloc(c, &c->mat, ss * ncm * c->nn, false)) return 1;
+    c->lam = c->mat;
+    c->mu = (char *)c->mat + ss * c->NP;
+    c->tau = c->visco ? (char *)c->mat + 2 * ss * c->NP : nullptr;
     if (dalloc(c, &c->rho, rs * c->nn, false)) return 1;
     if (c->aniso) {
         if (dalloc(c, &c->ca, ss * c->nn, false) || dalloc(c, &c->cb, ss * c->nn, false) ||
@@ -1335,19 +1339,19 @@ int s3d_init(s3d_ctx *c)
                            c->stage[S3D_RHOINV], c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q],
                            c->stage[S3D_A], c->stage[S3D_B], c->stage[S3D_C], (bf16 *)c->lam,
                            (bf16 *)c->mu, (bf16 *)c->tau, (float *)c->rho, (bf16 *)c->ca, (bf16 *)c->cb,
-                           (bf16 *)c->cc, c->nel, c->NP, c->g.dt);
+                           (bf16 *)c->cc, c->nel, c->NP, ncm, c->g.dt);
     else if (c->fp32)
         hipLaunchKernelGGL((material_kernel<float, float>), dim3(gm), dim3(256), 0, c->stream,
                            c->stage[S3D_RHOINV], c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q],
                            c->stage[S3D_A], c->stage[S3D_B], c->stage[S3D_C], (float *)c->lam,
                            (float *)c->mu, (float *)c->tau, (float *)c->rho, (float *)c->ca, (float *)c->cb,
-                           (float *)c->cc, c->nel, c->NP, c->g.dt);
+                           (float *)c->cc, c->nel, c->NP, ncm, c->g.dt);
     else
         hipLaunchKernelGGL((material_kernel<double, double>), dim3(gm), dim3(256), 0, c->stream, c->stage[S3D_RHOINV],
                            c->stage[S3D_LAMBDA], c->stage[S3D_MU], c->stage[S3D_Q], c->stage[S3D_A],
                            c->stage[S3D_B], c->stage[S3D_C], (double *)c->lam, (double *)c->mu,
                            (double *)c->tau, (double *)c->rho, (double *)c->ca, (double *)c->cb,
-                           (double *)c->cc, c->nel, c->NP, c->g.dt);
+                           (double *)c->cc, c->nel, c->NP, ncm, c->g.dt);
     HIPCHK(hipGetLastError());
     int rc = c->fp32 ? with_p<MassRun, float>(c->p, c, (const double *)d_sin, (const double *)d_r, (const double *)d_w)
                      : with_p<MassRun, double>(c->p, c, (const double *)d_sin, (const double *)d_r, (const double *)d_w);

@@ -2,12 +2,14 @@
 //
 // Layout in HBM (one simulation):
 //   element-local fields, e = ex + nx*(ey + ny*ez), n = i + P1*(j + P1*k):
-//     state blocks (stress, its compensation, memory variables), element-major
-//       f[e][c][n]: one element's components are one contiguous run, so the
-//       workgroup of a tile of consecutive elements reads and writes one
-//       contiguous region per block field instead of one per component
-//       (fewer concurrent DRAM streams: measured 6 % faster than f[c][e][n]);
-//     material and element force, component-major f[c][e][n].
+//     block fields (stress, its compensation, memory variables, element
+//       force, the moduli lam/mu/tau), element-major f[e][c][n]: one element's
+//       components are one
+//       contiguous run, so the workgroup of a tile of consecutive elements
+//       reads and writes one contiguous region per block field instead of one
+//       per component (fewer concurrent DRAM streams: measured 7 % faster
+//       than f[c][e][n]);
+//     density and anisotropy coefficients, component-major f[c][e][n].
 //     Either way the (p+1)^3 nodes of one element are contiguous and a
 //     workgroup's loads/stores of an element are fully coalesced.
 //   unique-node fields (velocity, its compensation, 1/M):

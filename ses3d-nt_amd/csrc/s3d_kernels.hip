@@ -105,19 +105,20 @@ struct TileState {
     unsigned lo2[3];    // stress compensation, kahan_sigma 2 (packed BF16 pairs)
 };
 
+// Element-major block fields (stress, its compensation, memory variables,
+// element force, lam/mu/tau): component c of element e occupies the NP
+// entries at (e * nc + c) * NP.
+__device__ __forceinline__ long em_index(long e, int nc, int c, int NP, int node)
+{
+    return (e * nc + c) * NP + node;
+}
+
 // Issue the loads of tile `tile` into `s` (no waits). Lanes without an
 // element (the tail of the last tile) read element 0's addresses, so every
 // load is unconditional and in bounds; such lanes store nothing. KS: stress
 // compensation 0 off, 1 FP32 words, 2 BF16 words in pairs (a template
 // parameter: a run-time choice leaves register joins that wait for every
 // load in flight).
-// Element-major block fields (stress, its compensation, memory variables):
-// component c of element e occupies the NP entries at (e * nc + c) * NP.
-__device__ __forceinline__ long em_index(long e, int nc, int c, int NP, int node)
-{
-    return (e * nc + c) * NP + node;
-}
-
 template <int P, int E, typename R, bool VISCO, bool S16, bool LOADM, int KS>
 __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileState<R, VISCO> &s)
 {
@@ -131,7 +132,6 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     const int ee = valid ? e : 0;
     const int ex = ee % a.nx, ey = (ee / a.nx) % a.ny, ez = ee / (a.nx * a.ny);
     const int i = node % P1, j = (node / P1) % P1, k = node / (P1 * P1);
-    const long eb = (long)ee * NP + node;
     const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
     s.v[0] = a.vx[u];
     s.v[1] = a.vy[u];
@@ -139,14 +139,16 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
 #pragma unroll
     for (int c = 0; c < 6; ++c) s.sg[c] = a.sig[em_index(ee, 6, c, NP, node)];
     s.tau = R(0);
+    // lam, mu (, tau): one element-major block, the pointers at its components
+    const long mb = em_index(ee, VISCO ? 3 : 2, 0, NP, node);
     if constexpr (S16) {
-        s.lam = (R)(float)a.s16.lam[eb];
-        s.mu = (R)(float)a.s16.mu[eb];
-        if constexpr (VISCO) s.tau = (R)(float)a.s16.tau[eb];
+        s.lam = (R)(float)a.s16.lam[mb];
+        s.mu = (R)(float)a.s16.mu[mb];
+        if constexpr (VISCO) s.tau = (R)(float)a.s16.tau[mb];
     } else {
-        s.lam = a.lam[eb];
-        s.mu = a.mu[eb];
-        if constexpr (VISCO) s.tau = a.tau[eb];
+        s.lam = a.lam[mb];
+        s.mu = a.mu[mb];
+        if constexpr (VISCO) s.tau = a.tau[mb];
     }
     if constexpr (VISCO && LOADM) {
 #pragma unroll
@@ -190,7 +192,6 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     const bool inside = slot < EPB;
     const int i = node % P1, j = (node / P1) % P1, k = node / (P1 * P1);
     const int L = (inside ? slot : 0) * NP;
-    const long fs = (long)a.nel * NP;      // field stride
     const int tile0 = blk;
 
     // ---- memory variables of the FP32 kernels: streamed straight into LDS
@@ -476,7 +477,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         if (valid) {
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                a.force[c * fs + eb] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
+                a.force[em_index(ev, 3, c, NP, node)] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
 
             // ---- store tapered stress (K13) and memory variables ----
             if constexpr (KS == 1) {
@@ -537,7 +538,6 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
     const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
     if ((bx == a.nx && li) || (by == a.ny && lj) || (bz == a.nz && lk)) return;
 
-    const long fs = (long)a.nel * NP;
     // Copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P,
     // lk+dz*P), from clamped (always valid) addresses. Only present copies
     // are loaded (lane-predicated loads: an interior node, the common case,
@@ -556,13 +556,13 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
         const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
         const long e = (long)(dx ? ex1 : ex0) + (long)a.nx * ((long)(dy ? ey1 : ey0) + (long)a.ny * (dz ? ez1 : ez0));
         const int n = (li + dx * P) + P1 * ((lj + dy * P) + P1 * (lk + dz * P));
-        addr[m] = e * NP + (n < NP ? n : NP - 1);
+        addr[m] = e * 3 * NP + (n < NP ? n : NP - 1);
         pres[m] = (dx ? px1 : px0) && (dy ? py1 : py0) && (dz ? pz1 : pz0);
     }
     R acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const R *f = a.force + c * fs;
+        const R *f = a.force + c * NP;
         R v[8];
 #pragma unroll
         for (int m = 0; m < 8; ++m) v[m] = R(0);
@@ -828,23 +828,24 @@ template <typename R, typename S>
 __global__ void material_kernel(const double *rhoinv, const double *lam, const double *mu,
                                 const double *q, const double *A, const double *B, const double *C,
                                 S *dlam, S *dmu, S *dtau, R *drho, S *dA, S *dB, S *dC,
-                                long nel, int NP, double dt)
+                                long nel, int NP, int ncm, double dt)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nel * NP) return;
     long e = t / NP;
     int n = (int)(t - e * NP);
     long s = (long)n * nel + e;
+    long tm = em_index(e, ncm, 0, NP, n);   // lam/mu/tau: element-major block of ncm
     const double l = lam ? lam[s] : 0.0, m = mu ? mu[s] : 0.0;
     if (dtau) {
         double qq = q ? q[s] : 1.79769313486231570815e+308;
         const double tau = (double)6.0814f * pow(qq, (double)-1.015f);
-        dtau[t] = narrow<S>(tau);
-        dlam[t] = narrow<S>(dt * (l - (2.0 / 3.0) * tau * m));
-        dmu[t] = narrow<S>(dt * (m * (1.0 + tau)));
+        dtau[tm] = narrow<S>(tau);
+        dlam[tm] = narrow<S>(dt * (l - (2.0 / 3.0) * tau * m));
+        dmu[tm] = narrow<S>(dt * (m * (1.0 + tau)));
     } else {
-        dlam[t] = narrow<S>(dt * l);
-        dmu[t] = narrow<S>(dt * m);
+        dlam[tm] = narrow<S>(dt * l);
+        dmu[tm] = narrow<S>(dt * m);
     }
     double ri = rhoinv ? rhoinv[s] : 0.0;
     drho[t] = (R)(ri > 0.0 ? 1.0 / ri : 0.0);
