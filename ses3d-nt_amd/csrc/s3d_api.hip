@@ -152,6 +152,7 @@ struct s3d_ctx {
     s3d_grid g;
     s3d_options o;
     int p, P1, NP, PB;
+    int ns = 6;    // components of the state block
     long nel, nn, nblk, nuniq;
     bool fp32 = false, visco = false, aniso = false, inited = false;
     int dev = 0;
@@ -237,14 +238,19 @@ size_t rsize(const s3d_ctx *c);
 int alloc_state(s3d_ctx *c)
 {
     const size_t rs = rsize(c);
-    const size_t ls = c->o.kahan_sigma == 2 ? 2 : rs;   // BF16 compensation in pairs
     for (int q = 0; q < 3; ++q) {
         if (dalloc(c, &c->v[q], rs * c->nuniq)) return 1;
         if (c->fp32 && c->o.kahan_v && dalloc(c, &c->lo[q], rs * c->nuniq)) return 1;
     }
-    if (dalloc(c, &c->sig, rs * 6 * c->nn)) return 1;
-    if (c->o.kahan_sigma && dalloc(c, &c->sig_lo, ls * 6 * c->nn)) return 1;
-    if (c->visco && dalloc(c, &c->mem, rs * 18 * c->nn)) return 1;
+    // stress, its compensation and the memory variables: one element-major
+    // block of ns words (FP32: 4-byte words, the BF16 compensation in pairs;
+    // FP64 has no compensation); sig_lo and mem point at their first
+    // components and are not separate allocations
+    c->ns = state_components(c->o.kahan_sigma, c->visco);
+    if (dalloc(c, &c->sig, rs * c->ns * c->nn)) return 1;
+    const int ncl = c->ns - 6 - (c->visco ? 18 : 0);
+    c->sig_lo = ncl ? (char *)c->sig + rs * 6 * c->NP : nullptr;
+    c->mem = c->visco ? (char *)c->sig + rs * (6 + ncl) * c->NP : nullptr;
     if (dalloc(c, &c->force, rs * 3 * c->nn, false)) return 1;
     if (dalloc(c, (void **)&c->d_it, sizeof(int))) return 1;
     return 0;
@@ -360,6 +366,7 @@ RecArgs<R> rec_args(s3d_ctx *c)
     a.vx = (decltype(a.vx))((const R *)c->v[0]); a.vy = (decltype(a.vy))((const R *)c->v[1]); a.vz = (decltype(a.vz))((const R *)c->v[2]);
     a.lx = (decltype(a.lx))((const R *)c->lo[0]); a.ly = (decltype(a.ly))((const R *)c->lo[1]); a.lz = (decltype(a.lz))((const R *)c->lo[2]);
     a.sig = (decltype(a.sig))((const R *)c->sig);
+    a.ns = c->ns;
     a.rec_e = (decltype(a.rec_e))(c->d_rec_e); a.rec_nch = (decltype(a.rec_nch))(c->d_rec_nch); a.rec_ch = (decltype(a.rec_ch))(c->d_rec_ch);
     a.rec_vm = (decltype(a.rec_vm))(c->d_rec_vm);
     a.traces = (decltype(a.traces))(c->d_traces);
@@ -1108,8 +1115,9 @@ int s3d_destroy(s3d_ctx *c)
         c->model = nullptr;
     }
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
+    c->sig_lo = c->mem = nullptr;   // parts of the state block c->sig
     void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->sig,
-                    c->sig_lo, c->mem, c->mat, c->rho, c->ca, c->cb, c->cc,
+                    c->mat, c->rho, c->ca, c->cb, c->cc,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
                     c->d_rec_ch, c->d_rec_vm, c->d_traces};
@@ -1602,14 +1610,12 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
     } else if ((field >= S3D_SXX && field <= S3D_SZZ) || (field >= S3D_MXX && field < S3D_MXX + 18)) {
         const void *src = nullptr;
         const void *lo = nullptr;
-        int nc = 1;     // components of the element-major block
+        const int nc = c->ns;   // components of the state block
         if (field <= S3D_SZZ) {
-            nc = 6;
             src = (const char *)c->sig + rsize(c) * c->NP * (field - S3D_SXX);
             if (c->sig_lo && c->o.kahan_sigma == 1) lo = (const char *)c->sig_lo + rsize(c) * c->NP * (field - S3D_SXX);
             if (c->sig_lo && c->o.kahan_sigma == 2) lo = (const char *)c->sig_lo + 4 * c->NP * ((field - S3D_SXX) / 2);
         } else if (c->mem) {
-            nc = 18;
             src = (const char *)c->mem + rsize(c) * c->NP * (field - S3D_MXX);
         }
         if (!src) {
@@ -1623,7 +1629,7 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
                 HIPCHK(hipMalloc(&tmp, sizeof(double) * nn));
                 if (c->o.kahan_sigma == 2)
                     hipLaunchKernelGGL(lo_pair_from_device, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                                       (const unsigned *)lo, (field - S3D_SXX) % 2, tmp, c->nel, c->NP);
+                                       (const unsigned *)lo, (field - S3D_SXX) % 2, tmp, c->nel, c->NP, nc);
                 else
                     hipLaunchKernelGGL(from_device_layout<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
                                        (const float *)lo, tmp, c->nel, c->NP, nc);
@@ -1667,19 +1673,17 @@ int s3d_set_field(s3d_ctx *c, int field, const double *in)
                      : with_p<GatherRun, double>(c->p, c, (const double *)buf, c->v[field], (void *)nullptr);
     } else if ((field >= S3D_SXX && field <= S3D_SZZ) || (field >= S3D_MXX && field < S3D_MXX + 18)) {
         void *dst = nullptr;
-        int nc = 1;     // components of the element-major block
+        const int nc = c->ns;   // components of the state block
         if (field <= S3D_SZZ) {
-            nc = 6;
             dst = (char *)c->sig + rsize(c) * c->NP * (field - S3D_SXX);
             if (c->sig_lo && c->o.kahan_sigma == 1)
                 hipLaunchKernelGGL(clear_component<float>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
-                                   (float *)c->sig_lo + c->NP * (field - S3D_SXX), c->nel, c->NP, 6);
+                                   (float *)c->sig_lo + c->NP * (field - S3D_SXX), c->nel, c->NP, nc);
             if (c->sig_lo && c->o.kahan_sigma == 2)
                 hipLaunchKernelGGL(lo_pair_clear, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
                                    (unsigned *)c->sig_lo + c->NP * ((field - S3D_SXX) / 2), (field - S3D_SXX) % 2,
-                                   c->nel, c->NP);
+                                   c->nel, c->NP, nc);
         } else if (c->mem) {
-            nc = 18;
             dst = (char *)c->mem + rsize(c) * c->NP * (field - S3D_MXX);
         } else {
             rc = fail("memory variables exist only for a visco-elastic model");

@@ -2,12 +2,13 @@
 //
 // Layout in HBM (one simulation):
 //   element-local fields, e = ex + nx*(ey + ny*ez), n = i + P1*(j + P1*k):
-//     block fields (stress, its compensation, memory variables, element
-//       force, the moduli lam/mu/tau), element-major f[e][c][n]: one element's
-//       components are one
+//     block fields, element-major f[e][c][n]: the state block (stress, its
+//       compensation and the memory variables in one block, see
+//       state_components), the element force and the moduli lam/mu/tau.
+//       One element's components are one
 //       contiguous run, so the workgroup of a tile of consecutive elements
 //       reads and writes one contiguous region per block field instead of one
-//       per component (fewer concurrent DRAM streams: measured 7 % faster
+//       per component (fewer concurrent DRAM streams: measured 8 % faster
 //       than f[c][e][n]);
 //     density and anisotropy coefficients, component-major f[c][e][n].
 //     Either way the (p+1)^3 nodes of one element are contiguous and a
@@ -38,6 +39,14 @@ namespace s3d {
 
 constexpr int MAXP1 = 8;       // p <= 7
 constexpr int NRDISS = 3;      // relaxation mechanisms
+
+// Components of the element-major state block: stress (6), its compensation
+// (6 FP32 words for kahan_sigma 1, 3 packed BF16 pairs for 2) and the 18
+// memory variables of a visco-elastic model, in that order.
+constexpr int state_components(int kahan_sigma, bool visco)
+{
+    return 6 + (kahan_sigma == 1 ? 6 : kahan_sigma == 2 ? 3 : 0) + (visco ? 18 : 0);
+}
 
 // strain-rate / stress component order used everywhere: xx xy xz yy yz zz
 enum { XX = 0, XY, XZ, YY, YZ, ZZ };

@@ -105,9 +105,10 @@ struct TileState {
     unsigned lo2[3];    // stress compensation, kahan_sigma 2 (packed BF16 pairs)
 };
 
-// Element-major block fields (stress, its compensation, memory variables,
-// element force, lam/mu/tau): component c of element e occupies the NP
-// entries at (e * nc + c) * NP.
+// Element-major block fields (the state block, element force, lam/mu/tau):
+// component c of element e occupies the NP entries at (e * nc + c) * NP.
+// The state block's parts are addressed through pointers at their first
+// components (sig, sig_lo, mem) with nc = state_components(KS, VISCO).
 __device__ __forceinline__ long em_index(long e, int nc, int c, int NP, int node)
 {
     return (e * nc + c) * NP + node;
@@ -124,6 +125,7 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
 {
     using S = Shape<P, E>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
+    constexpr int NS = state_components(KS, VISCO);
     const int t = threadIdx.x;
     const int slot = t / NP;
     const int node = t - slot * NP;
@@ -137,7 +139,7 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     s.v[1] = a.vy[u];
     s.v[2] = a.vz[u];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) s.sg[c] = a.sig[em_index(ee, 6, c, NP, node)];
+    for (int c = 0; c < 6; ++c) s.sg[c] = a.sig[em_index(ee, NS, c, NP, node)];
     s.tau = R(0);
     // lam, mu (, tau): one element-major block, the pointers at its components
     const long mb = em_index(ee, VISCO ? 3 : 2, 0, NP, node);
@@ -152,7 +154,7 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     }
     if constexpr (VISCO && LOADM) {
 #pragma unroll
-        for (int c = 0; c < 18; ++c) s.M[c] = a.mem[em_index(ee, 18, c, NP, node)];
+        for (int c = 0; c < 18; ++c) s.M[c] = a.mem[em_index(ee, NS, c, NP, node)];
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) s.lo[c] = R(0);
@@ -160,10 +162,10 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     for (int q = 0; q < 3; ++q) s.lo2[q] = 0u;
     if constexpr (KS == 2) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) s.lo2[q] = a.sig_lo2[em_index(ee, 3, q, NP, node)];
+        for (int q = 0; q < 3; ++q) s.lo2[q] = a.sig_lo2[em_index(ee, NS, q, NP, node)];
     } else if constexpr (KS == 1) {
 #pragma unroll
-        for (int c = 0; c < 6; ++c) s.lo[c] = a.sig_lo[em_index(ee, 6, c, NP, node)];
+        for (int c = 0; c < 6; ++c) s.lo[c] = a.sig_lo[em_index(ee, NS, c, NP, node)];
     }
 }
 
@@ -180,6 +182,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     using S = Shape<P, E>;
     using C = coef_t<R>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB, NE = EPB;
+    constexpr int NS = state_components(KS, VISCO);
     __shared__ R lds[6][EPB * NP];
     __shared__ C cf[4][P1 * P1];     // Dx, Dy, Dz, dl*w
     __shared__ C cw[P1];             // GLL weights
@@ -203,7 +206,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         // operands in registers first: each asm statement clobbers memory,
         // and argument blocks read through a pointer would be reloaded
         const int e = tile0 * EPB + slot;
-        const R *mem = a.mem + em_index((inside && e < a.nel) ? e : 0, 18, 0, NP, node);
+        const R *mem = a.mem + em_index((inside && e < a.nel) ? e : 0, NS, 0, NP, node);
         const unsigned base = (unsigned)(size_t)&mbuf[0][(t >> 6) << 6];
 #pragma unroll
         for (int c = 0; c < 18; ++c)
@@ -483,23 +486,23 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             if constexpr (KS == 1) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
-                    a.sig[em_index(ev, 6, c, NP, node)] = sg[c] - sg[c] * dmp;
-                    a.sig_lo[em_index(ev, 6, c, NP, node)] = lo[c] - lo[c] * dmp;
+                    a.sig[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
+                    a.sig_lo[em_index(ev, NS, c, NP, node)] = lo[c] - lo[c] * dmp;
                 }
             } else if constexpr (KS == 2) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) a.sig[em_index(ev, 6, c, NP, node)] = sg[c] - sg[c] * dmp;
+                for (int c = 0; c < 6; ++c) a.sig[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
 #pragma unroll
                 for (int q = 0; q < 3; ++q)
-                    a.sig_lo2[em_index(ev, 3, q, NP, node)] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
+                    a.sig_lo2[em_index(ev, NS, q, NP, node)] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
                                                        (float)(lo[2 * q + 1] - lo[2 * q + 1] * dmp));
             } else {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) a.sig[em_index(ev, 6, c, NP, node)] = sg[c] - sg[c] * dmp;
+                for (int c = 0; c < 6; ++c) a.sig[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
             }
             if constexpr (VISCO) {
 #pragma unroll
-                for (int c = 0; c < 18; ++c) a.mem[em_index(ev, 18, c, NP, node)] = M[c];
+                for (int c = 0; c < 18; ++c) a.mem[em_index(ev, NS, c, NP, node)] = M[c];
             }
         }
     }
@@ -626,6 +629,7 @@ node_kernel_batch(const NodeArgs<R> *__restrict__ args, const int2 *__restrict__
 template <typename R>
 struct RecArgs {
     int nx, ny, nel, nt;
+    int ns;     // components of the state block
     gptr<const R> vx, vy, vz, lx, ly, lz, sig;
     gptr<const int> rec_e, rec_nch, rec_ch;
     gptr<const double> rec_vm;
@@ -652,8 +656,8 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
         acc[0] += w * ((double)a.vx[u] - (a.lx ? (double)a.lx[u] : 0.0));
         acc[1] += w * ((double)a.vy[u] - (a.ly ? (double)a.ly[u] : 0.0));
         acc[2] += w * ((double)a.vz[u] - (a.lz ? (double)a.lz[u] : 0.0));
-        const double tr = (double)a.sig[em_index(e, 6, XX, NP, n)] + (double)a.sig[em_index(e, 6, YY, NP, n)] +
-                          (double)a.sig[em_index(e, 6, ZZ, NP, n)];
+        const double tr = (double)a.sig[em_index(e, a.ns, XX, NP, n)] + (double)a.sig[em_index(e, a.ns, YY, NP, n)] +
+                          (double)a.sig[em_index(e, a.ns, ZZ, NP, n)];
         acc[3] += -((double)(1.0f / 3.0f) * tr) * w;
     }
 #pragma unroll
@@ -743,26 +747,27 @@ __global__ void clear_component(R *p, long nel, int NP, int nc)
     p[e * nc * NP + n] = R(0);
 }
 
-// one half of the packed BF16 stress compensation (3 packed components per
-// element) -> reference layout
-__global__ void lo_pair_from_device(const unsigned *src, int half, double *dst, long nel, int NP)
+// one half of a packed BF16 stress compensation component of a block of nc
+// -> reference layout
+__global__ void lo_pair_from_device(const unsigned *src, int half, double *dst, long nel, int NP, int nc)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nel * NP) return;
     long n = t / nel;
     long e = t - n * nel;
-    const unsigned w = src[e * 3 * NP + n];
+    const unsigned w = src[e * nc * NP + n];
     dst[t] = (double)(half ? bf16_hi(w) : bf16_lo(w));
 }
 
-// clear one half of one packed BF16 stress compensation component
-__global__ void lo_pair_clear(unsigned *p, int half, long nel, int NP)
+// clear one half of one packed BF16 stress compensation component of a block
+// of nc
+__global__ void lo_pair_clear(unsigned *p, int half, long nel, int NP, int nc)
 {
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nel * NP) return;
     long e = t / NP;
     int n = (int)(t - e * NP);
-    p[e * 3 * NP + n] &= half ? 0x0000ffffu : 0xffff0000u;
+    p[e * nc * NP + n] &= half ? 0x0000ffffu : 0xffff0000u;
 }
 
 // unique-node field -> reference layout (every copy)

@@ -197,6 +197,29 @@ def test_precision_modes_against_fp64():
     assert 1e-5 < out["kv+bf16_static"][0] < 5e-2
 
 
+@pytest.mark.parametrize("ks", [False, True, "bf16"])
+def test_fp32_state_block_fields(ks):
+    """Stress and memory variables share one element-major block whose
+    component count depends on the stress compensation (24, 30 or 27
+    words): every field reads back as set (FP32 rounding) and a few steps
+    follow the FP64 oracle. A wrong block stride or component offset gives
+    O(1) errors."""
+    grid = small_grid(nx=3, ny=3, nz=2)
+    rng = np.random.default_rng(11)
+    o, s = _pair(grid, random_model(grid, rng, visco=True), precision="fp32", kahan_v=True,
+                 kahan_sigma=ks)
+    assert o.visco
+    _random_state(o, s, grid, rng, visco=True)
+    state = list(range(api.SXX, api.SZZ + 1)) + list(range(api.MXX, api.MXX + 18))
+    for f in state:
+        assert rel_l2(s.get(f), o.get(f)) < 1e-7, f
+    for _ in range(3):
+        o.step(1)
+        s.step(1)
+    for f in FIELDS + state[6:]:
+        assert rel_l2(s.get(f), o.get(f)) < 1e-4, f
+
+
 def test_bf16_stress_compensation_field_io():
     """With BF16 compensation words a stress field reads back as the FP32
     value that was set (the compensation is cleared on set), and a run
