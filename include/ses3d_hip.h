@@ -61,6 +61,9 @@ enum {
     S3D_VX = 0, S3D_VY, S3D_VZ,
     S3D_SXX, S3D_SXY, S3D_SXZ, S3D_SYY, S3D_SYZ, S3D_SZZ,  /* *_pml stresses */
     S3D_MM = 18, S3D_TAPER = 19,
+    /* adjoint sensitivity densities (reference grad_mod,
+       src/SOURCE/ses3d_grad.F90:33-37), see s3d_grad_add */
+    S3D_GRAD_RHO = 20, S3D_GRAD_CS, S3D_GRAD_CP, S3D_GRAD_CSH, S3D_GRAD_CSV,
     S3D_MXX = 32   /* memory variable: S3D_MXX + 6*mechanism + comp,
                       comp order xx, xy, xz, yy, yz, zz; mechanism 0..2 */
 };
@@ -186,6 +189,21 @@ int s3d_get_trace(s3d_ctx *ctx, int receiver, int channel, double *out);
  * value of each unique node from the highest-index element that holds it. */
 int s3d_get_field(s3d_ctx *ctx, int field, double *out6d);
 int s3d_set_field(s3d_ctx *ctx, int field, const double *in6d);
+
+/* One step of the adjoint workflow's sensitivity accumulation (reference
+ * ses3d_grad, src/SOURCE/ses3d_grad.F90:212-213, called from the adjoint loop
+ * at src/ses3d-nt.F90:318): per element-local node
+ *   grad_rho += increment*dt * (vx_fw*vx + vy_fw*vy + vz_fw*vz) / J,
+ * J = dx dy dz / 8, with the forward velocity snapshot vx_fw, vy_fw, vz_fw
+ * given in the global 6-D layout (FP64) and vx, vy, vz the current
+ * (adjoint) velocity on the device. grad_rho is FP64 on the device, zero
+ * until the first call, read with s3d_get_field(S3D_GRAD_RHO). The caller
+ * picks the steps and snapshots (the reference's saving_vector, :159-174).
+ * The reference accumulates grad_cp, grad_cs, grad_csh and grad_csv from
+ * forward strain rates that it never loads (:184-191); S3D_GRAD_CS,
+ * S3D_GRAD_CP, S3D_GRAD_CSH and S3D_GRAD_CSV read as zero. */
+int s3d_grad_add(s3d_ctx *ctx, const double *vx_fw6d, const double *vy_fw6d, const double *vz_fw6d,
+                 int increment);
 
 /* min and max of vx, vy, vz (reference v_ptp_loc, src/elastic_vars.F90:179-185). */
 int s3d_vrange(s3d_ctx *ctx, double *out2);

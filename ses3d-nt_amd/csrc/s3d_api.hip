@@ -169,6 +169,7 @@ struct s3d_ctx {
     void *v[3] = {}, *lo[3] = {}, *dt_inv_mm = nullptr, *sig = nullptr, *sig_lo = nullptr,
          *mem = nullptr, *lam = nullptr, *mu = nullptr, *tau = nullptr, *rho = nullptr,
          *ca = nullptr, *cb = nullptr, *cc = nullptr, *force = nullptr, *tth = nullptr, *mat = nullptr,
+         *grad_rho = nullptr,
          *tr = nullptr, *wdev = nullptr;
     double *mm64 = nullptr, *dpx = nullptr, *dpy = nullptr, *dpz = nullptr;
     double bytes = 0;
@@ -632,6 +633,21 @@ struct ExpandRun {
     {
         hipLaunchKernelGGL((expand_unique<P, R>), dim3(nblocks(c->nn, 256)), dim3(256), 0, c->stream,
                            (const R *)src, (const R *)lo, dst, c->g.nx, c->g.ny, c->g.nz);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+};
+
+template <int P, typename R>
+struct GradRun {
+    static int run(s3d_ctx *c, const double *f, double inc_dt)
+    {
+        const long nn = c->nn;
+        hipLaunchKernelGGL((grad_rho_kernel<P, R>), dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                           (double *)c->grad_rho, f, f + nn, f + 2 * nn, (const R *)c->v[0],
+                           (const R *)c->v[1], (const R *)c->v[2], (const R *)c->lo[0],
+                           (const R *)c->lo[1], (const R *)c->lo[2], c->g.nx, c->g.ny, c->g.nz,
+                           inc_dt, c->jac);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -1117,7 +1133,7 @@ int s3d_destroy(s3d_ctx *c)
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
     c->sig_lo = c->mem = nullptr;   // parts of the state block c->sig
     void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->sig,
-                    c->mat, c->rho, c->ca, c->cb, c->cc,
+                    c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
                     c->d_rec_ch, c->d_rec_vm, c->d_traces};
@@ -1582,6 +1598,17 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
     if (!c->inited) return fail("s3d_init first");
     HIPCHK(hipSetDevice(c->dev));
     const long nn = c->nn;
+    if (field >= S3D_GRAD_RHO && field <= S3D_GRAD_CSV) {
+        // grad_cs/cp/csh/csv: the reference builds them from forward strain
+        // rates it never loads; they read as zero
+        if (field == S3D_GRAD_RHO && c->grad_rho) {
+            HIPCHK(hipMemcpyAsync(out, c->grad_rho, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        } else {
+            std::memset(out, 0, sizeof(double) * nn);
+        }
+        return 0;
+    }
     if (field == S3D_TAPER) {
         const int P1 = c->P1;
         for (int k = 0; k < P1; ++k)
@@ -1703,6 +1730,34 @@ int s3d_set_field(s3d_ctx *c, int field, const double *in)
     (void)hipFree(buf);
     if (rc) return rc;
     if (e != hipSuccess) return fail(std::string("set_field: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int s3d_grad_add(s3d_ctx *c, const double *vx_fw, const double *vy_fw, const double *vz_fw, int increment)
+{
+    if (!c || !vx_fw || !vy_fw || !vz_fw) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    if (increment < 1) return fail("grad_add: increment must be positive");
+    HIPCHK(hipSetDevice(c->dev));
+    const long nn = c->nn;
+    if (!c->grad_rho && dalloc(c, &c->grad_rho, sizeof(double) * nn)) return 1;
+    double *f = nullptr;
+    HIPCHK(hipMalloc(&f, sizeof(double) * 3 * nn));
+    const double *src[3] = {vx_fw, vy_fw, vz_fw};
+    int rc = 0;
+    for (int q = 0; q < 3 && !rc; ++q)
+        if (hipMemcpyAsync(f + q * nn, src[q], sizeof(double) * nn, hipMemcpyHostToDevice, c->stream) != hipSuccess)
+            rc = fail("grad_add: upload failed");
+    // increment*dt as the reference forms it, before the product and the
+    // division by J
+    const double inc_dt = (double)increment * c->g.dt;
+    if (!rc)
+        rc = c->fp32 ? with_p<GradRun, float>(c->p, c, (const double *)f, inc_dt)
+                     : with_p<GradRun, double>(c->p, c, (const double *)f, inc_dt);
+    hipError_t e = hipStreamSynchronize(c->stream);
+    (void)hipFree(f);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(std::string("grad_add: ") + hipGetErrorString(e));
     return 0;
 }
 

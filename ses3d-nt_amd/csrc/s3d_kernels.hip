@@ -770,6 +770,30 @@ __global__ void lo_pair_clear(unsigned *p, int half, long nel, int NP, int nc)
     p[e * nc * NP + n] &= half ? 0x0000ffffu : 0xffff0000u;
 }
 
+// Adjoint density sensitivity (reference ses3d_grad,
+// src/SOURCE/ses3d_grad.F90:212-213), all arrays in the reference layout
+// [n][e]: g += inc_dt * (fx*vx + fy*vy + fz*vz) / jac, the velocity (hi - lo)
+// gathered from the unique nodes.
+template <int P, typename R>
+__global__ void grad_rho_kernel(double *g, const double *fx, const double *fy, const double *fz,
+                                const R *vx, const R *vy, const R *vz, const R *lx, const R *ly,
+                                const R *lz, int nx, int ny, int nz, double inc_dt, double jac)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1;
+    const long nel = (long)nx * ny * nz;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nel * NP) return;
+    long n = t / nel;
+    long e = t - n * nel;
+    int i = (int)(n % P1), j = (int)((n / P1) % P1), k = (int)(n / (P1 * P1));
+    int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((long)nx * ny));
+    long u = unique_index<P>(ex, ey, ez, i, j, k, nx, ny);
+    const double ax = (double)vx[u] - (lx ? (double)lx[u] : 0.0);
+    const double ay = (double)vy[u] - (ly ? (double)ly[u] : 0.0);
+    const double az = (double)vz[u] - (lz ? (double)lz[u] : 0.0);
+    g[t] = g[t] + inc_dt * ((fx[t] * ax + fy[t] * ay) + fz[t] * az) / jac;
+}
+
 // unique-node field -> reference layout (every copy)
 template <int P, typename R>
 __global__ void expand_unique(const R *src, const R *lo, double *dst, int nx, int ny, int nz)

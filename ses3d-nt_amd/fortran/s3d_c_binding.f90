@@ -35,7 +35,8 @@ MODULE ses3d_hip_c
         S3D_MU = 2, S3D_A = 3, S3D_B = 4, S3D_C = 5, S3D_Q = 6
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_VX = 0, S3D_VY = 1, S3D_VZ = 2, &
         S3D_SXX = 3, S3D_SXY = 4, S3D_SXZ = 5, S3D_SYY = 6, S3D_SYZ = 7, S3D_SZZ = 8, &
-        S3D_MM = 18, S3D_TAPER = 19
+        S3D_MM = 18, S3D_TAPER = 19, S3D_GRAD_RHO = 20, S3D_GRAD_CS = 21, &
+        S3D_GRAD_CP = 22, S3D_GRAD_CSH = 23, S3D_GRAD_CSV = 24
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_CH_VX = 0, S3D_CH_VY = 1, &
         S3D_CH_VZ = 2, S3D_CH_N = 3, S3D_CH_E = 4, S3D_CH_Z = 5, &
         S3D_CH_PRESSURE = 6
@@ -48,7 +49,8 @@ MODULE ses3d_hip_c
               s3d_add_receiver, s3d_step, s3d_current_step, s3d_get_trace, &
               s3d_get_field, s3d_set_field, s3d_vrange, s3d_kinetic_energy, &
               s3d_synchronize, s3d_autotune, s3d_set_tile, s3d_create_shared, &
-              s3d_batch_create, s3d_batch_step, s3d_batch_synchronize, s3d_batch_destroy
+              s3d_batch_create, s3d_batch_step, s3d_batch_synchronize, s3d_batch_destroy, &
+              s3d_grad_add
     PUBLIC :: check, s3d_error_message
 
     INTERFACE
@@ -160,6 +162,13 @@ MODULE ses3d_hip_c
             TYPE(c_ptr), VALUE :: ctx
             INTEGER(c_int), VALUE :: field
             REAL(c_double), INTENT(IN) :: in(*)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_grad_add(ctx, vx_fw, vy_fw, vz_fw, increment) &
+                BIND(C, NAME='s3d_grad_add')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(IN) :: vx_fw(*), vy_fw(*), vz_fw(*)
+            INTEGER(c_int), VALUE :: increment
         END FUNCTION
         INTEGER(c_int) FUNCTION s3d_vrange(ctx, out2) BIND(C, NAME='s3d_vrange')
             IMPORT :: c_int, c_ptr, c_double

@@ -70,6 +70,8 @@ MODULE configuration_mod
         TYPE(source_entry), ALLOCATABLE :: sources(:)
         TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
         TYPE(output_entry), ALLOCATABLE :: outputs(:)
+        ! &grad (workflow 'adjoint'); prefix './'
+        TYPE(output_entry) :: grad
     END TYPE
 
     TYPE(configuration_typ), PUBLIC :: configuration
@@ -141,8 +143,8 @@ CONTAINS
         IF (n /= 1) CALL abort('Found ' // i2c(n) // ' &general-groups in config-file!')
         READ(lun, NML=general, IOSTAT=ios)
         IF (ios /= 0) CALL abort('Error while parsing &general-group!')
-        IF (TRIM(workflow) /= 'forward' .AND. TRIM(workflow) /= 'model') &
-            CALL abort('Unknown workflow ' // TRIM(workflow))
+        IF (TRIM(workflow) /= 'forward' .AND. TRIM(workflow) /= 'model' .AND. &
+            TRIM(workflow) /= 'adjoint') CALL abort('Unknown workflow ' // TRIM(workflow))
         cf%workflow = workflow; cf%event_name = event_name; cf%log_file_dir = log_file_dir
 
         ! ---- &model (src/config/config.F90:160-189) ----
@@ -219,6 +221,7 @@ CONTAINS
         CALL parse_sources(lun, cf)
         CALL parse_receivers(lun, cf)
         CALL parse_outputs(lun, cf)
+        IF (TRIM(cf%workflow) == 'adjoint') CALL parse_grad(lun, cf)
         IF (count_groups(lun, 'output_netcdf') > 0) &
             CALL abort('&output_netcdf: NetCDF output is not provided (use &output_raw)')
         CLOSE(lun)
@@ -333,6 +336,32 @@ CONTAINS
             cf%outputs(i) = output_entry(timestep_start, timestep_increment, timestep_end, &
                                          attributes, override, prefix)
         END DO
+    END SUBROUTINE
+
+    !> The &grad group of workflow 'adjoint' (src/parser/parser.f90:465-500,
+    !! src/SOURCE/ses3d_grad.F90:237-292): exactly one, with at least one
+    !! attribute. timestep_increment must be positive: it is the snapshot
+    !! spacing and the factor of the time integral (the reference would run
+    !! with its default -1, reading a snapshot every step and integrating
+    !! with -dt).
+    SUBROUTINE parse_grad(lun, cf)
+        INTEGER, INTENT(IN) :: lun
+        TYPE(configuration_typ), INTENT(INOUT) :: cf
+        INTEGER :: timestep_start, timestep_increment, timestep_end
+        CHARACTER(LEN=8) :: attributes(max_attributes)
+        NAMELIST /grad/ timestep_start, timestep_increment, timestep_end, attributes
+        INTEGER :: n, ios
+
+        n = count_groups(lun, 'grad')
+        IF (n /= 1) CALL abort('Found ' // i2c(n) // ' &grad-groups in config-file!')
+        timestep_start = -1; timestep_increment = -1; timestep_end = -1; attributes = ''
+        READ(lun, NML=grad, IOSTAT=ios)
+        IF (ios /= 0) CALL abort('Error while parsing &grad-group!')
+        IF (COUNT(attributes /= ' ') < 1) CALL abort('No gradients specified')
+        IF (timestep_increment < 1) CALL abort('Non-conforming time steps in &grad-group!')
+        cf%grad = output_entry(timestep_start, timestep_increment, timestep_end, &
+                               attributes, .FALSE., './')
+        REWIND(lun)
     END SUBROUTINE
 
     !> Number of occurrences of &name in the file; leaves the file rewound.

@@ -1,10 +1,13 @@
 !> Ses3d-NT on one MI355X (reference main program, src/ses3d-nt.F90:30-359),
-!! workflows 'forward' and 'model'.
+!! workflows 'forward', 'adjoint' and 'model'.
 !!
 !!   ses3d_nt <config-file>
 !!
 !! forward: reads the reference's NAMELIST config, uploads the model, runs nt
 !! steps on the GPU and writes one SAC file per receiver channel.
+!! adjoint: the forward loop with the sensitivity accumulation after each
+!! step (src/ses3d-nt.F90:281-321; sources are the adjoint sources, the
+!! forward velocity snapshots come from a forward run's &output_raw).
 !! model: writes the built-in model of model_type to the parameter files
 !! named in the &model group (no GPU needed).
 PROGRAM ses3d_nt
@@ -14,6 +17,7 @@ PROGRAM ses3d_nt
                               write_model_parameters
     USE channels_mod, ONLY : source_cls, receiver_cls, parse_source_file, parse_receiver_file
     USE snapshots_mod, ONLY : write_snapshots, check_snapshot_attributes
+    USE grad_mod, ONLY : ses3d_grad
     IMPLICIT NONE
     CHARACTER(LEN=512) :: conf_file_name
     TYPE(source_cls), ALLOCATABLE :: sources(:)
@@ -53,13 +57,14 @@ PROGRAM ses3d_nt
                 ptp = v_ptp_loc()
                 WRITE(OUTPUT_UNIT, fmt) it, 100 * it / configuration%nt, ptp
             END IF
+            IF (TRIM(configuration%workflow) == 'adjoint') CALL ses3d_grad(ctx, it)
         END DO
 
         DO i = 1, SIZE(receivers)
             CALL receivers(i)%write(ctx)
         END DO
         CALL finalize_evolution()
-        WRITE(OUTPUT_UNIT, '(A, /, "Workflow ''forward'' successfully completed! ", /, A)') &
-            REPEAT('=', 50), REPEAT('=', 50)
+        WRITE(OUTPUT_UNIT, '(A, /, "Workflow ''", A, "'' successfully completed! ", /, A)') &
+            REPEAT('=', 50), TRIM(configuration%workflow), REPEAT('=', 50)
     END IF
 END PROGRAM ses3d_nt

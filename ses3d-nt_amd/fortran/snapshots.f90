@@ -4,7 +4,9 @@
 !! each attribute is written to prefix//attribute//'_'//it//'.vol' as the
 !! global 6-D array, native endian, in the step precision (float32 for
 !! 'fp32', float64 for 'fp64', like the reference's real_kind).
-!! Attributes: vx vy vz sxx sxy sxz syy syz szz pressure taper.
+!! Attributes: vx vy vz sxx sxy sxz syy syz szz pressure taper, and the
+!! sensitivity densities grad_rho grad_cs grad_cp grad_csh grad_csv
+!! (src/volume_snapshot.f90:281-293).
 MODULE snapshots_mod
     USE, INTRINSIC :: ISO_C_BINDING
     USE ses3d_hip_c
@@ -26,7 +28,8 @@ CONTAINS
         DO i = 1, SIZE(configuration%outputs)
             DO j = 1, SIZE(configuration%outputs(i)%attributes)
                 SELECT CASE (TRIM(configuration%outputs(i)%attributes(j)))
-                CASE ('', 'vx', 'vy', 'vz', 'sxx', 'sxy', 'sxz', 'syy', 'syz', 'szz', 'pressure', 'taper')
+                CASE ('', 'vx', 'vy', 'vz', 'sxx', 'sxy', 'sxz', 'syy', 'syz', 'szz', 'pressure', 'taper', &
+                      'grad_rho', 'grad_cs', 'grad_cp', 'grad_csh', 'grad_csv')
                 CASE DEFAULT
                     CALL abort('Snapshot attribute not supported on the GPU path: ' // &
                                TRIM(configuration%outputs(i)%attributes(j)))
@@ -70,6 +73,11 @@ CONTAINS
                 CASE ('syz'); CALL get(ctx, S3D_SYZ, f)
                 CASE ('szz'); CALL get(ctx, S3D_SZZ, f)
                 CASE ('taper'); CALL get(ctx, S3D_TAPER, f)
+                CASE ('grad_rho'); CALL get(ctx, S3D_GRAD_RHO, f)
+                CASE ('grad_cs');  CALL get(ctx, S3D_GRAD_CS, f)
+                CASE ('grad_cp');  CALL get(ctx, S3D_GRAD_CP, f)
+                CASE ('grad_csh'); CALL get(ctx, S3D_GRAD_CSH, f)
+                CASE ('grad_csv'); CALL get(ctx, S3D_GRAD_CSV, f)
                 CASE ('pressure')
                     ! -(sxx + syy + szz) / 3 (src/config/channel_mod.f90:795)
                     IF (.NOT. ALLOCATED(g)) ALLOCATE(g(n))

@@ -26,6 +26,7 @@ RHOINV, LAMBDA, MU, A, B, C_, Q = range(7)
 VX, VY, VZ = 0, 1, 2
 SXX, SXY, SXZ, SYY, SYZ, SZZ = 3, 4, 5, 6, 7, 8
 MM, TAPER = 18, 19
+GRAD_RHO, GRAD_CS, GRAD_CP, GRAD_CSH, GRAD_CSV = 20, 21, 22, 23, 24
 MXX = 32
 CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6}
 WAVELET = {"RICKER": 0, "GAUSS": 1, "SIN**3": 2, "BRUNE": 3, "DELTA": 4}
@@ -38,7 +39,7 @@ EXPORTS = [
     "s3d_get_field", "s3d_set_field", "s3d_vrange", "s3d_kinetic_energy",
     "s3d_synchronize", "s3d_timing_enable", "s3d_kernel_times",
     "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step", "s3d_set_tile",
-    "s3d_batch_synchronize", "s3d_batch_destroy",
+    "s3d_batch_synchronize", "s3d_batch_destroy", "s3d_grad_add",
 ]
 
 
@@ -95,6 +96,7 @@ def lib():
         L.s3d_get_trace.argtypes = [vp, C.c_int, C.c_int, dp]
         L.s3d_get_field.argtypes = [vp, C.c_int, dp]
         L.s3d_set_field.argtypes = [vp, C.c_int, dp]
+        L.s3d_grad_add.argtypes = [vp, dp, dp, dp, C.c_int]
         L.s3d_vrange.argtypes = [vp, dp]
         L.s3d_kinetic_energy.argtypes = [vp, C.POINTER(C.c_double)]
         L.s3d_synchronize.argtypes = [vp]
@@ -266,6 +268,13 @@ class Simulation:
 
     def set(self, field, arr):
         _chk(lib().s3d_set_field(self._h, field, np.ascontiguousarray(arr, np.float64).reshape(-1)))
+
+    def grad_add(self, vx_fw, vy_fw, vz_fw, increment):
+        """grad_rho += increment*dt*(v_fw . v)/J with a forward velocity
+        snapshot in the 6-D layout (reference ses3d_grad,
+        src/SOURCE/ses3d_grad.F90:212-213)."""
+        f = [np.ascontiguousarray(x, np.float64).reshape(-1) for x in (vx_fw, vy_fw, vz_fw)]
+        _chk(lib().s3d_grad_add(self._h, f[0], f[1], f[2], increment))
 
     def vrange(self):
         out = np.zeros(2)

@@ -23,7 +23,7 @@ HIP_FLAGS = [
 
 FORTRAN_SOURCES = ["s3d_c_binding.f90", "gll_mod.f90", "ses3d_config.f90", "earth_models.f90",
                    "sac_io.f90", "channels.f90", "evolution_mod.f90",
-                   "snapshots.f90", "ses3d_nt.f90"]
+                   "snapshots.f90", "grad.f90", "ses3d_nt.f90"]
 
 
 def _run(cmd, cwd=None):

@@ -16,8 +16,8 @@ EXE = os.path.join(ROOT, "ses3d-nt_amd", "bin", "ses3d_nt")
 pytestmark = pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran host not built")
 
 
-def run(cfg, timeout=300):
-    return subprocess.run([EXE, cfg], capture_output=True, text=True, timeout=timeout)
+def run(cfg, timeout=300, cwd=None):
+    return subprocess.run([EXE, cfg], capture_output=True, text=True, timeout=timeout, cwd=cwd)
 
 
 @pytest.mark.parametrize("edit,msg", [
@@ -215,3 +215,75 @@ def test_tile_option_checked(tmp_path):
     open(cfg, "w").write(text)
     r = run(cfg)
     assert r.returncode != 0 and "tile must be" in r.stderr
+
+
+def _raw(prefix, start, inc, end, atts):
+    return ("\n&output_raw\n timestep_start = %d\n timestep_increment = %d\n"
+            " timestep_end = %d\n attributes = %s\n override = .TRUE.\n prefix = '%s'\n/\n"
+            % (start, inc, end, " ".join("'%s'" % a for a in atts), prefix))
+
+
+@pytest.mark.gpu
+def test_adjoint_workflow_accumulates_grad_rho(tmp_path):
+    """Workflow 'adjoint' (src/ses3d-nt.F90:281-321, src/SOURCE/ses3d_grad.F90):
+    a forward run writes velocity snapshots every 2 steps; the adjoint run
+    reads snapshot nt+1-it at every adjoint step it where that is a multiple
+    of timestep_increment and accumulates 2*dt*(v_fw . v)/J. The density it
+    writes at the last step equals that sum over the adjoint run's own
+    velocity snapshots."""
+    import math
+    nt, inc = 20, 2
+    case = synth.homogeneous(nt=nt, nx=6, ny=6, nz=5, n_rec=2)
+    d = str(tmp_path)
+    fwd = write_config(case, d, precision="fp64",
+                       extra=_raw("./", inc, inc, nt, ["vx", "vy", "vz"]))
+    r = run(fwd, cwd=d)
+    assert r.returncode == 0, r.stderr
+    adj_dir = tmp_path / "adj"
+    adj_dir.mkdir()
+    adj = write_config(case, str(adj_dir), precision="fp64", workflow="adjoint",
+                       extra="\n&grad\n timestep_start = 1\n timestep_increment = %d\n"
+                             " timestep_end = %d\n attributes = 'grad_rho'\n/\n" % (inc, nt)
+                             + _raw("./adj_", 1, 2, nt - 1, ["vx", "vy", "vz"])
+                             + _raw("./adj_", nt, 1, nt, ["grad_rho", "grad_cp"]))
+    r = run(adj, cwd=d)
+    print(r.stdout[-400:], r.stderr[-400:])
+    assert r.returncode == 0, r.stderr
+    assert "Workflow 'adjoint' successfully completed" in r.stdout
+    g = case.grid
+    J = (math.radians(g.lat_max - g.lat_min) / g.nx * math.radians(g.lon_max - g.lon_min) / g.ny
+         * (g.rad_max - g.rad_min) / g.nz / 8.0)
+    rd = lambda name: np.fromfile(os.path.join(d, name), dtype="<f8")
+    want = 0.0
+    used = 0
+    for it in range(1, nt + 1):
+        s_fw = nt + 1 - it
+        if s_fw % inc:
+            continue
+        used += 1
+        dot = sum(rd("%s_%d.vol" % (c, s_fw)) * rd("adj_%s_%d.vol" % (c, it)) for c in ("vx", "vy", "vz"))
+        want = want + inc * g.dt * dot / J
+    assert used == nt // inc
+    got = rd("adj_grad_rho_%d.vol" % nt)
+    assert np.abs(want).max() > 0
+    assert rel_l2(got, want) < 1e-12
+    assert not rd("adj_grad_cp_%d.vol" % nt).any()
+
+
+def test_adjoint_needs_one_grad_group(tmp_path):
+    case = synth.homogeneous(nt=20, nx=6, ny=6, nz=5, n_rec=2)
+    cfg = write_config(case, str(tmp_path), workflow="adjoint")
+    r = run(cfg)
+    assert r.returncode != 0
+    assert "Found 0 &grad-groups in config-file!" in r.stderr
+
+
+def test_adjoint_grad_needs_attributes_and_increment(tmp_path):
+    case = synth.homogeneous(nt=20, nx=6, ny=6, nz=5, n_rec=2)
+    for body, msg in (("timestep_increment = 2", "No gradients specified"),
+                      ("attributes = 'grad_rho'", "Non-conforming time steps in &grad-group!")):
+        cfg = write_config(case, str(tmp_path), workflow="adjoint",
+                           extra="\n&grad\n %s\n/\n" % body)
+        r = run(cfg)
+        assert r.returncode != 0
+        assert msg in r.stderr
