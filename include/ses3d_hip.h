@@ -68,10 +68,13 @@ enum {
                       comp order xx, xy, xz, yy, yz, zz; mechanism 0..2 */
 };
 
-/* Receiver channels (reference src/config/channel_mod.f90:270-336). */
+/* Receiver channels (reference src/config/channel_mod.f90:270-336). The curl
+ * channels rot_vx/vy/vz (:620-774) combine the strain rates the step forms
+ * from its start velocity with the velocity at its end, as the reference's
+ * records do; they are not available in batches. */
 enum {
     S3D_CH_VX = 0, S3D_CH_VY, S3D_CH_VZ, S3D_CH_N, S3D_CH_E, S3D_CH_Z,
-    S3D_CH_PRESSURE
+    S3D_CH_PRESSURE, S3D_CH_ROT_VX, S3D_CH_ROT_VY, S3D_CH_ROT_VZ
 };
 
 /* Wavelets (reference src/misc/source_signature_mod.f90:50-147). */

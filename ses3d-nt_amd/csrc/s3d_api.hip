@@ -187,6 +187,11 @@ struct s3d_ctx {
     bool rec_dirty = false;
     int *d_rec_e = nullptr, *d_rec_nch = nullptr, *d_rec_ch = nullptr;
     double *d_rec_vm = nullptr, *d_traces = nullptr;
+    // curl receiver channels: slot per receiver (-1 none), element per slot,
+    // strain-rate parts [nrot][10][NP]
+    int nrot = 0;
+    int *d_rec_rot = nullptr, *d_rot_e = nullptr;
+    double *d_rot_buf = nullptr;
     int nrec_dev = 0;
     int it = 0;
     int *d_it = nullptr;       // device copy of it, advanced by advance_kernel
@@ -370,6 +375,8 @@ RecArgs<R> rec_args(s3d_ctx *c)
     a.ns = c->ns;
     a.rec_e = (decltype(a.rec_e))(c->d_rec_e); a.rec_nch = (decltype(a.rec_nch))(c->d_rec_nch); a.rec_ch = (decltype(a.rec_ch))(c->d_rec_ch);
     a.rec_vm = (decltype(a.rec_vm))(c->d_rec_vm);
+    a.rec_rot = (decltype(a.rec_rot))(c->nrot > 0 ? c->d_rec_rot : nullptr);
+    a.rot_buf = (decltype(a.rot_buf))(c->d_rot_buf);
     a.traces = (decltype(a.traces))(c->d_traces);
     a.it_dev = (decltype(a.it_dev))(c->d_it);
     return a;
@@ -493,8 +500,35 @@ int autotune_p(s3d_ctx *c, int *tile_out, double *ms_out)
 }
 
 template <int P, typename R>
+CurlArgs<R> curl_args(s3d_ctx *c)
+{
+    CurlArgs<R> a;
+    const int P1 = P + 1;
+    a.nx = c->g.nx; a.ny = c->g.ny; a.nrot = c->nrot;
+    a.vx = (decltype(a.vx))((const R *)c->v[0]); a.vy = (decltype(a.vy))((const R *)c->v[1]); a.vz = (decltype(a.vz))((const R *)c->v[2]);
+    a.lx = (decltype(a.lx))((const R *)c->lo[0]); a.ly = (decltype(a.ly))((const R *)c->lo[1]); a.lz = (decltype(a.lz))((const R *)c->lo[2]);
+    a.tth = (decltype(a.tth))((const ThTab<double> *)c->tth);
+    a.tr = (decltype(a.tr))((const RTab<double> *)c->tr);
+    a.rot_e = (decltype(a.rot_e))(c->d_rot_e);
+    a.buf = (decltype(a.buf))(c->d_rot_buf);
+    for (int i = 0; i < P1; ++i)
+        for (int q = 0; q < P1; ++q) {
+            a.Dx[i * P1 + q] = 2.0 * c->dl[q + P1 * i] / c->dx;
+            a.Dy[i * P1 + q] = 2.0 * c->dl[q + P1 * i] / c->dy;
+            a.Dz[i * P1 + q] = 2.0 * c->dl[q + P1 * i] / c->dz;
+        }
+    return a;
+}
+
+template <int P, typename R>
 int launch_step(s3d_ctx *c)
 {
+    if (c->nrot > 0) {
+        // before the node kernel: the curl channels use the start velocity
+        hipLaunchKernelGGL((curl_strain_kernel<P, R>), dim3(c->nrot), dim3(64), 0, c->stream,
+                           curl_args<P, R>(c));
+        HIPCHK(hipGetLastError());
+    }
     ElemArgs<R> ea = elem_args<P, R>(c);
     ev_begin(c, 0);
     if (int rc = launch_elem<P, R>(c, ea, c->stream, c->epb)) return rc;
@@ -765,6 +799,25 @@ int upload_receivers(s3d_ctx *c)
     HIPCHK(hipMemcpy(c->d_rec_nch, c->rec_nch.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_rec_ch, c->rec_ch.data(), sizeof(int) * 8 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_rec_vm, c->rec_vm.data(), sizeof(double) * c->NP * (size_t)n, hipMemcpyHostToDevice));
+    // curl receiver channels
+    dfree(c->d_rec_rot); dfree(c->d_rot_e); dfree(c->d_rot_buf);
+    c->d_rec_rot = c->d_rot_e = nullptr;
+    c->d_rot_buf = nullptr;
+    std::vector<int> slot(n, -1), rot_e;
+    for (int r = 0; r < n; ++r)
+        for (int q = 0; q < c->rec_nch[r]; ++q)
+            if (c->rec_ch[8 * r + q] >= S3D_CH_ROT_VX && slot[r] < 0) {
+                slot[r] = (int)rot_e.size();
+                rot_e.push_back(c->rec_e[r]);
+            }
+    c->nrot = (int)rot_e.size();
+    if (c->nrot > 0) {
+        HIPCHK(hipMalloc(&c->d_rec_rot, sizeof(int) * n));
+        HIPCHK(hipMalloc(&c->d_rot_e, sizeof(int) * c->nrot));
+        HIPCHK(hipMalloc(&c->d_rot_buf, sizeof(double) * 10 * c->NP * (size_t)c->nrot));
+        HIPCHK(hipMemcpy(c->d_rec_rot, slot.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->d_rot_e, rot_e.data(), sizeof(int) * c->nrot, hipMemcpyHostToDevice));
+    }
     c->nrec_dev = n;
     c->rec_dirty = false;
     return 0;
@@ -820,6 +873,7 @@ int batch_build_group_e(BatchGroup &G)
         ea[q].remap = 0;
         na[q] = node_args<P, R>(c);
         ra[q] = rec_args<R>(c);
+        ra[q].rec_rot = nullptr;   // no curl channels in a batch (refused at creation)
         es[q + 1] = es[q] + (int)nblocks(c->nel, S::EPB);
         ns[q + 1] = ns[q] + (int)nblocks(c->nuniq, 256);
         rs[q + 1] = rs[q] + c->nrec_dev;
@@ -1136,7 +1190,7 @@ int s3d_destroy(s3d_ctx *c)
                     c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
-                    c->d_rec_ch, c->d_rec_vm, c->d_traces};
+                    c->d_rec_ch, c->d_rec_vm, c->d_traces, c->d_rec_rot, c->d_rot_e, c->d_rot_buf};
     for (void *b : bufs) dfree(b);
     for (int k = 0; k < 3; ++k)
         for (hipEvent_t e : c->ev[k]) (void)hipEventDestroy(e);
@@ -1530,8 +1584,11 @@ int s3d_add_receiver(s3d_ctx *c, const int *exyz, const double *vm, int nch, con
     if (ex < 0 || ex >= c->g.nx || ey < 0 || ey >= c->g.ny || ez < 0 || ez >= c->g.nz)
         return fail("receiver element outside the mesh");
     for (int q = 0; q < nch; ++q)
-        if (channels[q] < S3D_CH_VX || channels[q] > S3D_CH_PRESSURE)
+        if (channels[q] < S3D_CH_VX || channels[q] > S3D_CH_ROT_VZ)
             return fail("unsupported receiver channel " + std::to_string(channels[q]));
+    for (int q = 0; q < nch; ++q)
+        if (channels[q] >= S3D_CH_ROT_VX && c->batches > 0)
+            return fail("curl receiver channels are not supported in a batch");
     c->rec_e.push_back((int)(ex + (long)c->g.nx * (ey + (long)c->g.ny * ez)));
     c->rec_nch.push_back(nch);
     for (int q = 0; q < 8; ++q) c->rec_ch.push_back(q < nch ? channels[q] : -1);
@@ -1826,6 +1883,8 @@ int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out)
         if (!sims[q] || !sims[q]->inited) return fail("batch members must be initialised (s3d_init)");
         if (sims[q]->dev != sims[0]->dev) return fail("batch members must share one device");
         if (sims[q]->o.bf16_static) return fail("BF16 static fields are not supported in a batch");
+        for (int ch : sims[q]->rec_ch)
+            if (ch >= S3D_CH_ROT_VX) return fail("curl receiver channels are not supported in a batch");
         for (int r = 0; r < q; ++r)
             if (sims[r] == sims[q]) return fail("a simulation appears twice in the batch");
     }

@@ -631,6 +631,8 @@ struct RecArgs {
     int nx, ny, nel, nt;
     int ns;     // components of the state block
     gptr<const R> vx, vy, vz, lx, ly, lz, sig;
+    gptr<const int> rec_rot;      // curl slot per receiver, -1 none (null: no curl receivers)
+    gptr<const double> rot_buf;   // per curl slot [10][NP], see curl_strain_kernel
     gptr<const int> rec_e, rec_nch, rec_ch;
     gptr<const double> rec_vm;
     gptr<double> traces;
@@ -647,21 +649,34 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
     const int lane = threadIdx.x;
     const int e = a.rec_e[r];
     const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
-    double acc[4] = {0, 0, 0, 0};   // vx, vy, vz, pressure
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};   // vx, vy, vz, pressure, rot_vx, rot_vy, rot_vz
+    const int rs = a.rec_rot ? a.rec_rot[r] : -1;
     for (int n = lane; n < NP; n += 64) {
         const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
         const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
         const double w = a.rec_vm[(long)r * NP + n];
         // compensated velocity: value = hi - lo
-        acc[0] += w * ((double)a.vx[u] - (a.lx ? (double)a.lx[u] : 0.0));
-        acc[1] += w * ((double)a.vy[u] - (a.ly ? (double)a.ly[u] : 0.0));
+        const double vxn = (double)a.vx[u] - (a.lx ? (double)a.lx[u] : 0.0);
+        const double vyn = (double)a.vy[u] - (a.ly ? (double)a.ly[u] : 0.0);
+        acc[0] += w * vxn;
+        acc[1] += w * vyn;
         acc[2] += w * ((double)a.vz[u] - (a.lz ? (double)a.lz[u] : 0.0));
         const double tr = (double)a.sig[em_index(e, a.ns, XX, NP, n)] + (double)a.sig[em_index(e, a.ns, YY, NP, n)] +
                           (double)a.sig[em_index(e, a.ns, ZZ, NP, n)];
         acc[3] += -((double)(1.0f / 3.0f) * tr) * w;
+        if (rs >= 0) {
+            // curl channels (src/config/channel_mod.f90:620-774): strain-rate
+            // parts of the step's start velocity, this velocity at its end
+            const double *b = a.rot_buf + (long)rs * 10 * NP + n;
+            const double eyz = b[0], ezy = b[NP], ezx = b[2 * NP], exz = b[3 * NP], exy = b[4 * NP],
+                         eyx = b[5 * NP], rr = b[6 * NP], rst = b[7 * NP], ct = b[8 * NP], st = b[9 * NP];
+            acc[4] += w * (-vyn / rr + eyz / rst + ezy);
+            acc[5] += w * (-ezx + (vxn - exz) / rr);
+            acc[6] += w * ((exy + vyn * ct - eyx / st) / rr);
+        }
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < 7; ++c)
         for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_down(acc[c], off, 64);
     if (lane == 0) {
         const int nch = a.rec_nch[r];
@@ -676,10 +691,91 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
             case 4: val = acc[1]; break;          // e = vy
             case 5: val = acc[2]; break;          // z = vz
             case 6: val = acc[3]; break;          // pressure
+            case 7: val = acc[4]; break;          // rot_vx
+            case 8: val = acc[5]; break;          // rot_vy
+            case 9: val = acc[6]; break;          // rot_vz
             default: break;
             }
             a.traces[((long)r * 8 + q) * a.nt + (it - 1)] = val;
         }
+    }
+}
+
+// Strain-rate parts of the curl receiver channels (reference record_rot_v*,
+// src/config/channel_mod.f90:620-774, which read the step's strain-rate
+// arrays): for each curl receiver's element, from the velocity at the start
+// of the step (hi - lo, FP64), the raw derivatives (src/evolution.f90:126-159),
+// their spherical completion (:165-173) and the symmetrisation (:206-208) as
+// the step forms them; stored per node: EYZ, EZY, EZX, EXZ, EXY, EYX (EYZ,
+// EXZ, EXY symmetrised; EZY, EZX, EYX completed only), r, r sin(theta),
+// cot(theta), sin(theta). Launched before the node kernel updates v.
+template <typename R>
+struct CurlArgs {
+    int nx, ny, nrot;
+    gptr<const R> vx, vy, vz, lx, ly, lz;
+    gptr<const ThTab<double>> tth;
+    gptr<const RTab<double>> tr;
+    gptr<const int> rot_e;       // element of each curl slot
+    gptr<double> buf;            // [nrot][10][NP]
+    double Dx[MAXP1 * MAXP1], Dy[MAXP1 * MAXP1], Dz[MAXP1 * MAXP1];   // D[i*P1 + q] = 2 l'_q(xi_i) / d
+};
+
+template <typename R>
+__device__ __forceinline__ double curl_vel(gptr<const R> v, gptr<const R> l, long u)
+{
+    return (double)v[u] - (l ? (double)l[u] : 0.0);
+}
+
+template <int P, typename R>
+__global__ void __launch_bounds__(64) curl_strain_kernel(const CurlArgs<R> a)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1;
+    const int s = blockIdx.x;
+    if (s >= a.nrot) return;
+    const int e = a.rot_e[s];
+    const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
+    for (int n = threadIdx.x; n < NP; n += 64) {
+        const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
+        double exy = 0.0, exz = 0.0, eyx = 0.0, eyz = 0.0, ezx = 0.0, ezy = 0.0;
+        for (int q = 0; q < P1; ++q) {
+            const double d1 = a.Dx[i * P1 + q];
+            const long u = unique_index<P>(ex, ey, ez, q, j, k, a.nx, a.ny);
+            exy = exy + curl_vel(a.vy, a.ly, u) * d1;
+            exz = exz + curl_vel(a.vz, a.lz, u) * d1;
+        }
+        for (int q = 0; q < P1; ++q) {
+            const double d2 = a.Dy[j * P1 + q];
+            const long u = unique_index<P>(ex, ey, ez, i, q, k, a.nx, a.ny);
+            eyx = eyx + curl_vel(a.vx, a.lx, u) * d2;
+            eyz = eyz + curl_vel(a.vz, a.lz, u) * d2;
+        }
+        for (int q = 0; q < P1; ++q) {
+            const double d3 = a.Dz[k * P1 + q];
+            const long u = unique_index<P>(ex, ey, ez, i, j, q, a.nx, a.ny);
+            ezx = ezx + curl_vel(a.vx, a.lx, u) * d3;
+            ezy = ezy + curl_vel(a.vy, a.ly, u) * d3;
+        }
+        const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
+        const double vxn = curl_vel(a.vx, a.lx, u), vyn = curl_vel(a.vy, a.ly, u);
+        const double r = a.tr[ez * P1 + k].r, st = a.tth[ex * P1 + i].sin_t, ct = a.tth[ex * P1 + i].cot;
+        const double rs = r * st;
+        ezx = -ezx;
+        exz = (exz - vxn) / r;
+        ezy = -ezy;
+        eyz = -vyn / r + eyz / rs;
+        exy = exy / r;
+        eyx = -(vyn * ct) / r + eyx / rs;
+        double *b = a.buf + (long)s * 10 * NP + n;
+        b[0] = (eyz + ezy) / 2.0;
+        b[NP] = ezy;
+        b[2 * NP] = ezx;
+        b[3 * NP] = (exz + ezx) / 2.0;
+        b[4 * NP] = (exy + eyx) / 2.0;
+        b[5 * NP] = eyx;
+        b[6 * NP] = r;
+        b[7 * NP] = rs;
+        b[8 * NP] = ct;
+        b[9 * NP] = st;
     }
 }
 

@@ -204,6 +204,9 @@ CONTAINS
                 CASE ('e');  r%codes(r%nch) = S3D_CH_E
                 CASE ('z');  r%codes(r%nch) = S3D_CH_Z
                 CASE ('pressure'); r%codes(r%nch) = S3D_CH_PRESSURE
+                CASE ('rot_vx'); r%codes(r%nch) = S3D_CH_ROT_VX
+                CASE ('rot_vy'); r%codes(r%nch) = S3D_CH_ROT_VY
+                CASE ('rot_vz'); r%codes(r%nch) = S3D_CH_ROT_VZ
                 CASE DEFAULT
                     CALL abort('Receiver channel not supported on the GPU path: ' // &
                                TRIM(r%info%attributes(j)))

@@ -28,7 +28,8 @@ SXX, SXY, SXZ, SYY, SYZ, SZZ = 3, 4, 5, 6, 7, 8
 MM, TAPER = 18, 19
 GRAD_RHO, GRAD_CS, GRAD_CP, GRAD_CSH, GRAD_CSV = 20, 21, 22, 23, 24
 MXX = 32
-CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6}
+CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6,
+      "rot_vx": 7, "rot_vy": 8, "rot_vz": 9}
 WAVELET = {"RICKER": 0, "GAUSS": 1, "SIN**3": 2, "BRUNE": 3, "DELTA": 4}
 
 EXPORTS = [

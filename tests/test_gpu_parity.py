@@ -354,3 +354,28 @@ def test_batch_with_bf16_compensation_matches_individual_runs():
             for ch in range(3):
                 assert np.array_equal(s2.trace(r, ch), s1.trace(r, ch))
     b.close()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+def test_curl_channels_match_oracle(precision, tol):
+    """rot_vx, rot_vy, rot_vz (src/config/channel_mod.f90:620-774): the strain
+    rates formed from the step's start velocity combined with the velocity at
+    its end, against the FP64 oracle's record of the same channels; the run
+    is longer than one captured chunk, so the graph path records them too."""
+    from dataclasses import replace
+    case = synth.homogeneous(nt=120, nx=6, ny=6, nz=5, n_rec=4, visco=True)
+    case = replace(case, receivers=[replace(r, channels=("vx", "rot_vx", "rot_vy", "rot_vz"))
+                                    for r in case.receivers])
+    o, recs = oracle.build_case(case)
+    o.step(case.grid.nt)
+    s, recs2 = api.build_case(case, precision=precision)
+    assert len(recs2) == len(recs)
+    s.step(case.grid.nt)
+    for ri in range(len(recs)):
+        for ci in range(4):
+            ref = o.trace(ri, ci)
+            assert np.linalg.norm(ref) > 0
+            assert rel_l2(s.trace(ri, ci), ref) < tol, (ri, ci)
+    with pytest.raises(api.S3dError):
+        api.Batch([s])
+    s.close()
