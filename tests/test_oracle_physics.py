@@ -379,3 +379,59 @@ def test_anisotropic_stress_of_a_dilatation():
     assert np.max(np.abs(sxx - exp_h)) < 1e-12 * abs(exp_h)
     assert np.max(np.abs(syy - exp_h)) < 1e-12 * abs(exp_h)
     assert np.max(np.abs(szz - exp_z)) < 1e-12 * abs(exp_z)
+
+
+
+def test_curl_channels_closed_form():
+    """rot_vx, rot_vy, rot_vz as the reference's record_rot_v* read them
+    (src/config/channel_mod.f90:620-774), for a velocity field linear in
+    (theta, phi, r), which the GLL basis differentiates exactly. With zero
+    moduli the step leaves v as it is, and the records at a node (vm = delta)
+    follow from the analytic derivatives through the step's completion
+    (src/evolution.f90:165-173) and symmetrisation (:206-208). The
+    coefficients make every term of each channel comparable in size, so a
+    dropped term, a sign, or a wrong geometry factor changes the value."""
+    g = GRID
+    o = _oracle(g, vp=0.0, vs=0.0)
+    th, ph, r = node_coords(o)
+    P1 = g.lpd + 1
+    ex, ey, ez, i, j, k = 1, 1, 1, 1, 2, 3
+    node = (k, j, i, ez, ey, ex)
+    t0, p0, r0 = th[node], ph[node], r[node]
+    s_, ct = math.sin(t0), math.cos(t0) / math.sin(t0)
+    assert abs(ct) > 1e-2
+    # v_theta = a (r - r0) + b (phi - phi0) + cx; v_phi = d (theta - theta0) +
+    # e (r - r0) + cy; v_r = f (phi - phi0) + h (theta - theta0)
+    a, cx = 0.5, 0.5 * r0
+    e, cy = 1.0, r0
+    f = 2.0 * r0 * r0 * s_
+    h = 4.0 * r0 * r0 * 0.25
+    b = abs(cy * ct) * r0 * s_
+    d = abs(cy * ct) * r0
+    _set_v(o, a * (r - r0) + b * (ph - p0) + cx,
+           d * (th - t0) + e * (r - r0) + cy,
+           f * (ph - p0) + h * (th - t0))
+    vm = np.zeros(P1 ** 3)
+    vm[i + P1 * (j + P1 * k)] = 1.0
+    o.add_receiver(np.array([ex, ey, ez]), vm, ["rot_vx", "rot_vy", "rot_vz"])
+    o.step(1)
+    rs = r0 * s_
+    # completed rates (z runs downwards in the element order: the completed
+    # radial derivative is d/dr)
+    ezx, ezy = a, e
+    exz = (h - cx) / r0
+    eyz = -cy / r0 + f / rs
+    exy = d / r0
+    eyx = -(cy * ct) / r0 + b / rs
+    EXY, EXZ, EYZ = (exy + eyx) / 2, (exz + ezx) / 2, (eyz + ezy) / 2
+    terms = [[-cy / r0, EYZ / rs, ezy],
+             [-ezx, cx / r0, -EXZ / r0],
+             [EXY / r0, cy * ct / r0, -(eyx / s_) / r0]]
+    for q in range(3):
+        want = sum(terms[q])
+        got = o.trace(0, q)[0]
+        assert abs(got - want) < 1e-9 * abs(want), (q, got, want)
+    # every term is at least a tenth of its channel
+    for q, row in enumerate(terms):
+        for n, x in enumerate(row):
+            assert abs(x) > 0.1 * abs(sum(row)), (q, n, row)
