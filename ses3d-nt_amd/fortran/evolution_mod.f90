@@ -13,7 +13,7 @@ MODULE evolution_mod
     TYPE(c_ptr), PUBLIC :: ctx = c_null_ptr
 
     PUBLIC :: init_temp, ses3d_evolution, v_ptp_loc, finalize_evolution, &
-              write_model_parameters
+              write_model_parameters, model_param
 
 CONTAINS
 
@@ -139,6 +139,27 @@ CONTAINS
             CLOSE(lun)
             WRITE(*, '(" writing: ", A)') TRIM(cf%file(p))
         END DO
+        END ASSOCIATE
+    END SUBROUTINE
+
+    !> Model parameter p (1 rhoinv, 2 lambda, 3 mu, 4 a, 5 b, 6 c, 7 q) in
+    !! the global 6-D layout, FP64, as init_temp uploads it: from the built-in
+    !! model of model_type or from the parameter file. Without a file (and for
+    !! q of a built-in model) a, b, c are 0 and q is HUGE (elastic).
+    SUBROUTINE model_param(p, buf)
+        INTEGER, INTENT(IN) :: p
+        REAL(c_double), INTENT(OUT) :: buf(:)
+        REAL(rk), ALLOCATABLE :: v(:, :, :)
+        ASSOCIATE (cf => configuration)
+        IF (cf%model_type /= -1 .AND. p <= 6) THEN
+            ALLOCATE(v(6, 0:cf%nz - 1, 0:cf%lpd))
+            CALL builtin_profile(cf, v)
+            CALL expand_profile(cf, v(p, :, :), buf)
+        ELSE IF (cf%model_type /= -1 .OR. cf%file(p) == 'N/A') THEN
+            buf = MERGE(REAL(HUGE(0.0), c_double), 0.0_c_double, p == 7)
+        ELSE
+            CALL read_model_file(cf%file(p), SIZE(buf, KIND=8), buf)
+        END IF
         END ASSOCIATE
     END SUBROUTINE
 

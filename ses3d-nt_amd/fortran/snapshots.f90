@@ -6,11 +6,15 @@
 !! 'fp32', float64 for 'fp64', like the reference's real_kind).
 !! Attributes: vx vy vz sxx sxy sxz syy syz szz pressure taper, and the
 !! sensitivity densities grad_rho grad_cs grad_cp grad_csh grad_csv
-!! (src/volume_snapshot.f90:281-293).
+!! (src/volume_snapshot.f90:281-293), and the model: rho lambda mu vs/cs
+!! vp/cp a b c q (:254-279), rebuilt on the host from the model files or the
+!! built-in model. Not provided: the displacement-gradient fields exx..ezz
+!! (not integrated on the GPU) and volume curl fields.
 MODULE snapshots_mod
     USE, INTRINSIC :: ISO_C_BINDING
     USE ses3d_hip_c
     USE configuration_mod, ONLY : configuration, output_entry, abort
+    USE evolution_mod, ONLY : model_param
     IMPLICIT NONE
     PRIVATE
     PUBLIC :: write_snapshots, check_snapshot_attributes
@@ -29,13 +33,24 @@ CONTAINS
             DO j = 1, SIZE(configuration%outputs(i)%attributes)
                 SELECT CASE (TRIM(configuration%outputs(i)%attributes(j)))
                 CASE ('', 'vx', 'vy', 'vz', 'sxx', 'sxy', 'sxz', 'syy', 'syz', 'szz', 'pressure', 'taper', &
-                      'grad_rho', 'grad_cs', 'grad_cp', 'grad_csh', 'grad_csv')
+                      'grad_rho', 'grad_cs', 'grad_cp', 'grad_csh', 'grad_csv', 'rho', 'lambda', 'mu', &
+                      'vs', 'cs', 'vp', 'cp', 'a', 'b', 'c', 'q')
                 CASE DEFAULT
                     CALL abort('Snapshot attribute not supported on the GPU path: ' // &
                                TRIM(configuration%outputs(i)%attributes(j)))
                 END SELECT
             END DO
         END DO
+    END SUBROUTINE
+
+    !> rho = 1/rhoinv where rhoinv > 0, else 0 (src/model_paras.F90:398-402)
+    SUBROUTINE rho_of(f)
+        REAL(c_double), INTENT(INOUT) :: f(:)
+        WHERE (f > 0.0_c_double)
+            f = 1.0_c_double / f
+        ELSEWHERE
+            f = 0.0_c_double
+        END WHERE
     END SUBROUTINE
 
     SUBROUTINE get(ctx, id, f)
@@ -73,6 +88,31 @@ CONTAINS
                 CASE ('syz'); CALL get(ctx, S3D_SYZ, f)
                 CASE ('szz'); CALL get(ctx, S3D_SZZ, f)
                 CASE ('taper'); CALL get(ctx, S3D_TAPER, f)
+                CASE ('rho')
+                    CALL model_param(1, f)
+                    CALL rho_of(f)
+                CASE ('lambda'); CALL model_param(2, f)
+                CASE ('mu'); CALL model_param(3, f)
+                CASE ('a'); CALL model_param(4, f)
+                CASE ('b'); CALL model_param(5, f)
+                CASE ('c'); CALL model_param(6, f)
+                CASE ('q'); CALL model_param(7, f)
+                CASE ('vs', 'cs')
+                    ! SQRT(mu / rho) (src/volume_snapshot.f90:260-261)
+                    IF (.NOT. ALLOCATED(g)) ALLOCATE(g(n))
+                    CALL model_param(1, g)
+                    CALL rho_of(g)
+                    CALL model_param(3, f)
+                    f = SQRT(f / g)
+                CASE ('vp', 'cp')
+                    ! SQRT((lambda + 2 mu) / rho) (src/volume_snapshot.f90:263-264)
+                    IF (.NOT. ALLOCATED(g)) ALLOCATE(g(n))
+                    CALL model_param(2, f)
+                    CALL model_param(3, g)
+                    f = f + 2 * g
+                    CALL model_param(1, g)
+                    CALL rho_of(g)
+                    f = SQRT(f / g)
                 CASE ('grad_rho'); CALL get(ctx, S3D_GRAD_RHO, f)
                 CASE ('grad_cs');  CALL get(ctx, S3D_GRAD_CS, f)
                 CASE ('grad_cp');  CALL get(ctx, S3D_GRAD_CP, f)

@@ -287,3 +287,27 @@ def test_adjoint_grad_needs_attributes_and_increment(tmp_path):
         r = run(cfg)
         assert r.returncode != 0
         assert msg in r.stderr
+
+
+@pytest.mark.gpu
+def test_model_snapshot_attributes(tmp_path):
+    """&output_raw attributes rho lambda mu vs vp a q
+    (src/volume_snapshot.f90:254-279) against the model files: rho =
+    1/rhoinv, vs = sqrt(mu/rho), vp = sqrt((lambda + 2 mu)/rho); a without
+    a file is 0, q without a file HUGE (elastic)."""
+    case = synth.homogeneous(nt=20, nx=6, ny=6, nz=5, n_rec=2)
+    d = str(tmp_path)
+    atts = ["rho", "lambda", "mu", "vs", "vp", "a", "q"]
+    cfg = write_config(case, d, precision="fp64", extra=_raw(d + "/", 1, 1, 1, atts))
+    r = run(cfg)
+    assert r.returncode == 0, r.stderr
+    rd = lambda name: np.fromfile(os.path.join(d, name), dtype="<f8")
+    rhoinv, lam, mu = rd("rhoinv"), rd("lambda"), rd("mu")
+    rho = 1.0 / rhoinv
+    want = {"rho": rho, "lambda": lam, "mu": mu, "vs": np.sqrt(mu / rho),
+            "vp": np.sqrt((lam + 2 * mu) / rho), "a": np.zeros_like(mu),
+            "q": np.full_like(mu, float(np.finfo(np.float32).max))}
+    for a in atts:
+        got = rd("%s_1.vol" % a)
+        assert got.shape == want[a].shape
+        assert np.allclose(got, want[a], rtol=1e-15, atol=0), a
