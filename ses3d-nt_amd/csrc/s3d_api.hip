@@ -169,7 +169,7 @@ struct s3d_ctx {
     void *v[3] = {}, *lo[3] = {}, *dt_inv_mm = nullptr, *sig = nullptr, *sig_lo = nullptr,
          *mem = nullptr, *lam = nullptr, *mu = nullptr, *tau = nullptr, *rho = nullptr,
          *ca = nullptr, *cb = nullptr, *cc = nullptr, *force = nullptr, *tth = nullptr, *mat = nullptr,
-         *grad_rho = nullptr,
+         *grad_rho = nullptr, *grad_stage = nullptr,   // FP64 density; 3 snapshot components
          *tr = nullptr, *wdev = nullptr;
     double *mm64 = nullptr, *dpx = nullptr, *dpy = nullptr, *dpz = nullptr;
     double bytes = 0;
@@ -1187,7 +1187,7 @@ int s3d_destroy(s3d_ctx *c)
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
     c->sig_lo = c->mem = nullptr;   // parts of the state block c->sig
     void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->sig,
-                    c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho,
+                    c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho, c->grad_stage,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
                     c->d_rec_ch, c->d_rec_vm, c->d_traces, c->d_rec_rot, c->d_rot_e, c->d_rot_buf};
@@ -1798,8 +1798,9 @@ int s3d_grad_add(s3d_ctx *c, const double *vx_fw, const double *vy_fw, const dou
     HIPCHK(hipSetDevice(c->dev));
     const long nn = c->nn;
     if (!c->grad_rho && dalloc(c, &c->grad_rho, sizeof(double) * nn)) return 1;
-    double *f = nullptr;
-    HIPCHK(hipMalloc(&f, sizeof(double) * 3 * nn));
+    // the snapshot staging buffer is kept for the next saved step
+    if (!c->grad_stage && dalloc(c, &c->grad_stage, sizeof(double) * 3 * nn, false)) return 1;
+    double *f = (double *)c->grad_stage;
     const double *src[3] = {vx_fw, vy_fw, vz_fw};
     int rc = 0;
     for (int q = 0; q < 3 && !rc; ++q)
@@ -1812,7 +1813,6 @@ int s3d_grad_add(s3d_ctx *c, const double *vx_fw, const double *vy_fw, const dou
         rc = c->fp32 ? with_p<GradRun, float>(c->p, c, (const double *)f, inc_dt)
                      : with_p<GradRun, double>(c->p, c, (const double *)f, inc_dt);
     hipError_t e = hipStreamSynchronize(c->stream);
-    (void)hipFree(f);
     if (rc) return rc;
     if (e != hipSuccess) return fail(std::string("grad_add: ") + hipGetErrorString(e));
     return 0;
