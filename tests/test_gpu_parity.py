@@ -356,14 +356,15 @@ def test_batch_with_bf16_compensation_matches_individual_runs():
     b.close()
 
 
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
-def test_curl_channels_match_oracle(precision, tol):
+@pytest.mark.parametrize("precision,tol,lpd", [("fp64", 1e-10, 4), ("fp32", 1e-4, 4),
+                                               ("fp64", 1e-10, 2), ("fp64", 1e-10, 7)])
+def test_curl_channels_match_oracle(precision, tol, lpd):
     """rot_vx, rot_vy, rot_vz (src/config/channel_mod.f90:620-774): the strain
     rates formed from the step's start velocity combined with the velocity at
     its end, against the FP64 oracle's record of the same channels; the run
     is longer than one captured chunk, so the graph path records them too."""
     from dataclasses import replace
-    case = synth.homogeneous(nt=120, nx=6, ny=6, nz=5, n_rec=4, visco=True)
+    case = synth.homogeneous(nt=120, nx=6, ny=6, nz=5, lpd=lpd, n_rec=4, visco=True)
     case = replace(case, receivers=[replace(r, channels=("vx", "rot_vx", "rot_vy", "rot_vz"))
                                     for r in case.receivers])
     o, recs = oracle.build_case(case)
