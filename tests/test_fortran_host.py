@@ -224,7 +224,8 @@ def _raw(prefix, start, inc, end, atts):
 
 
 @pytest.mark.gpu
-def test_adjoint_workflow_accumulates_grad_rho(tmp_path):
+@pytest.mark.parametrize("precision,dtype,tol", [("fp64", "<f8", 1e-12), ("fp32", "<f4", 1e-5)])
+def test_adjoint_workflow_accumulates_grad_rho(tmp_path, precision, dtype, tol):
     """Workflow 'adjoint' (src/ses3d-nt.F90:281-321, src/SOURCE/ses3d_grad.F90):
     a forward run writes velocity snapshots every 2 steps; the adjoint run
     reads snapshot nt+1-it at every adjoint step it where that is a multiple
@@ -235,13 +236,13 @@ def test_adjoint_workflow_accumulates_grad_rho(tmp_path):
     nt, inc = 20, 2
     case = synth.homogeneous(nt=nt, nx=6, ny=6, nz=5, n_rec=2)
     d = str(tmp_path)
-    fwd = write_config(case, d, precision="fp64",
+    fwd = write_config(case, d, precision=precision,
                        extra=_raw("./", inc, inc, nt, ["vx", "vy", "vz"]))
     r = run(fwd, cwd=d)
     assert r.returncode == 0, r.stderr
     adj_dir = tmp_path / "adj"
     adj_dir.mkdir()
-    adj = write_config(case, str(adj_dir), precision="fp64", workflow="adjoint",
+    adj = write_config(case, str(adj_dir), precision=precision, workflow="adjoint",
                        extra="\n&grad\n timestep_start = 1\n timestep_increment = %d\n"
                              " timestep_end = %d\n attributes = 'grad_rho'\n/\n" % (inc, nt)
                              + _raw("./adj_", 1, 2, nt - 1, ["vx", "vy", "vz"])
@@ -253,7 +254,7 @@ def test_adjoint_workflow_accumulates_grad_rho(tmp_path):
     g = case.grid
     J = (math.radians(g.lat_max - g.lat_min) / g.nx * math.radians(g.lon_max - g.lon_min) / g.ny
          * (g.rad_max - g.rad_min) / g.nz / 8.0)
-    rd = lambda name: np.fromfile(os.path.join(d, name), dtype="<f8")
+    rd = lambda name: np.fromfile(os.path.join(d, name), dtype=dtype).astype(np.float64)
     want = 0.0
     used = 0
     for it in range(1, nt + 1):
@@ -266,7 +267,7 @@ def test_adjoint_workflow_accumulates_grad_rho(tmp_path):
     assert used == nt // inc
     got = rd("adj_grad_rho_%d.vol" % nt)
     assert np.abs(want).max() > 0
-    assert rel_l2(got, want) < 1e-12
+    assert rel_l2(got, want) < tol
     assert not rd("adj_grad_cp_%d.vol" % nt).any()
 
 
