@@ -200,7 +200,9 @@ int s3d_set_field(s3d_ctx *ctx, int field, const double *in6d);
  * J = dx dy dz / 8, with the forward velocity snapshot vx_fw, vy_fw, vz_fw
  * given in the global 6-D layout (FP64) and vx, vy, vz the current
  * (adjoint) velocity on the device. grad_rho is FP64 on the device, zero
- * until the first call, read with s3d_get_field(S3D_GRAD_RHO). The caller
+ * until the first call, read with s3d_get_field(S3D_GRAD_RHO); the first
+ * call also allocates a staging buffer of three FP64 fields, both kept until
+ * s3d_destroy (4 x 8 B per element-local node). The caller
  * picks the steps and snapshots (the reference's saving_vector, :159-174).
  * The reference accumulates grad_cp, grad_cs, grad_csh and grad_csv from
  * forward strain rates that it never loads (:184-191); S3D_GRAD_CS,
