@@ -71,11 +71,20 @@ enum {
 /* Receiver channels (reference src/config/channel_mod.f90:270-336). The curl
  * channels rot_vx/vy/vz (:620-774) combine the strain rates the step forms
  * from its start velocity with the velocity at its end, as the reference's
- * records do; they are not available in batches. */
+ * records do; they are not available in batches. The source-echo channels
+ * mxx .. mzz and fx, fy, fz (record_m**, record_f*, :871-1085) sample the
+ * moment-tensor glut and the single force that the step's add(it) wrote into
+ * the receiver's element, i.e. sum(vm * src_**) with the source-time sample
+ * of this step; they read 0 in an element without such a source component. */
 enum {
     S3D_CH_VX = 0, S3D_CH_VY, S3D_CH_VZ, S3D_CH_N, S3D_CH_E, S3D_CH_Z,
-    S3D_CH_PRESSURE, S3D_CH_ROT_VX, S3D_CH_ROT_VY, S3D_CH_ROT_VZ
+    S3D_CH_PRESSURE, S3D_CH_ROT_VX, S3D_CH_ROT_VY, S3D_CH_ROT_VZ,
+    S3D_CH_MXX, S3D_CH_MXY, S3D_CH_MXZ, S3D_CH_MYY, S3D_CH_MYZ, S3D_CH_MZZ,
+    S3D_CH_FX, S3D_CH_FY, S3D_CH_FZ
 };
+/* Channels one receiver may record (the reference's max_attributes is 30,
+ * src/examples/dummy_parameters.f90:39). */
+#define S3D_MAX_CHANNELS 32
 
 /* Wavelets (reference src/misc/source_signature_mod.f90:50-147). */
 enum { S3D_RICKER = 0, S3D_GAUSS, S3D_SIN3, S3D_BRUNE, S3D_DELTA };
@@ -161,7 +170,8 @@ int s3d_wavelet(int wavelet, double onset, double width, double dt, int nt,
  * yy, yz, zz) at step it. All-zero channels are dropped, and a later source
  * overwrites an earlier one's component in the same element, like the
  * reference's assignment in add_m** (src/config/source.f90:274-327,
- * src/config/channel_mod.f90:859-999). */
+ * src/config/channel_mod.f90:859-999). Sources are refused on a member of a
+ * batch (destroy the batch first). */
 int s3d_add_source_mt(s3d_ctx *ctx, const int *exyz, const double *vm,
                       const double *stf6);
 /* Single-force point source: stf3[c*nt + it-1] for x, y, z
@@ -169,14 +179,17 @@ int s3d_add_source_mt(s3d_ctx *ctx, const int *exyz, const double *vm,
 int s3d_add_source_sf(s3d_ctx *ctx, const int *exyz, const double *vm,
                       const double *stf3);
 
-/* Receiver with nch channel codes (S3D_CH_*); returns its index in *id.
+/* Receiver with nch (1 .. S3D_MAX_CHANNELS) channel codes (S3D_CH_*);
+ * returns its index in *id. Refused on a member of a batch (destroy the
+ * batch first).
  * Replaces parse_receiver_file's channel setup (src/config/receiver.f90:435-492). */
 int s3d_add_receiver(s3d_ctx *ctx, const int *exyz, const double *vm,
                      int nch, const int *channels, int *id);
 
 /* Advance nsteps iterations of the forward loop body
  * (reference src/ses3d-nt.F90:240-270): ses3d_evolution, every source's
- * add(it) and every receiver's record(it). */
+ * add(it) and every receiver's record(it). Refused on a member of a batch,
+ * which s3d_batch_step advances. */
 int s3d_step(s3d_ctx *ctx, int nsteps);
 
 /* Iterations completed so far. */
@@ -204,6 +217,8 @@ int s3d_set_field(s3d_ctx *ctx, int field, const double *in6d);
  * call also allocates a staging buffer of three FP64 fields, both kept until
  * s3d_destroy (4 x 8 B per element-local node). The caller
  * picks the steps and snapshots (the reference's saving_vector, :159-174).
+ * increment is the reference's signed timestep_increment (nonzero; the
+ * reference's default -1 integrates with -dt).
  * The reference accumulates grad_cp, grad_cs, grad_csh and grad_csv from
  * forward strain rates that it never loads (:184-191); S3D_GRAD_CS,
  * S3D_GRAD_CP, S3D_GRAD_CSH and S3D_GRAD_CSV read as zero. */

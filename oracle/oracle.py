@@ -25,7 +25,9 @@ SX, SY, SZ = 23, 24, 25
 MXX = 32
 RHOINV, LAMBDA, MU, A, B, C_, Q = range(7)
 CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6,
-      "rot_vx": 7, "rot_vy": 8, "rot_vz": 9}
+      "rot_vx": 7, "rot_vy": 8, "rot_vz": 9,
+      "mxx": 10, "mxy": 11, "mxz": 12, "myy": 13, "myz": 14, "mzz": 15,
+      "fx": 16, "fy": 17, "fz": 18}
 WAVELET = {"RICKER": 0, "GAUSS": 1, "SIN**3": 2, "BRUNE": 3, "DELTA": 4}
 
 

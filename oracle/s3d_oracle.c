@@ -923,7 +923,13 @@ static void receivers_record(ora_ctx *c, int it)
                                          c->e[EYX][n] / c->sin_theta[n]) / c->r[n];
                             f = vmv * el;
                         } break;
-                        default: f = 0.0;
+                        default:
+                            /* mxx .. mzz, fx, fy, fz: the glut or force that
+                               sources_add(it) just assigned to this element */
+                            if (R->ch[q] >= ORA_CH_MXX && R->ch[q] <= ORA_CH_FZ)
+                                f = vmv * c->src[R->ch[q] - ORA_CH_MXX][n];
+                            else
+                                f = 0.0;
                         }
                         acc = acc + f;
                     }

@@ -48,7 +48,10 @@ enum { ORA_RHOINV = 0, ORA_LAMBDA, ORA_MU, ORA_A, ORA_B, ORA_C, ORA_Q };
 /* Receiver channel codes (reference src/config/channel_mod.f90:270-336). */
 enum {
     ORA_CH_VX = 0, ORA_CH_VY, ORA_CH_VZ, ORA_CH_N, ORA_CH_E, ORA_CH_Z,
-    ORA_CH_PRESSURE, ORA_CH_ROT_VX, ORA_CH_ROT_VY, ORA_CH_ROT_VZ
+    ORA_CH_PRESSURE, ORA_CH_ROT_VX, ORA_CH_ROT_VY, ORA_CH_ROT_VZ,
+    /* source echoes: record_m** / record_f* (src/config/channel_mod.f90:871-1085) */
+    ORA_CH_MXX, ORA_CH_MXY, ORA_CH_MXZ, ORA_CH_MYY, ORA_CH_MYZ, ORA_CH_MZZ,
+    ORA_CH_FX, ORA_CH_FY, ORA_CH_FZ
 };
 
 typedef struct ora_ctx ora_ctx;

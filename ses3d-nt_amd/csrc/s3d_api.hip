@@ -16,6 +16,7 @@
 
 using namespace s3d;
 
+
 namespace {
 
 thread_local std::string g_err;
@@ -181,11 +182,12 @@ struct s3d_ctx {
     bool src_dirty = false;
     int *d_slot = nullptr, *d_chan = nullptr;
     double *d_coef = nullptr, *d_stf = nullptr;
-    // receivers
-    std::vector<int> rec_e, rec_nch, rec_ch;
+    // receivers: receiver r records rec_nch[r] channels rec_ch[rec_off[r] + q],
+    // traced at rows rec_off[r] + q of the trace buffer [channels][nt]
+    std::vector<int> rec_e, rec_nch, rec_off, rec_ch;
     std::vector<double> rec_vm;
     bool rec_dirty = false;
-    int *d_rec_e = nullptr, *d_rec_nch = nullptr, *d_rec_ch = nullptr;
+    int *d_rec_e = nullptr, *d_rec_nch = nullptr, *d_rec_off = nullptr, *d_rec_ch = nullptr;
     double *d_rec_vm = nullptr, *d_traces = nullptr;
     // curl receiver channels: slot per receiver (-1 none), element per slot,
     // strain-rate parts [nrot][10][NP]
@@ -374,6 +376,11 @@ RecArgs<R> rec_args(s3d_ctx *c)
     a.sig = (decltype(a.sig))((const R *)c->sig);
     a.ns = c->ns;
     a.rec_e = (decltype(a.rec_e))(c->d_rec_e); a.rec_nch = (decltype(a.rec_nch))(c->d_rec_nch); a.rec_ch = (decltype(a.rec_ch))(c->d_rec_ch);
+    a.rec_off = (decltype(a.rec_off))(c->d_rec_off);
+    a.src_slot = (decltype(a.src_slot))(c->d_slot);
+    a.src_chan = (decltype(a.src_chan))(c->d_chan);
+    a.src_coef = (decltype(a.src_coef))(c->d_coef);
+    a.stf = (decltype(a.stf))(c->d_stf);
     a.rec_vm = (decltype(a.rec_vm))(c->d_rec_vm);
     a.rec_rot = (decltype(a.rec_rot))(c->nrot > 0 ? c->d_rec_rot : nullptr);
     a.rot_buf = (decltype(a.rot_buf))(c->d_rot_buf);
@@ -781,23 +788,30 @@ int upload_receivers(s3d_ctx *c)
     if (!c->rec_dirty) return 0;
     drop_graph(c);
     const int n = (int)c->rec_e.size();
-    const size_t tr = (size_t)n * 8 * c->g.nt;
+    const int nchan = (int)c->rec_ch.size();   // channels of all receivers
+    const size_t tr = (size_t)nchan * c->g.nt;
     double *traces = nullptr;
     HIPCHK(hipMalloc(&traces, sizeof(double) * std::max<size_t>(1, tr)));
     HIPCHK(hipMemsetAsync(traces, 0, sizeof(double) * std::max<size_t>(1, tr), c->stream));
-    if (c->d_traces && c->nrec_dev > 0)
-        HIPCHK(hipMemcpyAsync(traces, c->d_traces, sizeof(double) * (size_t)c->nrec_dev * 8 * c->g.nt,
+    // receivers are only appended: the rows recorded so far come first
+    if (c->d_traces && c->nrec_dev > 0) {
+        const size_t old_rows = (size_t)c->rec_off[c->nrec_dev - 1] + c->rec_nch[c->nrec_dev - 1];
+        HIPCHK(hipMemcpyAsync(traces, c->d_traces, sizeof(double) * old_rows * c->g.nt,
                               hipMemcpyDeviceToDevice, c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
-    dfree(c->d_traces); dfree(c->d_rec_e); dfree(c->d_rec_nch); dfree(c->d_rec_ch); dfree(c->d_rec_vm);
+    dfree(c->d_traces); dfree(c->d_rec_e); dfree(c->d_rec_nch); dfree(c->d_rec_off); dfree(c->d_rec_ch);
+    dfree(c->d_rec_vm);
     c->d_traces = traces;
     HIPCHK(hipMalloc(&c->d_rec_e, sizeof(int) * n));
     HIPCHK(hipMalloc(&c->d_rec_nch, sizeof(int) * n));
-    HIPCHK(hipMalloc(&c->d_rec_ch, sizeof(int) * 8 * n));
+    HIPCHK(hipMalloc(&c->d_rec_off, sizeof(int) * n));
+    HIPCHK(hipMalloc(&c->d_rec_ch, sizeof(int) * std::max(1, nchan)));
     HIPCHK(hipMalloc(&c->d_rec_vm, sizeof(double) * c->NP * (size_t)n));
     HIPCHK(hipMemcpy(c->d_rec_e, c->rec_e.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_rec_nch, c->rec_nch.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->d_rec_ch, c->rec_ch.data(), sizeof(int) * 8 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rec_off, c->rec_off.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->d_rec_ch, c->rec_ch.data(), sizeof(int) * nchan, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->d_rec_vm, c->rec_vm.data(), sizeof(double) * c->NP * (size_t)n, hipMemcpyHostToDevice));
     // curl receiver channels
     dfree(c->d_rec_rot); dfree(c->d_rot_e); dfree(c->d_rot_buf);
@@ -805,11 +819,13 @@ int upload_receivers(s3d_ctx *c)
     c->d_rot_buf = nullptr;
     std::vector<int> slot(n, -1), rot_e;
     for (int r = 0; r < n; ++r)
-        for (int q = 0; q < c->rec_nch[r]; ++q)
-            if (c->rec_ch[8 * r + q] >= S3D_CH_ROT_VX && slot[r] < 0) {
+        for (int q = 0; q < c->rec_nch[r]; ++q) {
+            const int ch = c->rec_ch[c->rec_off[r] + q];
+            if (ch >= S3D_CH_ROT_VX && ch <= S3D_CH_ROT_VZ && slot[r] < 0) {
                 slot[r] = (int)rot_e.size();
                 rot_e.push_back(c->rec_e[r]);
             }
+        }
     c->nrot = (int)rot_e.size();
     if (c->nrot > 0) {
         HIPCHK(hipMalloc(&c->d_rec_rot, sizeof(int) * n));
@@ -1190,7 +1206,7 @@ int s3d_destroy(s3d_ctx *c)
                     c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho, c->grad_stage,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
-                    c->d_rec_ch, c->d_rec_vm, c->d_traces, c->d_rec_rot, c->d_rot_e, c->d_rot_buf};
+                    c->d_rec_off, c->d_rec_ch, c->d_rec_vm, c->d_traces, c->d_rec_rot, c->d_rot_e, c->d_rot_buf};
     for (void *b : bufs) dfree(b);
     for (int k = 0; k < 3; ++k)
         for (hipEvent_t e : c->ev[k]) (void)hipEventDestroy(e);
@@ -1515,10 +1531,21 @@ int s3d_locate(s3d_ctx *c, double lat, double lon, double depth, int *exyz, doub
     return 0;
 }
 
+// A batch caches its members' device pointers (sources, receivers, traces)
+// in its launch arguments and graph; adding a source or a receiver to a
+// member reallocates those buffers, so members are closed to both, and to
+// stepping on their own, until the batch is destroyed.
+static int refuse_batch_member(const s3d_ctx *c, const char *what)
+{
+    return fail(std::string(what) + ": the simulation is a member of " + std::to_string(c->batches) +
+                " batch(es); destroy the batch first");
+}
+
 static int add_source(s3d_ctx *c, const int *exyz, const double *vm, const double *stf, int ncomp, int first)
 {
     if (!c || !exyz || !vm || !stf) return fail("null argument");
     if (!c->inited) return fail("s3d_init first");
+    if (c->batches > 0) return refuse_batch_member(c, "add_source");
     int ex = exyz[0], ey = exyz[1], ez = exyz[2];
     if (ex < 0 || ex >= c->g.nx || ey < 0 || ey >= c->g.ny || ez < 0 || ez >= c->g.nz)
         return fail("source element outside the mesh");
@@ -1579,19 +1606,19 @@ int s3d_add_receiver(s3d_ctx *c, const int *exyz, const double *vm, int nch, con
 {
     if (!c || !exyz || !vm || !channels) return fail("null argument");
     if (!c->inited) return fail("s3d_init first");
-    if (nch < 1 || nch > 8) return fail("a receiver records 1..8 channels");
+    if (nch < 1 || nch > S3D_MAX_CHANNELS)
+        return fail("a receiver records 1.." + std::to_string(S3D_MAX_CHANNELS) + " channels");
     int ex = exyz[0], ey = exyz[1], ez = exyz[2];
     if (ex < 0 || ex >= c->g.nx || ey < 0 || ey >= c->g.ny || ez < 0 || ez >= c->g.nz)
         return fail("receiver element outside the mesh");
     for (int q = 0; q < nch; ++q)
-        if (channels[q] < S3D_CH_VX || channels[q] > S3D_CH_ROT_VZ)
+        if (channels[q] < S3D_CH_VX || channels[q] > S3D_CH_FZ)
             return fail("unsupported receiver channel " + std::to_string(channels[q]));
-    for (int q = 0; q < nch; ++q)
-        if (channels[q] >= S3D_CH_ROT_VX && c->batches > 0)
-            return fail("curl receiver channels are not supported in a batch");
+    if (c->batches > 0) return refuse_batch_member(c, "add_receiver");
     c->rec_e.push_back((int)(ex + (long)c->g.nx * (ey + (long)c->g.ny * ez)));
     c->rec_nch.push_back(nch);
-    for (int q = 0; q < 8; ++q) c->rec_ch.push_back(q < nch ? channels[q] : -1);
+    c->rec_off.push_back((int)c->rec_ch.size());
+    c->rec_ch.insert(c->rec_ch.end(), channels, channels + nch);
     c->rec_vm.insert(c->rec_vm.end(), vm, vm + c->NP);
     c->rec_dirty = true;
     if (id) *id = (int)c->rec_e.size() - 1;
@@ -1603,6 +1630,7 @@ int s3d_step(s3d_ctx *c, int nsteps)
     if (!c) return fail("null argument");
     if (!c->inited) return fail("s3d_init first");
     if (nsteps < 0) return fail("negative step count");
+    if (c->batches > 0) return refuse_batch_member(c, "step");
     if (c->it + nsteps > c->g.nt) return fail("stepping past nt");
     HIPCHK(hipSetDevice(c->dev));
     if (upload_sources(c) || upload_receivers(c)) return 1;
@@ -1643,7 +1671,7 @@ int s3d_get_trace(s3d_ctx *c, int rec, int ch, double *out)
     if (ch < 0 || ch >= c->rec_nch[rec]) return fail("no such channel");
     HIPCHK(hipSetDevice(c->dev));
     if (upload_receivers(c)) return 1;
-    HIPCHK(hipMemcpyAsync(out, c->d_traces + ((size_t)rec * 8 + ch) * c->g.nt, sizeof(double) * c->g.nt,
+    HIPCHK(hipMemcpyAsync(out, c->d_traces + ((size_t)c->rec_off[rec] + ch) * c->g.nt, sizeof(double) * c->g.nt,
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
@@ -1794,7 +1822,7 @@ int s3d_grad_add(s3d_ctx *c, const double *vx_fw, const double *vy_fw, const dou
 {
     if (!c || !vx_fw || !vy_fw || !vz_fw) return fail("null argument");
     if (!c->inited) return fail("s3d_init first");
-    if (increment < 1) return fail("grad_add: increment must be positive");
+    if (increment == 0) return fail("grad_add: increment must be nonzero");
     HIPCHK(hipSetDevice(c->dev));
     const long nn = c->nn;
     if (!c->grad_rho && dalloc(c, &c->grad_rho, sizeof(double) * nn)) return 1;
@@ -1884,7 +1912,8 @@ int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out)
         if (sims[q]->dev != sims[0]->dev) return fail("batch members must share one device");
         if (sims[q]->o.bf16_static) return fail("BF16 static fields are not supported in a batch");
         for (int ch : sims[q]->rec_ch)
-            if (ch >= S3D_CH_ROT_VX) return fail("curl receiver channels are not supported in a batch");
+            if (ch >= S3D_CH_ROT_VX && ch <= S3D_CH_ROT_VZ)
+                return fail("curl receiver channels are not supported in a batch");
         for (int r = 0; r < q; ++r)
             if (sims[r] == sims[q]) return fail("a simulation appears twice in the batch");
     }

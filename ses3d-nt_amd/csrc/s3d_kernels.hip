@@ -75,12 +75,18 @@ __device__ inline void kahan_add(R &s, R &c, R inc)
 // compiler's wait accounting never sees it: callers issue these loads before
 // any compiler-issued load (vector memory operations complete in issue
 // order, so the compiler's own waits stay exact) and read the LDS words
-// only after wait_vmem_zero.
+// only after wait_vmem_zero. The statement writes M0, so M0 is in its clobber
+// list: the compiler keeps nothing there across it. LLVM reserves M0 and warns
+// about the clobber; in the gfx950 code of this library the only
+// instructions that touch M0 are these statements' own.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
 __device__ __forceinline__ void lds_dma_dword(const void *g, unsigned lds)
 {
     asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dword %0, off"
-                 :: "v"(g), "s"(__builtin_amdgcn_readfirstlane(lds)) : "memory");
+                 :: "v"(g), "s"(__builtin_amdgcn_readfirstlane(lds)) : "memory", "m0");
 }
+#pragma clang diagnostic pop
 
 // s_waitcnt vmcnt(0), returning a zero that the reads of LDS words written by
 // lds_dma_dword must add to their index: the data dependency orders those
@@ -633,10 +639,14 @@ struct RecArgs {
     gptr<const R> vx, vy, vz, lx, ly, lz, sig;
     gptr<const int> rec_rot;      // curl slot per receiver, -1 none (null: no curl receivers)
     gptr<const double> rot_buf;   // per curl slot [10][NP], see curl_strain_kernel
-    gptr<const int> rec_e, rec_nch, rec_ch;
+    gptr<const int> rec_e, rec_nch, rec_off, rec_ch;  // channels of receiver r: rec_ch[rec_off[r] + q]
     gptr<const double> rec_vm;
-    gptr<double> traces;
+    gptr<double> traces;          // [sum of channels][nt], receiver r's at rec_off[r]
     gptr<const int> it_dev;
+    // sources, for the echo channels: slot per element (null: no sources),
+    // channel row per slot and component, coefficients, source-time table
+    gptr<const int> src_slot, src_chan;
+    gptr<const double> src_coef, stf;
 };
 
 template <int P, typename R>
@@ -649,7 +659,20 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
     const int lane = threadIdx.x;
     const int e = a.rec_e[r];
     const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
+    // source-echo channels: add(it) of this step wrote, per component c with
+    // a source in this element, src_c = -stf(it) * coef_c (an assignment: the
+    // last source of the element wins, as in add_m** / add_f*,
+    // src/config/channel_mod.f90:859-1095); record_m** / record_f* sum vm *
+    // src_c (:871-1085)
+    const int slot = a.src_slot ? a.src_slot[e] : -1;
+    double sd[9];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        const int ch = slot >= 0 ? a.src_chan[slot * 9 + c] : -1;
+        sd[c] = ch >= 0 ? a.stf[(long)ch * a.nt + (it - 1)] : 0.0;
+    }
     double acc[7] = {0, 0, 0, 0, 0, 0, 0};   // vx, vy, vz, pressure, rot_vx, rot_vy, rot_vz
+    double accs[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // mxx .. mzz, fx, fy, fz
     const int rs = a.rec_rot ? a.rec_rot[r] : -1;
     for (int n = lane; n < NP; n += 64) {
         const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
@@ -674,14 +697,24 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
             acc[5] += w * (-ezx + (vxn - exz) / rr);
             acc[6] += w * ((exy + vyn * ct - eyx / st) / rr);
         }
+        if (slot >= 0) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c)
+                if (sd[c] != 0.0)
+                    accs[c] += w * -(sd[c] * a.src_coef[((long)slot * 9 + c) * NP + n]);
+        }
     }
 #pragma unroll
     for (int c = 0; c < 7; ++c)
         for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_down(acc[c], off, 64);
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        for (int off = 32; off > 0; off >>= 1) accs[c] += __shfl_down(accs[c], off, 64);
     if (lane == 0) {
         const int nch = a.rec_nch[r];
+        const int off = a.rec_off[r];
         for (int q = 0; q < nch; ++q) {
-            const int ch = a.rec_ch[r * 8 + q];
+            const int ch = a.rec_ch[off + q];
             double val = 0.0;
             switch (ch) {
             case 0: val = acc[0]; break;          // vx
@@ -694,9 +727,11 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
             case 7: val = acc[4]; break;          // rot_vx
             case 8: val = acc[5]; break;          // rot_vy
             case 9: val = acc[6]; break;          // rot_vz
-            default: break;
+            default:                              // mxx .. mzz, fx, fy, fz
+                if (ch >= 10 && ch < 19) val = accs[ch - 10];
+                break;
             }
-            a.traces[((long)r * 8 + q) * a.nt + (it - 1)] = val;
+            a.traces[((long)off + q) * a.nt + (it - 1)] = val;
         }
     }
 }

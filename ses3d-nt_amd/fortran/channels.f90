@@ -9,7 +9,8 @@
 MODULE channels_mod
     USE, INTRINSIC :: ISO_C_BINDING
     USE ses3d_hip_c
-    USE configuration_mod, ONLY : rk, configuration, source_entry, receiver_entry, abort
+    USE configuration_mod, ONLY : rk, configuration, source_entry, receiver_entry, abort, warn, &
+                                  lower, max_attributes
     USE sac_io_mod, ONLY : write_sac, read_sac, sac_file_name
     IMPLICIT NONE
     PRIVATE
@@ -24,14 +25,14 @@ MODULE channels_mod
         INTEGER :: id = -1
         TYPE(receiver_entry) :: info
         INTEGER :: nch = 0
-        INTEGER(c_int) :: codes(8)
-        CHARACTER(LEN=8) :: names(8)
+        INTEGER(c_int) :: codes(max_attributes)
+        CHARACTER(LEN=8) :: names(max_attributes)
     CONTAINS
         PROCEDURE :: record => receiver_record
         PROCEDURE :: write => receiver_write
     END TYPE
 
-    PUBLIC :: parse_source_file, parse_receiver_file
+    PUBLIC :: parse_source_file, parse_receiver_file, count_receiver_channels
 
 CONTAINS
 
@@ -175,12 +176,71 @@ CONTAINS
         sources = tmp(1:n)
     END SUBROUTINE
 
+    !> Channel code of a lower-case receiver attribute (the names of
+    !! construct_channel_container_receiver, src/config/channel_mod.f90:270-336,
+    !! and pressure); -1 when unknown.
+    INTEGER(c_int) FUNCTION channel_code(name) RESULT(code)
+        CHARACTER(LEN=*), INTENT(IN) :: name
+        SELECT CASE (TRIM(name))
+        CASE ('vx'); code = S3D_CH_VX
+        CASE ('vy'); code = S3D_CH_VY
+        CASE ('vz'); code = S3D_CH_VZ
+        CASE ('n');  code = S3D_CH_N
+        CASE ('e');  code = S3D_CH_E
+        CASE ('z');  code = S3D_CH_Z
+        CASE ('pressure'); code = S3D_CH_PRESSURE
+        CASE ('rot_vx'); code = S3D_CH_ROT_VX
+        CASE ('rot_vy'); code = S3D_CH_ROT_VY
+        CASE ('rot_vz'); code = S3D_CH_ROT_VZ
+        CASE ('mxx'); code = S3D_CH_MXX
+        CASE ('mxy', 'myx'); code = S3D_CH_MXY
+        CASE ('mxz', 'mzx'); code = S3D_CH_MXZ
+        CASE ('myy'); code = S3D_CH_MYY
+        CASE ('myz', 'mzy'); code = S3D_CH_MYZ
+        CASE ('mzz'); code = S3D_CH_MZZ
+        CASE ('fx'); code = S3D_CH_FX
+        CASE ('fy'); code = S3D_CH_FY
+        CASE ('fz'); code = S3D_CH_FZ
+        CASE DEFAULT; code = -1
+        END SELECT
+    END FUNCTION
+
+    !> Channels of a receiver entry as the reference builds them
+    !! (construct_receiver_cls, src/config/receiver.f90:379-392): empty
+    !! attributes are dropped, the others lower-cased; unknown ones are
+    !! reported in one warning and dropped.
+    SUBROUTINE receiver_channels(r)
+        TYPE(receiver_cls), INTENT(INOUT) :: r
+        INTEGER :: j
+        INTEGER(c_int) :: code
+        CHARACTER(LEN=8) :: a
+        CHARACTER(LEN=:), ALLOCATABLE :: unknown
+        r%nch = 0
+        unknown = ''
+        DO j = 1, SIZE(r%info%attributes)
+            a = lower(ADJUSTL(r%info%attributes(j)))
+            IF (LEN_TRIM(a) == 0) CYCLE
+            code = channel_code(a)
+            IF (code < 0) THEN
+                unknown = unknown // ' ' // TRIM(a)
+                CYCLE
+            END IF
+            r%nch = r%nch + 1
+            r%names(r%nch) = a
+            r%codes(r%nch) = code
+        END DO
+        IF (LEN(unknown) > 0) &
+            CALL warn('Unknown attributes' // unknown // " in receiver '" // &
+                      TRIM(r%info%network) // '.' // TRIM(r%info%station) // '.' // &
+                      TRIM(r%info%location) // "'.")
+    END SUBROUTINE
+
     !> Locate every receiver and register its channels on the device
     !! (src/config/receiver.f90:435-492).
     SUBROUTINE parse_receiver_file(ctx, receivers)
         TYPE(c_ptr), INTENT(IN) :: ctx
         TYPE(receiver_cls), ALLOCATABLE, INTENT(OUT) :: receivers(:)
-        INTEGER :: i, j, n
+        INTEGER :: i, n
         INTEGER(c_int) :: exyz(3), id
         REAL(c_double), ALLOCATABLE :: vm(:)
         TYPE(receiver_cls), ALLOCATABLE :: tmp(:)
@@ -190,28 +250,7 @@ CONTAINS
         n = 0
         DO i = 1, SIZE(configuration%receivers)
             r%info = configuration%receivers(i)
-            r%nch = 0
-            DO j = 1, SIZE(r%info%attributes)
-                IF (LEN_TRIM(r%info%attributes(j)) == 0) CYCLE
-                IF (r%nch == 8) CALL abort('At most 8 channels per receiver')
-                r%nch = r%nch + 1
-                r%names(r%nch) = r%info%attributes(j)
-                SELECT CASE (TRIM(r%info%attributes(j)))
-                CASE ('vx'); r%codes(r%nch) = S3D_CH_VX
-                CASE ('vy'); r%codes(r%nch) = S3D_CH_VY
-                CASE ('vz'); r%codes(r%nch) = S3D_CH_VZ
-                CASE ('n');  r%codes(r%nch) = S3D_CH_N
-                CASE ('e');  r%codes(r%nch) = S3D_CH_E
-                CASE ('z');  r%codes(r%nch) = S3D_CH_Z
-                CASE ('pressure'); r%codes(r%nch) = S3D_CH_PRESSURE
-                CASE ('rot_vx'); r%codes(r%nch) = S3D_CH_ROT_VX
-                CASE ('rot_vy'); r%codes(r%nch) = S3D_CH_ROT_VY
-                CASE ('rot_vz'); r%codes(r%nch) = S3D_CH_ROT_VZ
-                CASE DEFAULT
-                    CALL abort('Receiver channel not supported on the GPU path: ' // &
-                               TRIM(r%info%attributes(j)))
-                END SELECT
-            END DO
+            CALL receiver_channels(r)
             IF (r%nch == 0) CYCLE
             IF (s3d_locate(ctx, r%info%lat, r%info%lon, r%info%depth, exyz, vm) /= 0) THEN
                 WRITE(*, '("WARNING: receiver ", A, " not in computational domain!")') &
@@ -225,6 +264,23 @@ CONTAINS
             tmp(n) = r
         END DO
         receivers = tmp(1:n)
+    END SUBROUTINE
+
+    !> Channels of every receiver of the configuration, without a device: the
+    !! parse-only check of ses3d_nt (-c). Returns the receiver and channel counts.
+    SUBROUTINE count_receiver_channels(nrec, nch)
+        INTEGER, INTENT(OUT) :: nrec, nch
+        TYPE(receiver_cls) :: r
+        INTEGER :: i
+        nrec = 0
+        nch = 0
+        DO i = 1, SIZE(configuration%receivers)
+            r%info = configuration%receivers(i)
+            CALL receiver_channels(r)
+            IF (r%nch == 0) CYCLE
+            nrec = nrec + 1
+            nch = nch + r%nch
+        END DO
     END SUBROUTINE
 
     SUBROUTINE source_add(self, it)

@@ -1,7 +1,9 @@
 !> Sensitivity densities of the adjoint workflow (reference grad_mod,
 !! src/SOURCE/ses3d_grad.F90): at adjoint step it, if the forward step
 !! s = nt + 1 - it is a multiple of the &grad timestep_increment (the
-!! reference's saving_vector, :159-174), the forward velocity snapshots
+!! reference's saving_vector, :159-174; a negative increment, such as the
+!! default -1, selects the multiples of its magnitude and makes the factor
+!! increment*dt negative, as there), the forward velocity snapshots
 !! ./vx_<s>.vol, ./vy_<s>.vol, ./vz_<s>.vol (written by a forward run's
 !! &output_raw, global 6-D layout, float32 or float64) are read and
 !! grad_rho += increment*dt*(v_fw . v)/J is accumulated on the device

@@ -39,7 +39,10 @@ MODULE ses3d_hip_c
         S3D_GRAD_CP = 22, S3D_GRAD_CSH = 23, S3D_GRAD_CSV = 24
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_CH_VX = 0, S3D_CH_VY = 1, &
         S3D_CH_VZ = 2, S3D_CH_N = 3, S3D_CH_E = 4, S3D_CH_Z = 5, &
-        S3D_CH_PRESSURE = 6, S3D_CH_ROT_VX = 7, S3D_CH_ROT_VY = 8, S3D_CH_ROT_VZ = 9
+        S3D_CH_PRESSURE = 6, S3D_CH_ROT_VX = 7, S3D_CH_ROT_VY = 8, S3D_CH_ROT_VZ = 9, &
+        S3D_CH_MXX = 10, S3D_CH_MXY = 11, S3D_CH_MXZ = 12, S3D_CH_MYY = 13, &
+        S3D_CH_MYZ = 14, S3D_CH_MZZ = 15, S3D_CH_FX = 16, S3D_CH_FY = 17, S3D_CH_FZ = 18
+    INTEGER(c_int), PARAMETER, PUBLIC :: S3D_MAX_CHANNELS = 32
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_RICKER = 0, S3D_GAUSS = 1, &
         S3D_SIN3 = 2, S3D_BRUNE = 3, S3D_DELTA = 4
 

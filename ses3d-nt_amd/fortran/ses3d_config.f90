@@ -70,13 +70,17 @@ MODULE configuration_mod
         TYPE(source_entry), ALLOCATABLE :: sources(:)
         TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
         TYPE(output_entry), ALLOCATABLE :: outputs(:)
+        ! &output_netcdf groups: parsed and checked; the build has no netCDF
+        ! library, so their steps only warn (as the reference built without
+        ! -DNetCDF does, src/output_netcdf.F90:210-217)
+        TYPE(output_entry), ALLOCATABLE :: netcdf(:)
         ! &grad (workflow 'adjoint'); prefix './'
         TYPE(output_entry) :: grad
     END TYPE
 
     TYPE(configuration_typ), PUBLIC :: configuration
 
-    PUBLIC :: parse_conf_file, abort
+    PUBLIC :: parse_conf_file, abort, warn, lower
 
 CONTAINS
 
@@ -84,6 +88,12 @@ CONTAINS
         CHARACTER(LEN=*), INTENT(IN) :: msg
         WRITE(ERROR_UNIT, '("ERROR: ", A)') msg
         ERROR STOP 1
+    END SUBROUTINE
+
+    !> Warning on the screen (reference WARN, src/misc/error_mod.F90:79-102).
+    SUBROUTINE warn(msg)
+        CHARACTER(LEN=*), INTENT(IN) :: msg
+        WRITE(*, '("WARNING: ", A)') msg
     END SUBROUTINE
 
     INTEGER FUNCTION open_conf(file) RESULT(lun)
@@ -220,10 +230,9 @@ CONTAINS
 
         CALL parse_sources(lun, cf)
         CALL parse_receivers(lun, cf)
-        CALL parse_outputs(lun, cf)
+        CALL parse_outputs(lun, cf, .FALSE.)
+        CALL parse_outputs(lun, cf, .TRUE.)
         IF (TRIM(cf%workflow) == 'adjoint') CALL parse_grad(lun, cf)
-        IF (count_groups(lun, 'output_netcdf') > 0) &
-            CALL abort('&output_netcdf: NetCDF output is not provided (use &output_raw)')
         CLOSE(lun)
         configuration = cf
     END SUBROUTINE
@@ -311,39 +320,65 @@ CONTAINS
         END DO
     END SUBROUTINE
 
-    !> &output_raw groups (src/parser/parser.f90:592-638)
-    SUBROUTINE parse_outputs(lun, cf)
+    !> &output_raw groups (netcdf false; src/parser/parser.f90:557-600) or
+    !! &output_netcdf groups (netcdf true; :502-555), with the defaults and
+    !! checks of the snapshot constructor (src/volume_snapshot.f90:70-105):
+    !! timestep_end defaults to nt, timestep_start to 1, the prefix to './';
+    !! attributes are lower-cased (get_field, :209).
+    SUBROUTINE parse_outputs(lun, cf, netcdf)
         INTEGER, INTENT(IN) :: lun
         TYPE(configuration_typ), INTENT(INOUT) :: cf
+        LOGICAL, INTENT(IN) :: netcdf
+        INTEGER, PARAMETER :: unset = HUGE(1)
         INTEGER :: timestep_start, timestep_increment, timestep_end
         LOGICAL :: override
         CHARACTER(LEN=fnl) :: prefix
         CHARACTER(LEN=8) :: attributes(max_attributes)
         NAMELIST /output_raw/ timestep_start, timestep_increment, timestep_end, &
             attributes, override, prefix
-        INTEGER :: n, i, ios
+        NAMELIST /output_netcdf/ timestep_start, timestep_increment, timestep_end, &
+            attributes, override, prefix
+        TYPE(output_entry), ALLOCATABLE :: out(:)
+        INTEGER :: n, i, j, ios
 
-        n = count_groups(lun, 'output_raw')
-        ALLOCATE(cf%outputs(n))
+        n = count_groups(lun, MERGE('output_netcdf', 'output_raw   ', netcdf))
+        ALLOCATE(out(n))
         REWIND(lun)
         DO i = 1, n
-            timestep_start = -1; timestep_increment = -1; timestep_end = -1
-            override = .FALSE.; prefix = './'; attributes = ''
-            READ(lun, NML=output_raw, IOSTAT=ios)
-            IF (ios /= 0) CALL abort('Error while parsing &output_raw-group!')
-            IF (timestep_start < 1 .OR. timestep_increment < 1 .OR. timestep_end < timestep_start) &
-                CALL abort('Non-conforming time steps in &output_raw-group!')
-            cf%outputs(i) = output_entry(timestep_start, timestep_increment, timestep_end, &
-                                         attributes, override, prefix)
+            timestep_start = unset; timestep_increment = unset; timestep_end = unset
+            override = .FALSE.; prefix = ''; attributes = ''
+            IF (netcdf) THEN
+                READ(lun, NML=output_netcdf, IOSTAT=ios)
+                IF (ios /= 0) CALL abort('Error while parsing &output_netcdf-group!')
+            ELSE
+                READ(lun, NML=output_raw, IOSTAT=ios)
+                IF (ios /= 0) CALL abort('Error while parsing &output_raw-group!')
+            END IF
+            IF (timestep_end == unset) timestep_end = cf%nt
+            IF (timestep_start == unset) timestep_start = 1
+            IF (LEN_TRIM(prefix) == 0) prefix = './'
+            IF (timestep_start < 1) CALL abort('timestep_start out of range')
+            IF (timestep_increment < 1 .OR. timestep_increment > cf%nt) &
+                CALL abort('timestep_increment out of range')
+            IF (timestep_end < timestep_start) CALL abort('timestep_end out of range')
+            DO j = 1, max_attributes
+                attributes(j) = lower(ADJUSTL(attributes(j)))
+            END DO
+            out(i) = output_entry(timestep_start, timestep_increment, timestep_end, &
+                                  attributes, override, prefix)
         END DO
+        IF (netcdf) THEN
+            cf%netcdf = out
+        ELSE
+            cf%outputs = out
+        END IF
     END SUBROUTINE
 
-    !> The &grad group of workflow 'adjoint' (src/parser/parser.f90:465-500,
+    !> The &grad group of workflow 'adjoint' (src/parser/parser.f90:462-497,
     !! src/SOURCE/ses3d_grad.F90:237-292): exactly one, with at least one
-    !! attribute. timestep_increment must be positive: it is the snapshot
-    !! spacing and the factor of the time integral (the reference would run
-    !! with its default -1, reading a snapshot every step and integrating
-    !! with -dt).
+    !! attribute. timestep_increment (default -1, as in the reference) is the
+    !! snapshot spacing |increment| and the signed factor increment*dt of the
+    !! time integral (ses3d_grad.F90:159-213); 0 is refused.
     SUBROUTINE parse_grad(lun, cf)
         INTEGER, INTENT(IN) :: lun
         TYPE(configuration_typ), INTENT(INOUT) :: cf
@@ -358,7 +393,7 @@ CONTAINS
         READ(lun, NML=grad, IOSTAT=ios)
         IF (ios /= 0) CALL abort('Error while parsing &grad-group!')
         IF (COUNT(attributes /= ' ') < 1) CALL abort('No gradients specified')
-        IF (timestep_increment < 1) CALL abort('Non-conforming time steps in &grad-group!')
+        IF (timestep_increment == 0) CALL abort('Non-conforming time steps in &grad-group!')
         cf%grad = output_entry(timestep_start, timestep_increment, timestep_end, &
                                attributes, .FALSE., './')
         REWIND(lun)

@@ -2,6 +2,8 @@
 !! workflows 'forward', 'adjoint' and 'model'.
 !!
 !!   ses3d_nt <config-file>
+!!   ses3d_nt -c <config-file>   parse and check the configuration only
+!!                               (no device, no model or source files read)
 !!
 !! forward: reads the reference's NAMELIST config, uploads the model, runs nt
 !! steps on the GPU and writes one SAC file per receiver channel.
@@ -15,21 +17,40 @@ PROGRAM ses3d_nt
     USE configuration_mod, ONLY : configuration, parse_conf_file, abort, rk
     USE evolution_mod, ONLY : ctx, init_temp, ses3d_evolution, v_ptp_loc, finalize_evolution, &
                               write_model_parameters
-    USE channels_mod, ONLY : source_cls, receiver_cls, parse_source_file, parse_receiver_file
-    USE snapshots_mod, ONLY : write_snapshots, check_snapshot_attributes
+    USE channels_mod, ONLY : source_cls, receiver_cls, parse_source_file, parse_receiver_file, &
+                             count_receiver_channels
+    USE snapshots_mod, ONLY : write_snapshots, check_snapshot_attributes, write_netcdf
     USE grad_mod, ONLY : ses3d_grad
     IMPLICIT NONE
     CHARACTER(LEN=512) :: conf_file_name
     TYPE(source_cls), ALLOCATABLE :: sources(:)
     TYPE(receiver_cls), ALLOCATABLE :: receivers(:)
-    INTEGER :: it, i, every
+    INTEGER :: it, i, every, nrec, nch
     REAL(rk) :: ptp(2)
+    CHARACTER(LEN=8) :: flag
+    LOGICAL :: check_only
     CHARACTER(LEN=*), PARAMETER :: fmt = '("Iteration ",I0,", ",I0,"%: ",G0," < v < ",G0)'
 
     IF (COMMAND_ARGUMENT_COUNT() < 1) CALL abort('Config-file missing!')
-    CALL GET_COMMAND_ARGUMENT(1, conf_file_name)
+    CALL GET_COMMAND_ARGUMENT(1, flag)
+    check_only = flag == '-c'
+    IF (check_only) THEN
+        IF (COMMAND_ARGUMENT_COUNT() < 2) CALL abort('Config-file missing!')
+        CALL GET_COMMAND_ARGUMENT(2, conf_file_name)
+    ELSE
+        CALL GET_COMMAND_ARGUMENT(1, conf_file_name)
+    END IF
     CALL parse_conf_file(TRIM(conf_file_name))
-    IF (TRIM(configuration%workflow) == 'model') THEN
+    IF (check_only) THEN
+        CALL check_snapshot_attributes()
+        CALL count_receiver_channels(nrec, nch)
+        WRITE(OUTPUT_UNIT, '("Configuration OK: workflow ''", A, "'', ", I0, "x", I0, "x", I0, &
+            &" elements, lpd ", I0, ", nt ", I0, ", ", I0, " sources, ", I0, " receivers with ", &
+            &I0, " channels, ", I0, " raw and ", I0, " netCDF outputs")') &
+            TRIM(configuration%workflow), configuration%nx, configuration%ny, configuration%nz, &
+            configuration%lpd, configuration%nt, SIZE(configuration%sources), nrec, nch, &
+            SIZE(configuration%outputs), SIZE(configuration%netcdf)
+    ELSE IF (TRIM(configuration%workflow) == 'model') THEN
         CALL write_model_parameters()
         WRITE(OUTPUT_UNIT, '(A, /, "Workflow ''model'' successfully completed! ", /, A)') &
             REPEAT('=', 50), REPEAT('=', 50)
@@ -53,6 +74,7 @@ PROGRAM ses3d_nt
                 CALL receivers(i)%record(it)
             END DO
             CALL write_snapshots(ctx, it)
+            CALL write_netcdf(it)
             IF (MOD(it, every) == 0 .OR. it == configuration%nt) THEN
                 ptp = v_ptp_loc()
                 WRITE(OUTPUT_UNIT, fmt) it, 100 * it / configuration%nt, ptp

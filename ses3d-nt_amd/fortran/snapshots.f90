@@ -13,11 +13,11 @@
 MODULE snapshots_mod
     USE, INTRINSIC :: ISO_C_BINDING
     USE ses3d_hip_c
-    USE configuration_mod, ONLY : configuration, output_entry, abort
+    USE configuration_mod, ONLY : configuration, output_entry, abort, warn
     USE evolution_mod, ONLY : model_param
     IMPLICIT NONE
     PRIVATE
-    PUBLIC :: write_snapshots, check_snapshot_attributes
+    PUBLIC :: write_snapshots, check_snapshot_attributes, write_netcdf
 
 CONTAINS
 
@@ -26,6 +26,18 @@ CONTAINS
         INTEGER, INTENT(IN) :: it
         due = it >= o%ts_start .AND. it <= o%ts_end .AND. MOD(it - o%ts_start, o%ts_increment) == 0
     END FUNCTION
+
+    !> &output_netcdf at step it: this build has no netCDF library, so a due
+    !! output only warns, as the reference built without -DNetCDF does
+    !! (write_netcdf, src/output_netcdf.F90:174-220).
+    SUBROUTINE write_netcdf(it)
+        INTEGER, INTENT(IN) :: it
+        INTEGER :: i
+        DO i = 1, SIZE(configuration%netcdf)
+            IF (due(configuration%netcdf(i), it)) &
+                CALL warn("Can't write netCDF file! Ses3d-NT is compiled without netCDF support.")
+        END DO
+    END SUBROUTINE
 
     SUBROUTINE check_snapshot_attributes()
         INTEGER :: i, j
