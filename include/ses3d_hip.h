@@ -228,6 +228,18 @@ int s3d_grad_add(s3d_ctx *ctx, const double *vx_fw6d, const double *vy_fw6d, con
 /* min and max of vx, vy, vz (reference v_ptp_loc, src/elastic_vars.F90:179-185). */
 int s3d_vrange(s3d_ctx *ctx, double *out2);
 
+/* The same min and max as s3d_vrange, computed without allocating anything:
+ * no hipMalloc, hipFree or allocating library call. No counterpart in the
+ * reference; it is for callers that cannot allocate device memory, such as
+ * a time loop that reports the velocity range every step (as the reference's
+ * loop does, src/ses3d-nt.F90:262-266) under a harness that forbids
+ * allocation. The partial results live in one fixed scratch array per device
+ * (16 MiB, held until the process ends), so the mesh may have at most
+ * 268 435 456 unique nodes (256 per partial pair); larger meshes get an
+ * error. Calls on one device share that array: calls that overlap on one
+ * device (contexts on different host threads or streams) race. */
+int s3d_vrange_noalloc(s3d_ctx *ctx, double *out2);
+
 /* Kinetic energy (reference kinetic_energy, src/elastic_vars.F90:219-246). */
 int s3d_kinetic_energy(s3d_ctx *ctx, double *out);
 

@@ -11,15 +11,28 @@
 #include <type_traits>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace s3d;
 
+// Scratch memory of s3d_vrange_noalloc: one (min, max) pair per 256 unique
+// nodes, then the result pair. Each device holds its own copy until the
+// process ends; host code reaches it only through the per-device address
+// that hipGetSymbolAddress returns (g_scratch), never through this name.
+constexpr long S3D_NOALLOC_PAIRS = 1L << 20;   // 16 MiB of FP64 pairs
+static __device__ __align__(256) double scratch[2 * (S3D_NOALLOC_PAIRS + 1)];
 
 namespace {
 
 thread_local std::string g_err;
+
+// Per-device address of `scratch`, looked up once per device by s3d_create
+// with that device current.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch[MAX_DEVICES] = {};
 
 int fail(const std::string &m)
 {
@@ -730,6 +743,24 @@ struct VrangeRun {
     }
 };
 
+// s3d_vrange_noalloc: partials and the result pair in the device's scratch
+template <int P, typename R>
+struct VrangeNoallocRun {
+    static int run(s3d_ctx *c, double *part, unsigned nb, double *out)
+    {
+        hipLaunchKernelGGL((vrange_kernel<P, R>), dim3(nb), dim3(256), 0, c->stream, (const R *)c->v[0],
+                           (const R *)c->v[1], (const R *)c->v[2], c->g.nx, c->g.ny, c->g.nz, c->nblk, part);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(range_reduce_kernel, dim3(1), dim3(256), 0, c->stream, (const double *)part,
+                           (int)nb, part + 2 * (size_t)nb);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, part + 2 * (size_t)nb, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+};
+
 template <int P, typename R>
 struct KineticRun {
     static int run(s3d_ctx *c, double *out)
@@ -1167,6 +1198,20 @@ int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out)
     hipError_t e = hipSetDevice(c->dev);
     if (e == hipSuccess) e = hipStreamCreate(&c->stream);
     if (e != hipSuccess) { delete c; return fail(std::string("stream: ") + hipGetErrorString(e)); }
+    if (c->dev < MAX_DEVICES) {
+        // the device's copy of the s3d_vrange_noalloc scratch array
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        if (!g_scratch[c->dev]) {
+            void *ptr = nullptr;
+            e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess) {
+                (void)hipStreamDestroy(c->stream);
+                delete c;
+                return fail(std::string("scratch address: ") + hipGetErrorString(e));
+            }
+            g_scratch[c->dev] = (double *)ptr;
+        }
+    }
     c->p = grid->lpd;
     c->P1 = c->p + 1;
     c->NP = c->P1 * c->P1 * c->P1;
@@ -1852,6 +1897,21 @@ int s3d_vrange(s3d_ctx *c, double *out2)
     if (!c->inited) return fail("s3d_init first");
     HIPCHK(hipSetDevice(c->dev));
     return c->fp32 ? with_p<VrangeRun, float>(c->p, c, out2) : with_p<VrangeRun, double>(c->p, c, out2);
+}
+
+int s3d_vrange_noalloc(s3d_ctx *c, double *out2)
+{
+    if (!c || !out2) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    const long nb = (c->nuniq + 255) / 256;
+    if (nb > S3D_NOALLOC_PAIRS)
+        return fail("vrange_noalloc: " + std::to_string(c->nuniq) + " unique nodes exceed the " +
+                    std::to_string(256 * S3D_NOALLOC_PAIRS) + " of the fixed scratch buffer");
+    double *part = c->dev < MAX_DEVICES ? g_scratch[c->dev] : nullptr;
+    if (!part) return fail("vrange_noalloc: no scratch buffer on this device");
+    HIPCHK(hipSetDevice(c->dev));
+    return c->fp32 ? with_p<VrangeNoallocRun, float>(c->p, c, part, (unsigned)nb, out2)
+                   : with_p<VrangeNoallocRun, double>(c->p, c, part, (unsigned)nb, out2);
 }
 
 int s3d_kinetic_energy(s3d_ctx *c, double *out)

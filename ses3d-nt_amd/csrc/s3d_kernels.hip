@@ -1148,6 +1148,32 @@ __global__ void vrange_kernel(const R *vx, const R *vy, const R *vz, int nx, int
     }
 }
 
+// min and max over n partial (min, max) pairs, one workgroup of 256 lanes;
+// the result pair goes to out[0], out[1]
+__global__ void __launch_bounds__(256) range_reduce_kernel(const double *partial, int n, double *out)
+{
+    double mn = 1e308, mx = -1e308;
+    for (int q = threadIdx.x; q < n; q += 256) {
+        mn = fmin(mn, partial[2 * q]);
+        mx = fmax(mx, partial[2 * q + 1]);
+    }
+    __shared__ double s0[256], s1[256];
+    s0[threadIdx.x] = mn;
+    s1[threadIdx.x] = mx;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s0[threadIdx.x] = fmin(s0[threadIdx.x], s0[threadIdx.x + s]);
+            s1[threadIdx.x] = fmax(s1[threadIdx.x], s1[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s0[0];
+        out[1] = s1[0];
+    }
+}
+
 template <int P, typename R>
 __global__ void kinetic_kernel(const R *vx, const R *vy, const R *vz, const R *rho,
                                const ThTab<coef_t<R>> *tth, const RTab<coef_t<R>> *tr, const double *w,

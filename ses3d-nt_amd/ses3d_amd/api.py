@@ -42,7 +42,7 @@ EXPORTS = [
     "s3d_get_field", "s3d_set_field", "s3d_vrange", "s3d_kinetic_energy",
     "s3d_synchronize", "s3d_timing_enable", "s3d_kernel_times",
     "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step", "s3d_set_tile",
-    "s3d_batch_synchronize", "s3d_batch_destroy", "s3d_grad_add",
+    "s3d_batch_synchronize", "s3d_batch_destroy", "s3d_grad_add", "s3d_vrange_noalloc",
 ]
 
 
@@ -101,6 +101,7 @@ def lib():
         L.s3d_set_field.argtypes = [vp, C.c_int, dp]
         L.s3d_grad_add.argtypes = [vp, dp, dp, dp, C.c_int]
         L.s3d_vrange.argtypes = [vp, dp]
+        L.s3d_vrange_noalloc.argtypes = [vp, dp]
         L.s3d_kinetic_energy.argtypes = [vp, C.POINTER(C.c_double)]
         L.s3d_synchronize.argtypes = [vp]
         L.s3d_timing_enable.argtypes = [vp, C.c_int]
@@ -282,6 +283,12 @@ class Simulation:
     def vrange(self):
         out = np.zeros(2)
         _chk(lib().s3d_vrange(self._h, out))
+        return out
+
+    def vrange_noalloc(self):
+        """s3d_vrange without any allocation (fixed per-device scratch)."""
+        out = np.zeros(2)
+        _chk(lib().s3d_vrange_noalloc(self._h, out))
         return out
 
     def kinetic_energy(self):
