@@ -25,11 +25,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ses3d-nt_amd"))
 
 import numpy as np  # noqa: E402
+from dataclasses import replace  # noqa: E402
 
 from ses3d_amd import api, synth  # noqa: E402
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
 CPU_SAMPLE = dict(nx=24, ny=24, nz=12)   # oracle sample of the same workload
+MISFIT_STEPS = 1200        # FP32 vs FP64 misfit leg after the timing
 
 
 def log(*a):
@@ -81,12 +83,15 @@ def step_bytes(mode, u, visco, L=3, S=1):
 
 
 def elem_kernel_bytes(mode, u, visco):
-    """Algorithmic bytes per element-local node of one element-kernel launch:
-    stress r+w (12 words, +12 compensation words), memory variables r+w
-    (36 words), lambda, mu (+tau) reads, the velocity read once per unique
-    node (3u words) and the element force write (3 words)."""
+    """The element kernel's share of the step's algorithmic bytes per
+    element-local node: stress r+w (12 words, +12 compensation words),
+    memory variables r+w (36 words), lambda, mu (+tau) reads and the
+    velocity read once per unique node (3u words); the rest of step_bytes
+    (velocity write, its compensation r+w, dt/M) is the node kernel's. The
+    element force that the two kernels exchange (3 words written here, read
+    by the node kernel) is not among the algorithmic bytes."""
     s, _, ls, st = mode_sizes(mode)
-    return s * (12 + 36 * visco + 3 * u + 3) + ls * 12 + st * (2 + visco)
+    return s * (12 + 36 * visco + 3 * u) + ls * 12 + st * (2 + visco)
 
 
 def load_sims(case, modes, slab, remap=True):
@@ -123,6 +128,7 @@ def load_sims(case, modes, slab, remap=True):
                 sim.add_receiver(loc[0], loc[1], r.channels)
                 nrec += 1
         sim.synchronize()
+        sim.nrec = nrec
         log("[bench] %s ready: device %.1f GB, courant %.3f, %d receivers (%.0f s)"
             % (name, sim.device_bytes() / 1e9, sim.cfl(), nrec, time.time() - t0))
     return sims
@@ -139,9 +145,41 @@ def time_gpu(sim, steps, warmup):
     _torch_sync()
     wall = time.perf_counter() - t0
     ms, cnt = sim.kernel_times()
+    sim.timing(False)
     vr = sim.vrange()
-    sim.close()
     return wall, ms, cnt, vr
+
+
+def ring_receivers(src, radii=(0.5, 1.0, 1.5, 2.0), azimuths=8):
+    """Surface receivers on rings around a source (degrees): they see the
+    whole source wavelet within the misfit leg."""
+    out = []
+    for d in radii:
+        for q in range(azimuths):
+            a = 2 * np.pi * q / azimuths
+            out.append(synth.Receiver(src.lat + d * np.cos(a), src.lon + d * np.sin(a), 0.0,
+                                      "R%02d_%d" % (int(10 * d), q)))
+    return out
+
+
+def misfit_leg(sim32, sim64, n_rec, steps):
+    """Continue both simulations to `steps` and compare the traces (vx, vy,
+    vz) of every receiver: relative L2 per trace over the traces with signal
+    (norm above 1e-3 of the largest)."""
+    for sim in (sim32, sim64):
+        left = steps - sim.it
+        while left > 0:
+            n = min(400, left)
+            sim.step(n)
+            sim.synchronize()
+            left -= n
+    t32 = np.array([[sim32.trace(r, c) for c in range(3)] for r in range(n_rec)])
+    t64 = np.array([[sim64.trace(r, c) for c in range(3)] for r in range(n_rec)])
+    norms = np.linalg.norm(t64, axis=2)
+    keep = norms > 1e-3 * norms.max()
+    err = np.linalg.norm(t32 - t64, axis=2)[keep] / norms[keep]
+    return {"steps": steps, "traces": int(keep.sum()), "trace_relL2_median": float(np.median(err)),
+            "trace_relL2_max": float(err.max()), "reference": "FP64 GPU path, same mesh"}
 
 
 def _torch_sync():
@@ -162,6 +200,13 @@ def copy_probe(gb):
         raise RuntimeError("copy_probe failed: " + r.stderr)
 
 
+def cpu_calibration():
+    """The oracle against the reference on the reference's CPU-runnable case
+    (12x12x8 homogeneous, FP64), both timed on the same CPU type."""
+    p = os.path.join(ROOT, "profiles", "cpu_calibration_homogeneous_12x12x8_p4.json")
+    return json.load(open(p)) if os.path.exists(p) else None
+
+
 def cpu_baseline(case_kw, target_s=15.0):
     """FP64 oracle on a bounded sub-mesh of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -180,11 +225,22 @@ def cpu_baseline(case_kw, target_s=15.0):
     wall = time.perf_counter() - t0
     o.close()
     g = case.grid
-    return {"value": g.nodes * n / wall, "unit": "GLL-node updates/s",
-            "cores": int(threads), "kind": "port",
-            "sample": "FP64 oracle (C, OpenMP) on a %dx%dx%d-element, p=4 sub-mesh of the "
-                      "same layered visco-elastic workload, %d steps, %.1f s"
-                      % (g.nx, g.ny, g.nz, n, wall)}
+    out = {"value": g.nodes * n / wall, "unit": "GLL-node updates/s",
+           "cores": int(threads), "kind": "port",
+           "sample": "FP64 oracle (C, OpenMP) on a %dx%dx%d-element, p=4 sub-mesh of the "
+                     "same layered visco-elastic workload, %d steps, %.1f s"
+                     % (g.nx, g.ny, g.nz, n, wall)}
+    cal = cpu_calibration()
+    if cal:
+        r = cal["oracle_over_reference"]["8_cores"]
+        out["calibration"] = {
+            "oracle_over_reference_on_12x12x8_8_cores": r,
+            "reference_node_updates_per_s_12x12x8_8_ranks": cal["reference"]["node_updates_per_s_8_ranks"],
+            "oracle_node_updates_per_s_12x12x8_8_threads": cal["oracle"]["node_updates_per_s_8_threads"],
+            "reference_equivalent_value": out["value"] / r,
+            "note": "the reference's Fortran/MPI path cannot run here; on its own CPU-runnable "
+                    "case the oracle is %.2fx faster than it on the same cores" % r}
+    return out
 
 
 def precision_study(args):
@@ -366,6 +422,9 @@ def main():
     ap.add_argument("--study-steps", type=int, default=4000)
     ap.add_argument("--study-receivers", type=int, default=256)
     ap.add_argument("--study-variants", default="")
+    ap.add_argument("--misfit-steps", type=int, default=MISFIT_STEPS,
+                    help="after the timing, continue the timed and the FP64 simulation to this "
+                         "step and report the trace misfit (0: skip)")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="first run a plain device copy of this many GB (counter calibration)")
     args = ap.parse_args()
@@ -390,7 +449,10 @@ def main():
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
     nt = args.warmup + args.steps
-    case = synth.layered(seed=1, nt=nt, nx=args.nx, ny=args.ny, nz=args.nz, n_rec=256)
+    misfit_steps = 0 if (args.no_fp64 or args.mode == "fp64") else args.misfit_steps
+    case = synth.layered(seed=1, nt=max(nt, misfit_steps), nx=args.nx, ny=args.ny, nz=args.nz,
+                         n_rec=256)
+    case = replace(case, receivers=ring_receivers(case.sources[0]) + case.receivers)
     g = case.grid
     N = g.nodes
     u = g.unique_nodes / N
@@ -411,10 +473,16 @@ def main():
     achieved = eb / (elem_ms * 1e-3)
     Bstep = step_bytes(mode, u, 1)
     rtype = "double" if mode == "fp64" else "float"
+    s_state = mode_sizes(mode)[0]
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": None,
             "kernel": "elem_kernel<4,%s,visco>, %d elements per workgroup" % (rtype, tile),
-            "algorithmic_bytes_per_launch": eb}
+            "algorithmic_bytes_per_launch": eb,
+            "algorithmic_bytes_per_node": eb / N,
+            # the element force the element kernel writes for the node kernel
+            # is outside the algorithmic bytes; with it the kernel moves:
+            "force_write_bytes_per_node": 3 * s_state,
+            "achieved_with_force_write": (eb + 3 * s_state * N) / (elem_ms * 1e-3) / 1e9}
     roof["achieved"] = achieved / 1e9
     roof["peak"] = HBM_PEAK / 1e9
     # HBM traffic per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE,
@@ -422,6 +490,9 @@ def main():
     tf = os.path.join(ROOT, "profiles", "traffic_layered_192x192x96_p4_%s.json" % mode)
     if (g.nx, g.ny, g.nz) == (192, 192, 96) and os.path.exists(tf):
         roof["traffic"] = json.load(open(tf))["traffic_bytes_per_launch"]["elem_kernel"]
+        roof["traffic_kind"] = ("FETCH_SIZE + WRITE_SIZE of the element kernel (memory-side "
+                                "bytes below L2, including what the Infinity Cache serves), "
+                                "calibrated on a plain copy kernel; includes the force write")
     step_frac = Bstep * value / HBM_PEAK
 
     out = {
@@ -432,8 +503,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if mode == "fp64" else "f32",
         "data": "synthetic (seeded layered visco-elastic model, GRF vs perturbation)",
         "config": {"workload": "layered visco-elastic section 40x40 deg x 1400 km, "
-                               "%dx%dx%d elements, GLL order 4, 3 SLS mechanisms, %s"
-                               % (g.nx, g.ny, g.nz, MODE_TEXT[mode]),
+                               "%dx%dx%d elements, GLL order 4, 3 SLS mechanisms, %s, "
+                               "one source, 256 surface receivers and 32 on rings around "
+                               "the source" % (g.nx, g.ny, g.nz, MODE_TEXT[mode]),
                    "elements": g.n_elem, "gll_nodes": N, "unique_nodes": g.unique_nodes,
                    "precision": mode, "parallelism": "single GPU"},
         "roofline": roof,
@@ -450,6 +522,14 @@ def main():
                            "node_ms": ms64[1] / max(cnt64[1], 1),
                            "speedup_fp32_over_fp64": value / v64}
         log("[bench] fp64 %.3f ms/step" % (1e3 * wall64 / args.steps))
+        if misfit_steps > 0:
+            t0 = time.time()
+            out["fp32_vs_fp64_misfit"] = misfit_leg(sims[mode], sims["fp64"], sims[mode].nrec,
+                                                    misfit_steps)
+            log("[bench] misfit leg to step %d in %.0f s: %s" % (misfit_steps, time.time() - t0,
+                                                               out["fp32_vs_fp64_misfit"]))
+    for sim in sims.values():
+        sim.close()
     if not args.no_cpu:
         cb = cpu_baseline({"seed": 1})
         out["cpu_baseline"] = cb
