@@ -200,6 +200,27 @@ def copy_probe(gb):
         raise RuntimeError("copy_probe failed: " + r.stderr)
 
 
+def contraction_probe():
+    """ses3d-nt_amd/bin/contraction_probe: the GLL derivative contraction on
+    its own, FP64-coefficient VALU (the element kernel's form), FP32 VALU and
+    FP32 matrix cores with split coefficients, for one element's 25 lines x 3
+    components x 16 sources per workgroup. Prints one JSON line."""
+    import subprocess
+    exe = os.path.join(ROOT, "ses3d-nt_amd", "bin", "contraction_probe")
+    r = subprocess.run([exe, "layout"], capture_output=True, text=True)
+    log(r.stdout.strip())
+    if r.returncode != 0:
+        raise RuntimeError("contraction_probe layout check failed: " + r.stdout + r.stderr)
+    res = {}
+    for mode in ("valu64", "valu32", "mfma32"):
+        r = subprocess.run([exe, mode, "1200", "200", "2048"], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("contraction_probe %s failed: %s" % (mode, r.stderr))
+        res[mode] = json.loads(r.stdout)
+        log("[contraction] %s" % r.stdout.strip())
+    print(json.dumps({"probe": "derivative contraction, GLL order 4", "results": res}), flush=True)
+
+
 def cpu_calibration():
     """The oracle against the reference on the reference's CPU-runnable case
     (12x12x8 homogeneous, FP64), both timed on the same CPU type."""
@@ -345,7 +366,9 @@ def multi_source_bench(args):
     single copy of the model (Simulation.shared), advanced by s3d_batch_step
     with the members' workgroups source-innermost, against the same
     simulations stepped round robin (whole mesh per source, no sharing of
-    the static-field reads)."""
+    the static-field reads). More sources than --per-batch run as successive
+    batches of --per-batch (the state of 32 sources at 128x128x48 exceeds the
+    GPU's memory); the rates count every source over the summed times."""
     mode = args.mode
     kw = dict(MODES[mode])
     if kw.pop("bf16_static"):
@@ -355,44 +378,63 @@ def multi_source_bench(args):
     N = g.nodes
     S = len(case.sources)
     u = g.unique_nodes / N
+    per = max(1, min(args.per_batch, S))
     t0 = time.time()
-    sims, recs = api.build_shared_sources(case, slab=args.slab, autotune=True, **kw)
-    dev = sum(s_.device_bytes() for s_ in sims)
-    log("[multi-source] %dx%dx%d p=%d, %d sources, %d receivers, %d GLL nodes per source, "
-        "device %.1f GB, setup %.0f s" % (g.nx, g.ny, g.nz, g.lpd, S, len(recs), N, dev / 1e9,
-                                          time.time() - t0))
-    b = api.Batch(sims)
-    b.step(args.warmup)
-    b.synchronize()
-    t0 = time.perf_counter()
-    b.step(args.steps)
-    b.synchronize()
-    wall_b = time.perf_counter() - t0
-    b.close()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    owner, recs = api.build_case(replace(case, sources=[]), slab=args.slab, autotune=True, **kw)
+    log("[multi-source] %dx%dx%d p=%d, %d sources in batches of %d, %d receivers, %d GLL nodes "
+        "per source, model set up in %.0f s" % (g.nx, g.ny, g.nz, g.lpd, S, per, len(recs), N,
+                                                time.time() - t0))
+    wall_b = wall_r = 0.0
+    dev = 0.0
+    for first in range(0, S, per):
+        srcs = case.sources[first:first + per]
+        sims = []
+        for s_ in srcs:
+            sim = api.Simulation.shared(owner)
+            api._add_channels(sim, case, [s_])
+            sims.append(sim)
+        dev = max(dev, owner.device_bytes() + sum(x.device_bytes() for x in sims))
+        b = api.Batch(sims)
+        b.step(args.warmup)
+        b.synchronize()
+        t0 = time.perf_counter()
+        b.step(args.steps)
+        b.synchronize()
+        wb = time.perf_counter() - t0
+        b.close()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            for s_ in sims:
+                s_.step(1)
         for s_ in sims:
-            s_.step(1)
-    for s_ in sims:
-        s_.synchronize()
-    wall_r = time.perf_counter() - t0
-    B = step_bytes(mode, u, True, S=S)
+            s_.synchronize()
+        wr = time.perf_counter() - t0
+        log("[multi-source] sources %d..%d: batched %.3f ms/step, round robin %.3f ms/step, "
+            "device %.0f GB" % (first, first + len(srcs) - 1, 1e3 * wb / args.steps,
+                                1e3 * wr / args.steps, dev / 1e9))
+        wall_b += wb
+        wall_r += wr
+        for s_ in sims:
+            s_.close()
+    owner.close()
+    B = step_bytes(mode, u, True, S=per)
     value = N * S * args.steps / wall_b
     out = {"metric": "GLL-node updates/s (node x source)",
            "workload": "%d moment-tensor sources on one layered visco-elastic model, %dx%dx%d "
-                       "elements, GLL order %d, %s" % (S, g.nx, g.ny, g.nz, g.lpd, mode),
-           "gll_nodes_per_source": N, "sources": S, "steps": args.steps,
+                       "elements, GLL order %d, %s, batches of %d" % (S, g.nx, g.ny, g.nz, g.lpd, mode,
+                                                                     per),
+           "gll_nodes_per_source": N, "sources": S, "per_batch": per, "steps": args.steps,
            "batched_shared": {"value": value, "ms_per_step": 1e3 * wall_b / args.steps},
            "round_robin": {"value": N * S * args.steps / wall_r,
                            "ms_per_step": 1e3 * wall_r / args.steps},
            "speedup_batched_vs_round_robin": wall_r / wall_b,
            "step_roofline": {"bytes_per_node_source": B, "achieved_GBps": B * value / 1e9,
                              "frac": B * value / HBM_PEAK},
-           "device_bytes": dev}
-    log("[multi-source] batched %.3f ms/step (%.3e node-source updates/s, %.1f %% of 8 TB/s at "
-        "%.1f B), round robin %.3f ms/step" % (1e3 * wall_b / args.steps, value,
-                                              100 * B * value / HBM_PEAK, B,
-                                              1e3 * wall_r / args.steps))
+           "device_bytes_peak": dev}
+    log("[multi-source] %d sources: batched %.3f ms/step (%.3e node-source updates/s, %.1f %% of "
+        "8 TB/s at %.1f B), round robin %.3f ms/step" % (S, 1e3 * wall_b / args.steps, value,
+                                                        100 * B * value / HBM_PEAK, B,
+                                                        1e3 * wall_r / args.steps))
     print(json.dumps(out), flush=True)
 
 
@@ -415,7 +457,9 @@ def main():
     ap.add_argument("--batch-sims", type=int, default=64)
     ap.add_argument("--multi-source", action="store_true",
                     help="many sources sharing one model (128x128x48, p=4) instead of the single mesh")
-    ap.add_argument("--sources", type=int, default=16,
+    ap.add_argument("--sources", type=int, default=32,
+                    help="sources sharing the model (config: 32)")
+    ap.add_argument("--per-batch", type=int, default=16,
                     help="sources per shared-model batch (16 fit in HBM at 128x128x48)")
     ap.add_argument("--precision-study", action="store_true",
                     help="FP32 variants vs FP64 seismogram / snapshot misfit instead of timing")
@@ -425,6 +469,8 @@ def main():
     ap.add_argument("--misfit-steps", type=int, default=MISFIT_STEPS,
                     help="after the timing, continue the timed and the FP64 simulation to this "
                          "step and report the trace misfit (0: skip)")
+    ap.add_argument("--contraction-probe", action="store_true",
+                    help="time the derivative contraction on the VALU and on the matrix cores")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="first run a plain device copy of this many GB (counter calibration)")
     args = ap.parse_args()
@@ -438,6 +484,9 @@ def main():
         sys.exit(1)
     if args.precision_study:
         precision_study(args)
+        return
+    if args.contraction_probe:
+        contraction_probe()
         return
     if args.batch:
         batch_bench(args)

@@ -2,6 +2,9 @@
 
 libses3d_hip.so   hipcc --offload-arch=gfx950, one translation unit
                   (csrc/s3d_api.hip includes csrc/s3d_kernels.hip)
+bin/copy_probe, bin/contraction_probe
+                  stand-alone measurement programs (HBM streams; the
+                  derivative contraction on the VALU and on the matrix cores)
 bin/ses3d_nt      flang: the Fortran host (fortran/*.f90) linked against the
                   library through ISO_C_BINDING
 """
@@ -45,14 +48,19 @@ def build_hip():
     return out
 
 
-def build_copy_probe():
-    src = os.path.join(PKG, "csrc", "copy_probe.hip")
+def build_probe(name):
+    """A stand-alone measurement program csrc/<name>.hip -> bin/<name>."""
+    src = os.path.join(PKG, "csrc", name + ".hip")
     bindir = os.path.join(PKG, "bin")
     os.makedirs(bindir, exist_ok=True)
-    exe = os.path.join(bindir, "copy_probe")
+    exe = os.path.join(bindir, name)
     flags = [f for f in HIP_FLAGS if f not in ("-shared", "-fPIC")]
     _run([HIPCC] + flags + [src, "-o", exe])
     return exe
+
+
+def build_copy_probe():
+    return build_probe("copy_probe")
 
 
 def build_fortran():
@@ -83,6 +91,7 @@ def build_fortran():
 def build_all():
     lib = build_hip()
     build_copy_probe()
+    build_probe("contraction_probe")
     exe = build_fortran()
     return lib, exe
 
