@@ -94,9 +94,11 @@ def elem_kernel_bytes(mode, u, visco):
     return s * (12 + 36 * visco + 3 * u) + ls * 12 + st * (2 + visco)
 
 
-def load_sims(case, modes, slab, remap=True):
+def load_sims(case, modes, slab, remap=True, tune=None):
     """Build one Simulation per precision mode from a single pass over the
-    model slabs; returns a list of (mode, sim)."""
+    model slabs; returns a list of (mode, sim). tune = (tile, LDS layout)
+    sets the FP32 simulation's element tile and layout instead of the
+    autotuner."""
     g = case.grid
     sims = [(name, api.Simulation(g, xcd_remap=remap, **MODES[name])) for name in modes]
     t0 = time.time()
@@ -113,9 +115,15 @@ def load_sims(case, modes, slab, remap=True):
             log("[bench] model layers %d..%d uploaded (%.0f s)" % (ez0, ez0 + nez - 1, time.time() - t0))
     for name, sim in sims:
         sim.init()
-        tile, tms = sim.autotune()
-        sim.tile = tile
-        log("[bench] %s autotune: %d elements per workgroup, element kernel %.3f ms" % (name, tile, tms))
+        if tune is None or name == "fp64":
+            tile, tms = sim.autotune()
+            log("[bench] %s autotune: %d elements per workgroup, element kernel %.3f ms" % (name, tile, tms))
+        else:
+            sim.set_tile(tune[0])
+            sim.set_lds_layout(tune[1])
+        sim.tile, sim.remap, sim.layout = sim.tuning()
+        log("[bench] %s tuning: %d elements per workgroup, XCD order %d, LDS layout %d"
+            % (name, sim.tile, sim.remap, sim.layout))
         nrec = 0
         for s_ in case.sources:
             loc = sim.locate(s_.lat, s_.lon, s_.depth)
@@ -452,6 +460,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-remap", action="store_true", help="plain workgroup order (no XCD remap)")
+    ap.add_argument("--tile", type=int, default=None,
+                    help="element tile of the timed mode instead of the autotuner's (0, 1, 2, 4)")
+    ap.add_argument("--lds-layout", type=int, default=0,
+                    help="LDS layout with --tile: 0 planar, 1 node-interleaved")
     ap.add_argument("--batch", action="store_true",
                     help="uneven batch of independent simulations instead of the single mesh")
     ap.add_argument("--batch-sims", type=int, default=64)
@@ -510,8 +522,9 @@ def main():
 
     mode = args.mode
     modes = [mode] + ([] if args.no_fp64 or mode == "fp64" else ["fp64"])
-    sims = dict(load_sims(case, modes, args.slab, remap=not args.no_remap))
-    tile = getattr(sims[mode], "tile", 0)
+    tune = None if args.tile is None else (args.tile, args.lds_layout)
+    sims = dict(load_sims(case, modes, args.slab, remap=not args.no_remap, tune=tune))
+    tile = sims[mode].tile
     wall, ms, cnt, vr = time_gpu(sims[mode], args.steps, args.warmup)
     value = N * args.steps / wall
     log("[bench] %s %.3f ms/step, elem %.3f ms, node %.3f ms, v range %s"
@@ -525,7 +538,10 @@ def main():
     s_state = mode_sizes(mode)[0]
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": None,
-            "kernel": "elem_kernel<4,%s,visco>, %d elements per workgroup" % (rtype, tile),
+            "kernel": "elem_kernel<4,%s,visco>, %d elements per workgroup, %s LDS layout, %s "
+                      "workgroup order" % (rtype, tile,
+                                           "node-interleaved" if sims[mode].layout else "planar",
+                                           "XCD-aware" if sims[mode].remap else "plain"),
             "algorithmic_bytes_per_launch": eb,
             "algorithmic_bytes_per_node": eb / N,
             # the element force the element kernel writes for the node kernel

@@ -138,9 +138,9 @@ int s3d_init(s3d_ctx *ctx);
 int s3d_cfl(s3d_ctx *ctx, double *courant);
 
 /* Autotuner: times the element-kernel tile variants (elements per
- * workgroup) and the XCD-aware block order on this mesh and keeps the
- * fastest; *tile gets the elements per workgroup chosen and *ms the element
- * kernel time. Needs the state at rest (after s3d_init, before s3d_set_field
+ * workgroup), the XCD-aware block order and the LDS staging layout on this
+ * mesh and keeps the fastest; *tile gets the elements per workgroup chosen
+ * and *ms the element kernel time (s3d_get_tuning gives the rest). Needs the state at rest (after s3d_init, before s3d_set_field
  * and s3d_step): the trial launches then leave every field unchanged. No
  * counterpart in the reference. */
 int s3d_autotune(s3d_ctx *ctx, int *tile, double *ms);
@@ -149,6 +149,17 @@ int s3d_autotune(s3d_ctx *ctx, int *tile, double *ms);
  * elements per workgroup (where that fits 1024 lanes). No counterpart in the
  * reference. */
 int s3d_set_tile(s3d_ctx *ctx, int tile);
+
+/* LDS layout of the FP32 element kernel without timing: 0 planar (one array
+ * per staged component), 1 node-interleaved (the components one contraction
+ * term reads in one 16-byte vector). Every layout gives bit-identical
+ * results; FP64 and BF16-static simulations use 0. s3d_autotune searches
+ * tile x block order x layout. No counterpart in the reference. */
+int s3d_set_lds_layout(s3d_ctx *ctx, int layout);
+
+/* The element tile (elements per workgroup), block order (1: XCD-aware) and
+ * LDS layout in use, e.g. as chosen by s3d_autotune. */
+int s3d_get_tuning(s3d_ctx *ctx, int *tile, int *xcd_remap, int *lds_layout);
 
 /* GLL points, weights and derivative matrix dl[i + (p+1)*j] = l_i'(xi_j)
  * (reference get_knots / get_weights / get_dlgll, src/gll_mod.f90:178-275). */

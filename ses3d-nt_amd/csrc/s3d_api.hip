@@ -216,6 +216,7 @@ struct s3d_ctx {
     int graph_steps = 0;
     bool use_graph = true;
     int epb = 0;               // element tile (0 = default), see s3d_autotune
+    int lds_layout = 0;        // LDS staging layout of the FP32 element kernel, see Stage
     bool fields_set = false;   // s3d_set_field was called (state not at rest)
     // s3d_create_shared: the simulation whose static fields (moduli, tau, rho,
     // A/B/C, dt/M, FP64 mass, 1-D tables, taper profiles) this one uses, and
@@ -417,19 +418,35 @@ void ev_end(s3d_ctx *c, int k) { ev_begin(c, k); }
 // 2 and 4 elements per workgroup (where that fits 1024 lanes) besides the
 // default; FP64 uses the default tile only.
 // the stress-compensation mode is a kernel template parameter (FP32 only)
+template <int P, int E, typename R, bool VISCO, bool S16, int LL>
+void launch_elem_ll(int ks, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
+{
+    using S = Shape<P, E>;
+    if (ks == 1)
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 1, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
+    else if (ks == 2)
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 2, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
+    else
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
+}
+
+// stress-compensation mode and LDS layout are kernel template parameters
+// (FP32 only; FP64 and the BF16-static kernels use the planar layout)
 template <int P, int E, typename R, bool VISCO, bool S16>
-int launch_elem_ks(int ks, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
+int launch_elem_ks(int ks, int ll, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
 {
     using S = Shape<P, E>;
     if constexpr (std::is_same<R, float>::value) {
-        if (ks == 1)
-            hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 1>), dim3(ge), dim3(S::NT), 0, st, ea);
-        else if (ks == 2)
-            hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 2>), dim3(ge), dim3(S::NT), 0, st, ea);
-        else
-            hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0>), dim3(ge), dim3(S::NT), 0, st, ea);
+        bool done = false;
+        if constexpr (!S16) {
+            if (ll == 1) {
+                launch_elem_ll<P, E, R, VISCO, S16, 1>(ks, ea, st, ge);
+                done = true;
+            }
+        }
+        if (!done) launch_elem_ll<P, E, R, VISCO, S16, 0>(ks, ea, st, ge);
     } else {
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0>), dim3(ge), dim3(S::NT), 0, st, ea);
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0, 0>), dim3(ge), dim3(S::NT), 0, st, ea);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -440,14 +457,14 @@ int launch_elem_e(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st)
 {
     using S = Shape<P, E>;
     const unsigned ge = nblocks(c->nel, S::EPB);
-    const int ks = c->o.kahan_sigma;
+    const int ks = c->o.kahan_sigma, ll = c->lds_layout;
     if constexpr (std::is_same<R, float>::value && E == 0) {
         if (c->o.bf16_static)   // default tile only
-            return c->visco ? launch_elem_ks<P, 0, float, true, true>(ks, ea, st, ge)
-                            : launch_elem_ks<P, 0, float, false, true>(ks, ea, st, ge);
+            return c->visco ? launch_elem_ks<P, 0, float, true, true>(ks, 0, ea, st, ge)
+                            : launch_elem_ks<P, 0, float, false, true>(ks, 0, ea, st, ge);
     }
-    return c->visco ? launch_elem_ks<P, E, R, true, false>(ks, ea, st, ge)
-                    : launch_elem_ks<P, E, R, false, false>(ks, ea, st, ge);
+    return c->visco ? launch_elem_ks<P, E, R, true, false>(ks, ll, ea, st, ge)
+                    : launch_elem_ks<P, E, R, false, false>(ks, ll, ea, st, ge);
 }
 
 template <int P>
@@ -489,30 +506,34 @@ int autotune_p(s3d_ctx *c, int *tile_out, double *ms_out)
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     double best = 1e300;
-    int best_tile = c->epb, best_remap = c->o.xcd_remap;
-    for (int tile : tile_candidates<P>(c->fp32, c->o.bf16_static)) {
-        for (int remap = 0; remap < 2; ++remap) {
-            c->o.xcd_remap = remap;
-            ElemArgs<R> ea = elem_args<P, R>(c);
-            if (int rc = launch_elem<P, R>(c, ea, c->stream, tile)) return rc;   // warm-up
-            HIPCHK(hipEventRecord(e0, c->stream));
-            for (int r = 0; r < 3; ++r)
-                if (int rc = launch_elem<P, R>(c, ea, c->stream, tile)) return rc;
-            HIPCHK(hipEventRecord(e1, c->stream));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms / 3.0 < best) {
-                best = ms / 3.0;
-                best_tile = tile;
-                best_remap = remap;
+    int best_tile = c->epb, best_remap = c->o.xcd_remap, best_ll = c->lds_layout;
+    const int nll = (c->fp32 && !c->o.bf16_static) ? 2 : 1;
+    for (int tile : tile_candidates<P>(c->fp32, c->o.bf16_static))
+        for (int ll = 0; ll < nll; ++ll)
+            for (int remap = 0; remap < 2; ++remap) {
+                c->o.xcd_remap = remap;
+                c->lds_layout = ll;
+                ElemArgs<R> ea = elem_args<P, R>(c);
+                if (int rc = launch_elem<P, R>(c, ea, c->stream, tile)) return rc;   // warm-up
+                HIPCHK(hipEventRecord(e0, c->stream));
+                for (int r = 0; r < 3; ++r)
+                    if (int rc = launch_elem<P, R>(c, ea, c->stream, tile)) return rc;
+                HIPCHK(hipEventRecord(e1, c->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms / 3.0 < best) {
+                    best = ms / 3.0;
+                    best_tile = tile;
+                    best_remap = remap;
+                    best_ll = ll;
+                }
             }
-        }
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     c->epb = best_tile;
     c->o.xcd_remap = best_remap;
+    c->lds_layout = best_ll;
     drop_graph(c);
     if (tile_out) *tile_out = best_tile == 0 ? default_epb<P>() : best_tile;
     if (ms_out) *ms_out = best;
@@ -881,6 +902,7 @@ int upload_receivers(s3d_ctx *c)
 struct BatchGroup {
     int p = 0;
     int epb = 0;
+    int ll = 0;                 // LDS layout
     int ks = 0;                 // stress compensation mode
     bool fp32 = false, visco = false, kahan = false;
     std::vector<s3d_ctx *> sims;
@@ -972,7 +994,15 @@ template <int P, int E, typename R, bool VISCO, int KS>
 int batch_launch_group_k(const BatchGroup &G, hipStream_t st)
 {
     using S = Shape<P, E>;
-    hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS>), dim3(G.ewg), dim3(S::NT), 0, st,
+    if constexpr (std::is_same<R, float>::value) {
+        if (G.ll == 1) {
+            hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 1>), dim3(G.ewg), dim3(S::NT), 0, st,
+                               (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+    }
+    hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 0>), dim3(G.ewg), dim3(S::NT), 0, st,
                        (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
     HIPCHK(hipGetLastError());
     return 0;
@@ -1097,7 +1127,8 @@ int batch_build(s3d_batch *b)
         bool placed = false;
         for (auto &G : b->groups)
             if (G.p == c->p && G.fp32 == c->fp32 && G.visco == c->visco &&
-                G.kahan == (c->lo[0] != nullptr) && G.epb == c->epb && G.ks == c->o.kahan_sigma) {
+                G.kahan == (c->lo[0] != nullptr) && G.epb == c->epb && G.ks == c->o.kahan_sigma &&
+                G.ll == c->lds_layout) {
                 G.sims.push_back(c);
                 placed = true;
                 break;
@@ -1109,6 +1140,7 @@ int batch_build(s3d_batch *b)
             G.visco = c->visco;
             G.kahan = c->lo[0] != nullptr;
             G.epb = c->epb;
+            G.ll = c->fp32 ? c->lds_layout : 0;
             G.ks = c->o.kahan_sigma;
             G.sims.push_back(c);
             b->groups.push_back(G);
@@ -1291,6 +1323,7 @@ int s3d_create_shared(s3d_ctx *base, s3d_ctx **out)
     c->px1 = m->px1; c->py1 = m->py1; c->pz1 = m->pz1;
     c->courant = m->courant;
     c->epb = m->epb;
+    c->lds_layout = m->lds_layout;
     // static fields: the model's, not copied
     c->dt_inv_mm = m->dt_inv_mm; c->mm64 = m->mm64; c->lam = m->lam; c->mu = m->mu; c->tau = m->tau;
     c->rho = m->rho; c->ca = m->ca; c->cb = m->cb; c->cc = m->cc; c->tth = m->tth; c->tr = m->tr;
@@ -1535,6 +1568,36 @@ int s3d_set_tile(s3d_ctx *c, int tile)
     if (tile * P1 * P1 * P1 > 1024) return fail("tile does not fit 1024 lanes at this order");
     c->epb = tile;
     drop_graph(c);
+    return 0;
+}
+
+int s3d_set_lds_layout(s3d_ctx *c, int layout)
+{
+    if (!c) return fail("null argument");
+    if (layout != 0 && layout != 1) return fail("LDS layout must be 0 (planar) or 1 (node-interleaved)");
+    if (layout != 0 && (!c->fp32 || c->o.bf16_static))
+        return fail("FP64 and BF16-static simulations use the planar LDS layout");
+    c->lds_layout = layout;
+    drop_graph(c);
+    return 0;
+}
+
+int s3d_get_tuning(s3d_ctx *c, int *tile, int *xcd_remap, int *lds_layout)
+{
+    if (!c || !tile || !xcd_remap || !lds_layout) return fail("null argument");
+    int d = 1;
+    switch (c->p) {
+    case 2: d = default_epb<2>(); break;
+    case 3: d = default_epb<3>(); break;
+    case 4: d = default_epb<4>(); break;
+    case 5: d = default_epb<5>(); break;
+    case 6: d = default_epb<6>(); break;
+    case 7: d = default_epb<7>(); break;
+    default: break;
+    }
+    *tile = c->epb == 0 ? d : c->epb;
+    *xcd_remap = c->o.xcd_remap;
+    *lds_layout = c->lds_layout;
     return 0;
 }
 

@@ -99,6 +99,69 @@ __device__ __forceinline__ int wait_vmem_zero()
     return z;
 }
 
+typedef float lds4 __attribute__((ext_vector_type(4)));
+
+// LDS layout of the tile's staged velocity and effective stress (the
+// autotuner's choice; every layout gives the same arithmetic):
+//   LL = 0  planar: lds[c][node], one 4-byte read per component;
+//   LL = 1  node-interleaved (FP32 only): 48 bytes per node, the velocity in
+//           the first 16-byte vector and the stress as the three triples the
+//           divergence contracts, (xx xy xz | xy yy yz | xz yz zz), one
+//           16-byte read per contraction term instead of three 4-byte reads.
+template <typename R, int LL, int N>
+struct Stage {
+    static constexpr bool INTERLEAVED = LL == 1 && sizeof(R) == 4;
+    R planar[INTERLEAVED ? 1 : 6][INTERLEAVED ? 1 : N];
+    lds4 inter[INTERLEAVED ? N : 1][INTERLEAVED ? 3 : 1];
+
+    __device__ void put_v(int n, R v0, R v1, R v2)
+    {
+        if constexpr (INTERLEAVED) {
+            inter[n][0] = (lds4){v0, v1, v2, 0.f};
+        } else {
+            planar[0][n] = v0;
+            planar[1][n] = v1;
+            planar[2][n] = v2;
+        }
+    }
+    // velocity components at node n
+    __device__ void get_v(int n, R *v) const
+    {
+        if constexpr (INTERLEAVED) {
+            const lds4 w = inter[n][0];
+            v[0] = w.x; v[1] = w.y; v[2] = w.z;
+        } else {
+            v[0] = planar[0][n]; v[1] = planar[1][n]; v[2] = planar[2][n];
+        }
+    }
+    // effective stress s[xx xy xz yy yz zz]
+    __device__ void put_s(int n, const R *sv)
+    {
+        if constexpr (INTERLEAVED) {
+            inter[n][0] = (lds4){sv[XX], sv[XY], sv[XZ], 0.f};
+            inter[n][1] = (lds4){sv[XY], sv[YY], sv[YZ], 0.f};
+            inter[n][2] = (lds4){sv[XZ], sv[YZ], sv[ZZ], 0.f};
+        } else {
+#pragma unroll
+            for (int c = 0; c < 6; ++c) planar[c][n] = sv[c];
+        }
+    }
+    // the stress components that contraction d (0 theta, 1 phi, 2 r) reads at
+    // node n: (xx xy xz), (xy yy yz), (xz yz zz)
+    __device__ void get_s(int d, int n, R *w3) const
+    {
+        if constexpr (INTERLEAVED) {
+            const lds4 w = inter[n][d];
+            w3[0] = w.x; w3[1] = w.y; w3[2] = w.z;
+        } else {
+            constexpr int comp[3][3] = {{XX, XY, XZ}, {XY, YY, YZ}, {XZ, YZ, ZZ}};
+            w3[0] = planar[comp[d][0]][n];
+            w3[1] = planar[comp[d][1]][n];
+            w3[2] = planar[comp[d][2]][n];
+        }
+    }
+};
+
 // Per-lane state of one element tile (EPB elements, one lane per
 // element-local node).
 template <typename R, bool VISCO>
@@ -182,14 +245,14 @@ enum { G_SIN = 0, G_ISIN, G_COT, G_R, G_IR, G_R2, G_PX, G_PY, G_PZ, G_N };
 // Element kernel body: the workgroup advances tile `blk` (EPB elements, one
 // lane per element-local node). S16: the static per-node fields are stored
 // in BF16 (ElemArgs::s16). KS: stress compensation (see load_tile).
-template <int P, int E, typename R, bool VISCO, bool S16, int KS>
+template <int P, int E, typename R, bool VISCO, bool S16, int KS, int LL>
 __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
     using S = Shape<P, E>;
     using C = coef_t<R>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB, NE = EPB;
     constexpr int NS = state_components(KS, VISCO);
-    __shared__ R lds[6][EPB * NP];
+    __shared__ Stage<R, LL, EPB * NP> lds;
     __shared__ C cf[4][P1 * P1];     // Dx, Dy, Dz, dl*w
     __shared__ C cw[P1];             // GLL weights
     __shared__ C gtab[NE][G_N * P1]; // geometry rows of the tile's elements
@@ -276,11 +339,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         const long ev = valid ? e : 0;
         const C *gs = gtab[es];
 
-        if (inside) {
-            lds[0][L + node] = cur.v[0];
-            lds[1][L + node] = cur.v[1];
-            lds[2][L + node] = cur.v[2];
-        }
+        if (inside) lds.put_v(L + node, cur.v[0], cur.v[1], cur.v[2]);
         // tables and velocity in LDS
         __syncthreads();
 
@@ -296,11 +355,15 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
                 const int nyq = L + i + P1 * (q + P1 * k);
                 const int nzq = L + i + P1 * (j + P1 * q);
                 const C cx = Dxi[q], cy = Dyj[q], cz = Dzk[q];
+                R vx[3], vy[3], vz[3];
+                lds.get_v(nxq, vx);
+                lds.get_v(nyq, vy);
+                lds.get_v(nzq, vz);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    dxv[c] += (C)lds[c][nxq] * cx;
-                    dyv[c] += (C)lds[c][nyq] * cy;
-                    dzv[c] += (C)lds[c][nzq] * cz;
+                    dxv[c] += (C)vx[c] * cx;
+                    dyv[c] += (C)vy[c] * cy;
+                    dzv[c] += (C)vz[c] * cz;
                 }
             }
         }
@@ -417,8 +480,10 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 
         __syncthreads();   // LDS velocity no longer needed
         if (inside) {
+            R se[6];   // effective stress, with the glut
 #pragma unroll
-            for (int c = 0; c < 6; ++c) lds[c][L + node] = sg[c] + src[c];
+            for (int c = 0; c < 6; ++c) se[c] = sg[c] + src[c];
+            lds.put_s(L + node, se);
         }
 
         // ---- memory variables (loop form of src/evolution.f90:415-442) ----
@@ -465,18 +530,18 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             const int nxq = L + q + P1 * (j + P1 * k);
             const int nyq = L + i + P1 * (q + P1 * k);
             const int nzq = L + i + P1 * (j + P1 * q);
-            // x contraction: sxx, sxy, sxz feed sx, sy, sz
-            Sx[0] += ax[q] * (C)lds[XX][nxq];
-            Sx[1] += ax[q] * (C)lds[XY][nxq];
-            Sx[2] += ax[q] * (C)lds[XZ][nxq];
-            // y contraction: sxy, syy, syz
-            Sy[0] += ay[q] * (C)lds[XY][nyq];
-            Sy[1] += ay[q] * (C)lds[YY][nyq];
-            Sy[2] += ay[q] * (C)lds[YZ][nyq];
-            // z contraction: sxz, syz, szz
-            Sz[0] += az[q] * (C)lds[XZ][nzq];
-            Sz[1] += az[q] * (C)lds[YZ][nzq];
-            Sz[2] += az[q] * (C)lds[ZZ][nzq];
+            // x contraction: sxx, sxy, sxz feed sx, sy, sz; y: sxy, syy,
+            // syz; z: sxz, syz, szz
+            R wx[3], wy[3], wz[3];
+            lds.get_s(0, nxq, wx);
+            lds.get_s(1, nyq, wy);
+            lds.get_s(2, nzq, wz);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                Sx[c] += ax[q] * (C)wx[c];
+                Sy[c] += ay[q] * (C)wy[c];
+                Sz[c] += az[q] * (C)wz[c];
+            }
         }
         const C rk = gs[G_R * P1 + k];
         const C Cx = cw[j] * cw[k] * rk * a.d1;
@@ -514,14 +579,14 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     }
 }
 
-template <int P, int E, typename R, bool VISCO, bool S16, int KS>
+template <int P, int E, typename R, bool VISCO, bool S16, int KS, int LL>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel(const ElemArgs<R> a)
 {
-    elem_body<P, E, R, VISCO, S16, KS>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
+    elem_body<P, E, R, VISCO, S16, KS, LL>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
 }
 
-template <int P, int E, typename R, bool VISCO, int KS>
+template <int P, int E, typename R, bool VISCO, int KS, int LL>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel_batch(const ElemArgs<R> *__restrict__ args, const int2 *__restrict__ own)
 {
@@ -529,7 +594,7 @@ elem_kernel_batch(const ElemArgs<R> *__restrict__ args, const int2 *__restrict__
     // batch: each XCD works through a contiguous range of simulations
     const int2 o = own[remap_block(blockIdx.x, gridDim.x, 1)];
     if (o.x < 0) return;   // padding workgroup
-    elem_body<P, E, R, VISCO, false, KS>(args[o.x], o.y);
+    elem_body<P, E, R, VISCO, false, KS, LL>(args[o.x], o.y);
 }
 
 // ---------------------------------------------------------------------------

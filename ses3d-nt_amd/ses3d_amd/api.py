@@ -43,6 +43,7 @@ EXPORTS = [
     "s3d_synchronize", "s3d_timing_enable", "s3d_kernel_times",
     "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step", "s3d_set_tile",
     "s3d_batch_synchronize", "s3d_batch_destroy", "s3d_grad_add", "s3d_vrange_noalloc",
+    "s3d_set_lds_layout", "s3d_get_tuning",
 ]
 
 
@@ -88,6 +89,8 @@ def lib():
         L.s3d_cfl.argtypes = [vp, C.POINTER(C.c_double)]
         L.s3d_autotune.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.s3d_set_tile.argtypes = [vp, C.c_int]
+        L.s3d_set_lds_layout.argtypes = [vp, C.c_int]
+        L.s3d_get_tuning.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.s3d_gll.argtypes = [C.c_int, dp, dp, dp]
         L.s3d_locate.argtypes = [vp, C.c_double, C.c_double, C.c_double, ip, dp]
         L.s3d_wavelet.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, dp]
@@ -217,6 +220,17 @@ class Simulation:
         """Element tile without timing (s3d_set_tile): 0 (default), 1, 2 or 4
         elements per workgroup."""
         _chk(lib().s3d_set_tile(self._h, int(tile)))
+
+    def set_lds_layout(self, layout):
+        """LDS staging layout of the FP32 element kernel (s3d_set_lds_layout):
+        0 planar, 1 node-interleaved."""
+        _chk(lib().s3d_set_lds_layout(self._h, int(layout)))
+
+    def tuning(self):
+        """(elements per workgroup, XCD-aware block order, LDS layout) in use."""
+        t, r, l = C.c_int(), C.c_int(), C.c_int()
+        _chk(lib().s3d_get_tuning(self._h, C.byref(t), C.byref(r), C.byref(l)))
+        return t.value, r.value, l.value
 
     def cfl(self):
         x = C.c_double()

@@ -308,6 +308,38 @@ def test_set_tile_identical(tile, shape):
         assert np.array_equal(s.get(f), ref.get(f)), (tile, f)
 
 
+@pytest.mark.parametrize("tile", [1, 2, 4])
+def test_lds_layouts_identical(tile):
+    """The node-interleaved LDS layout stages the same values and contracts
+    them in the same order as the planar one: traces and fields equal bit for
+    bit, with every tile, in a batch too; the autotuner reports its choice."""
+    case = synth.layered(seed=8, nt=24, nx=9, ny=7, nz=4, lpd=4, n_rec=4)
+    ref, recs = api.build_case(case, precision="fp32", kahan_sigma="bf16")
+    ref.set_tile(tile)
+    ref.step(24)
+    s, _ = api.build_case(case, precision="fp32", kahan_sigma="bf16")
+    s.set_tile(tile)
+    s.set_lds_layout(1)
+    assert s.tuning()[2] == 1
+    s.step(24)
+    for r in range(len(recs)):
+        for ch in range(3):
+            assert np.array_equal(s.trace(r, ch), ref.trace(r, ch)), (tile, r, ch)
+    for f in (api.VX, api.SYZ, api.MXX + 7):
+        assert np.array_equal(s.get(f), ref.get(f)), (tile, f)
+    b_sim, _ = api.build_case(case, precision="fp32", kahan_sigma="bf16")
+    b_sim.set_tile(tile)
+    b_sim.set_lds_layout(1)
+    b = api.Batch([b_sim])
+    b.step(24)
+    b.synchronize()
+    for r in range(len(recs)):
+        assert np.array_equal(b_sim.trace(r, 2), ref.trace(r, 2)), (tile, r)
+    b.close()
+    t, _, lay = api.build_case(case, precision="fp32", autotune=True)[0].tuning()
+    assert t in (1, 2, 4) and lay in (0, 1)
+
+
 def test_set_tile_checks():
     elastic = synth.homogeneous(nt=10, nx=5, ny=5, nz=4, n_rec=1)
     s, _ = api.build_case(elastic, precision="fp32")
@@ -319,6 +351,11 @@ def test_set_tile_checks():
     d, _ = api.build_case(elastic, precision="fp64")
     with pytest.raises(api.S3dError, match="default tile"):
         d.set_tile(4)
+    with pytest.raises(api.S3dError, match="planar LDS layout"):
+        d.set_lds_layout(1)
+    with pytest.raises(api.S3dError, match="LDS layout must be"):
+        s.set_lds_layout(2)
+    assert d.tuning() == (2, 1, 0)
 
 
 def test_batch_member_outlives_batch():
