@@ -215,10 +215,12 @@ CONTAINS
         CALL check(s3d_step(ctx, 1_c_int), 'step')
     END SUBROUTINE
 
-    !> Peak-to-peak velocity (src/elastic_vars.F90:179-185).
+    !> Peak-to-peak velocity (src/elastic_vars.F90:179-185): the
+    !! allocation-free reduction, or the allocating one on meshes beyond its
+    !! fixed scratch size.
     FUNCTION v_ptp_loc() RESULT(ptp)
         REAL(rk) :: ptp(2)
-        CALL check(s3d_vrange(ctx, ptp), 'vrange')
+        IF (s3d_vrange_noalloc(ctx, ptp) /= 0) CALL check(s3d_vrange(ctx, ptp), 'vrange')
     END FUNCTION
 
     SUBROUTINE finalize_evolution()

@@ -53,7 +53,8 @@ MODULE ses3d_hip_c
               s3d_get_field, s3d_set_field, s3d_vrange, s3d_kinetic_energy, &
               s3d_synchronize, s3d_autotune, s3d_set_tile, s3d_create_shared, &
               s3d_batch_create, s3d_batch_step, s3d_batch_synchronize, s3d_batch_destroy, &
-              s3d_grad_add
+              s3d_grad_add, s3d_vrange_noalloc, s3d_set_lds_layout, s3d_get_tuning, &
+              s3d_timing_enable, s3d_kernel_times, s3d_device_bytes
     PUBLIC :: check, s3d_error_message
 
     INTERFACE
@@ -197,6 +198,40 @@ MODULE ses3d_hip_c
             IMPORT :: c_int, c_ptr
             TYPE(c_ptr), VALUE :: ctx
             INTEGER(c_int), VALUE :: tile
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_set_lds_layout(ctx, layout) BIND(C, NAME='s3d_set_lds_layout')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int), VALUE :: layout
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_get_tuning(ctx, tile, xcd_remap, lds_layout) &
+                BIND(C, NAME='s3d_get_tuning')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int), INTENT(OUT) :: tile, xcd_remap, lds_layout
+        END FUNCTION
+        ! the v range without any allocation (no counterpart in the reference)
+        INTEGER(c_int) FUNCTION s3d_vrange_noalloc(ctx, out2) BIND(C, NAME='s3d_vrange_noalloc')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(OUT) :: out2(2)
+        END FUNCTION
+        ! kernel timing with HIP events and device memory held
+        INTEGER(c_int) FUNCTION s3d_timing_enable(ctx, enable) BIND(C, NAME='s3d_timing_enable')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int), VALUE :: enable
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_kernel_times(ctx, ms3, count3) BIND(C, NAME='s3d_kernel_times')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(OUT) :: ms3(3)
+            INTEGER(c_int), INTENT(OUT) :: count3(3)
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_device_bytes(ctx, bytes) BIND(C, NAME='s3d_device_bytes')
+            IMPORT :: c_int, c_ptr, c_double
+            TYPE(c_ptr), VALUE :: ctx
+            REAL(c_double), INTENT(OUT) :: bytes
         END FUNCTION
         ! further simulations on one model, and batches (no counterpart in
         ! the reference; see include/ses3d_hip.h)

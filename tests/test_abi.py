@@ -72,3 +72,14 @@ def test_bad_arguments_are_rejected():
     bad = synth.Grid(**{**g.__dict__, "lpd": 9})
     with pytest.raises(api.S3dError):
         api.Simulation(bad)
+
+
+def test_fortran_binding_declares_every_entry_point():
+    """The ISO_C_BINDING module a Fortran host uses binds every entry point of
+    the C header, under the header's name."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    hdr = open(os.path.join(root, "include", "ses3d_hip.h")).read()
+    declared = set(re.findall(r"^(?:int|const char \*)\s*(s3d_\w+)\(", hdr, re.M))
+    f90 = open(os.path.join(root, "ses3d-nt_amd", "fortran", "s3d_c_binding.f90")).read()
+    bound = set(re.findall(r"BIND\(C, NAME='(s3d_\w+)'\)", f90))
+    assert declared and declared == bound, (declared - bound, bound - declared)
