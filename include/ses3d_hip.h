@@ -64,6 +64,10 @@ enum {
     /* adjoint sensitivity densities (reference grad_mod,
        src/SOURCE/ses3d_grad.F90:33-37), see s3d_grad_add */
     S3D_GRAD_RHO = 20, S3D_GRAD_CS, S3D_GRAD_CP, S3D_GRAD_CSH, S3D_GRAD_CSV,
+    /* displacement gradient, symmetric part (reference snapshot attributes
+       exx .. ezz, src/volume_snapshot.f90:220-236), read only, after
+       s3d_enable_displacement_gradient */
+    S3D_EXX = 26, S3D_EXY, S3D_EXZ, S3D_EYY, S3D_EYZ, S3D_EZZ,
     S3D_MXX = 32   /* memory variable: S3D_MXX + 6*mechanism + comp,
                       comp order xx, xy, xz, yy, yz, zz; mechanism 0..2 */
 };
@@ -235,6 +239,17 @@ int s3d_set_field(s3d_ctx *ctx, int field, const double *in6d);
  * S3D_GRAD_CP, S3D_GRAD_CSH and S3D_GRAD_CSV read as zero. */
 int s3d_grad_add(s3d_ctx *ctx, const double *vx_fw6d, const double *vy_fw6d, const double *vz_fw6d,
                  int increment);
+
+/* Integrate the displacement gradient from the next step on (reference K3,
+ * src/evolution.f90:179-187, which the reference runs every step for its
+ * exx .. ezz snapshots, src/volume_snapshot.f90:220-236): each step adds
+ * dt times the symmetric part of the velocity gradient that the element
+ * kernel forms, in FP64 (6 x 8 B per element-local node, allocated and
+ * zeroed by the first call, kept until s3d_destroy; 96 B of extra traffic
+ * per node and step). Read with s3d_get_field(S3D_EXX .. S3D_EZZ). Call it
+ * before the first step to match the reference. Refused for a batch
+ * member. */
+int s3d_enable_displacement_gradient(s3d_ctx *ctx);
 
 /* min and max of vx, vy, vz (reference v_ptp_loc, src/elastic_vars.F90:179-185). */
 int s3d_vrange(s3d_ctx *ctx, double *out2);

@@ -23,6 +23,8 @@ EXX, EYY, EZZ, EXY, EXZ, EYZ, EYX, EZX, EZY = range(9, 18)
 MM, TAPER, THETA, PHI, R = 18, 19, 20, 21, 22
 SX, SY, SZ = 23, 24, 25
 MXX = 32
+# displacement gradient d_a u_b at DXUX + 3*a + b (dxux dxuy dxuz dyux ... dzuz)
+DXUX = 50
 RHOINV, LAMBDA, MU, A, B, C_, Q = range(7)
 CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6,
       "rot_vx": 7, "rot_vy": 8, "rot_vz": 9,
@@ -78,6 +80,7 @@ def lib():
         L.ora_coords_1d.argtypes = [C.c_void_p, dp, dp, dp]
         L.ora_profiles_1d.argtypes = [C.c_void_p, dp, dp, dp]
         L.ora_visco_enabled.argtypes = [C.c_void_p]
+        L.ora_enable_displacement_gradient.argtypes = [C.c_void_p]
         _LIB = L
     return _LIB
 
@@ -166,6 +169,11 @@ class Oracle:
 
     def evolution(self):
         lib().ora_evolution(self._h)
+
+    def enable_displacement_gradient(self):
+        """Integrate the displacement gradient (K3) from the next step on."""
+        if lib().ora_enable_displacement_gradient(self._h):
+            raise MemoryError("displacement gradient")
 
     def get(self, field):
         out = np.zeros(self.shape)

@@ -90,6 +90,7 @@ struct ora_ctx {
     int anisotropy, visco;
     /* state */
     double *v[3], *s[3], *sig[6], *e[9], *mem; /* mem: 6*NRDISS fields */
+    double *du[9];   /* displacement gradient (K3), ORA_DXUX order, or NULL */
     double *src[9];  /* xx xy xz yy yz zz x y z */
     double *work[6];
     int it;           /* iterations completed */
@@ -283,7 +284,7 @@ void ora_destroy(ora_ctx *c)
     for (size_t q = 0; q < sizeof(f) / sizeof(f[0]); ++q) free(f[q]);
     for (int q = 0; q < 3; ++q) { free(c->v[q]); free(c->s[q]); }
     for (int q = 0; q < 6; ++q) { free(c->sig[q]); free(c->work[q]); }
-    for (int q = 0; q < 9; ++q) { free(c->e[q]); free(c->src[q]); }
+    for (int q = 0; q < 9; ++q) { free(c->e[q]); free(c->src[q]); free(c->du[q]); }
     for (int q = 0; q < c->nsrc; ++q) { free(c->srcs[q].vm); free(c->srcs[q].data); }
     for (int q = 0; q < c->nrec; ++q) { free(c->recs[q].vm); free(c->recs[q].data); }
     free(c->srcs); free(c->recs);
@@ -730,6 +731,19 @@ int ora_evolution(ora_ctx *c)
                     eyz = -vyn / r + eyz / rs;
                     exy = exy / r;
                     eyx = -(vyn * ct) / r + eyx / rs;
+                    /* K3: integrate the velocity gradient to the displacement
+                     * gradient (src/evolution.f90:179-187), when requested */
+                    if (c->du[0]) {
+                        c->du[8][n] = c->du[8][n] + dt * ezz;   /* dzuz */
+                        c->du[4][n] = c->du[4][n] + dt * eyy;   /* dyuy */
+                        c->du[0][n] = c->du[0][n] + dt * exx;   /* dxux */
+                        c->du[6][n] = c->du[6][n] + dt * ezx;   /* dzux */
+                        c->du[2][n] = c->du[2][n] + dt * exz;   /* dxuz */
+                        c->du[7][n] = c->du[7][n] + dt * ezy;   /* dzuy */
+                        c->du[5][n] = c->du[5][n] + dt * eyz;   /* dyuz */
+                        c->du[1][n] = c->du[1][n] + dt * exy;   /* dxuy */
+                        c->du[3][n] = c->du[3][n] + dt * eyx;   /* dyux */
+                    }
                     /* K4: symmetrise (src/evolution.f90:206-208) */
                     exy = (exy + eyx) / 2.0;
                     exz = (exz + ezx) / 2.0;
@@ -981,8 +995,19 @@ static double *field_ptr(ora_ctx *c, int f)
     case ORA_SZ: return c->s[2];
     default:
         if (f >= ORA_MXX && f < ORA_MXX + 6 * NRDISS) return c->mem + (long)(f - ORA_MXX) * c->nn;
+        if (f >= ORA_DXUX && f < ORA_DXUX + 9) return c->du[f - ORA_DXUX];
         return NULL;
     }
+}
+
+int ora_enable_displacement_gradient(ora_ctx *c)
+{
+    for (int q = 0; q < 9; ++q)
+        if (!c->du[q]) {
+            c->du[q] = (double *)calloc((size_t)c->nn, sizeof(double));
+            if (!c->du[q]) return 1;
+        }
+    return 0;
 }
 
 int ora_get_field(ora_ctx *c, int f, double *out)

@@ -39,6 +39,10 @@ enum {
     ORA_SX, ORA_SY, ORA_SZ,           /* assembled acceleration of last step */
     ORA_MXX = 32, /* memory variables: ORA_MXX + 6*(mech) + comp, comp order
                      xx, xy, xz, yy, yz, zz, mech 0..2 */
+    ORA_DXUX = 50, /* displacement gradient (K3, src/evolution.f90:179-187):
+                      ORA_DXUX + 3*a + b = d_a u_b, a, b over x, y, z
+                      (dxux dxuy dxuz dyux dyuy dyuz dzux dzuy dzuz);
+                      after ora_enable_displacement_gradient */
     ORA_NFIELDS = 64
 };
 
@@ -112,6 +116,9 @@ int ora_step(ora_ctx *c, int n);
 int ora_evolution(ora_ctx *c);
 
 int ora_get_field(ora_ctx *c, int field, double *out6d);
+/* Integrate the displacement gradient from the next step on (zero at the
+ * call); the reference always integrates it (src/evolution.f90:179-187). */
+int ora_enable_displacement_gradient(ora_ctx *c);
 int ora_set_field(ora_ctx *c, int field, const double *in6d);
 /* Trace of receiver rec, channel ch: nt samples. */
 int ora_get_trace(ora_ctx *c, int rec, int ch, double *out);

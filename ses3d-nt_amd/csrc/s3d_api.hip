@@ -184,6 +184,7 @@ struct s3d_ctx {
          *mem = nullptr, *lam = nullptr, *mu = nullptr, *tau = nullptr, *rho = nullptr,
          *ca = nullptr, *cb = nullptr, *cc = nullptr, *force = nullptr, *tth = nullptr, *mat = nullptr,
          *grad_rho = nullptr, *grad_stage = nullptr,   // FP64 density; 3 snapshot components
+         *dgrad = nullptr,   // FP64 displacement gradient [nel][6][NP], see s3d_enable_displacement_gradient
          *tr = nullptr, *wdev = nullptr;
     double *mm64 = nullptr, *dpx = nullptr, *dpy = nullptr, *dpz = nullptr;
     double bytes = 0;
@@ -334,6 +335,8 @@ ElemArgs<R> elem_args(s3d_ctx *c)
         a.ca = (decltype(a.ca))((const R *)c->ca); a.cb = (decltype(a.cb))((const R *)c->cb); a.cc = (decltype(a.cc))((const R *)c->cc);
     }
     a.force = (decltype(a.force))((R *)c->force);
+    a.dgrad = (decltype(a.dgrad))(c->dgrad);
+    a.dt = c->g.dt;
     using C = coef_t<R>;
     a.tth = (decltype(a.tth))((const ThTab<C> *)c->tth);
     a.tr = (decltype(a.tr))((const RTab<C> *)c->tr);
@@ -1280,7 +1283,7 @@ int s3d_destroy(s3d_ctx *c)
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
     c->sig_lo = c->mem = nullptr;   // parts of the state block c->sig
     void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->sig,
-                    c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho, c->grad_stage,
+                    c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho, c->grad_stage, c->dgrad,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
                     c->d_rec_off, c->d_rec_ch, c->d_rec_vm, c->d_traces, c->d_rec_rot, c->d_rot_e, c->d_rot_buf};
@@ -1827,6 +1830,14 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
                      : with_p<ExpandRun, double>(c->p, c, (const void *)c->v[field], (const void *)nullptr, buf);
     } else if (field == S3D_MM) {
         rc = with_p<ExpandRun, double>(c->p, c, (const void *)c->mm64, (const void *)nullptr, buf);
+    } else if (field >= S3D_EXX && field <= S3D_EZZ) {
+        if (!c->dgrad) {
+            (void)hipFree(buf);
+            return fail("the displacement gradient is not integrated: call s3d_enable_displacement_gradient "
+                        "before stepping");
+        }
+        hipLaunchKernelGGL(from_device_layout<double>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
+                           (const double *)c->dgrad + (long)c->NP * (field - S3D_EXX), buf, c->nel, c->NP, 6);
     } else if ((field >= S3D_SXX && field <= S3D_SZZ) || (field >= S3D_MXX && field < S3D_MXX + 18)) {
         const void *src = nullptr;
         const void *lo = nullptr;
@@ -1951,6 +1962,18 @@ int s3d_grad_add(s3d_ctx *c, const double *vx_fw, const double *vy_fw, const dou
     hipError_t e = hipStreamSynchronize(c->stream);
     if (rc) return rc;
     if (e != hipSuccess) return fail(std::string("grad_add: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int s3d_enable_displacement_gradient(s3d_ctx *c)
+{
+    if (!c) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    if (c->batches > 0) return refuse_batch_member(c, "enable_displacement_gradient");
+    if (c->dgrad) return 0;
+    HIPCHK(hipSetDevice(c->dev));
+    if (dalloc(c, &c->dgrad, sizeof(double) * 6 * c->nn)) return 1;
+    drop_graph(c);   // the captured launches carry the element kernel's arguments
     return 0;
 }
 

@@ -114,6 +114,8 @@ struct ElemArgs {
     gptr<const R> ca, cb, cc;  // anisotropy dt*A, dt*B, dt*C or null
     Static16 s16;              // the same fields in BF16 when lam is null
     gptr<R> force;             // [3][nel][NP]
+    gptr<double> dgrad;        // [nel][6][NP] displacement gradient (xx xy xz yy yz zz) or null
+    double dt;                 // time step (displacement gradient increment)
     gptr<const ThTab<coef_t<R>>> tth;   // [nx*P1]
     gptr<const RTab<coef_t<R>>> tr;     // [nz*P1]
     gptr<const double> px, py, pz;  // 1-D taper profiles [n*P1]

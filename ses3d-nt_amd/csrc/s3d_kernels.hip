@@ -381,12 +381,31 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             const C eyz_c = -uy * inv_r + dyv[2] * inv_rs;
             const C exy_c = dxv[1] * inv_r;
             const C eyx = -uy * cot * inv_r + dyv[0] * inv_rs;
-            exx = (R)((uz + dxv[0]) * inv_r);
-            eyy = (R)(dyv[1] * inv_rs + (uz + ux * cot) * inv_r);
-            ezz = (R)(-dzv[2]);
-            exy = (R)((exy_c + eyx) * 0.5);
-            exz = (R)((exz_c + ezx) * 0.5);
-            eyz = (R)((eyz_c + ezy) * 0.5);
+            const C gxx = (uz + dxv[0]) * inv_r;
+            const C gyy = dyv[1] * inv_rs + (uz + ux * cot) * inv_r;
+            const C gzz = -dzv[2];
+            const C gxy = (exy_c + eyx) * 0.5;
+            const C gxz = (exz_c + ezx) * 0.5;
+            const C gyz = (eyz_c + ezy) * 0.5;
+            exx = (R)gxx;
+            eyy = (R)gyy;
+            ezz = (R)gzz;
+            exy = (R)gxy;
+            exz = (R)gxz;
+            eyz = (R)gyz;
+            // K3: the displacement gradient integrates the velocity gradient
+            // (src/evolution.f90:179-187); kept as its symmetric part, the
+            // only form the reference outputs (src/volume_snapshot.f90:220-236),
+            // in FP64, and only when requested
+            if (a.dgrad && valid) {
+                double *g = a.dgrad + em_index(ev, 6, 0, NP, node);
+                g[0 * NP] += a.dt * (double)gxx;
+                g[1 * NP] += a.dt * (double)gxy;
+                g[2 * NP] += a.dt * (double)gxz;
+                g[3 * NP] += a.dt * (double)gyy;
+                g[4 * NP] += a.dt * (double)gyz;
+                g[5 * NP] += a.dt * (double)gzz;
+            }
         }
 
         // ---- K5: stress rate times dt (lam, mu, A, B, C carry the dt) ----

@@ -19,8 +19,10 @@ PROGRAM ses3d_nt
                               write_model_parameters
     USE channels_mod, ONLY : source_cls, receiver_cls, parse_source_file, parse_receiver_file, &
                              count_receiver_channels
-    USE snapshots_mod, ONLY : write_snapshots, check_snapshot_attributes, write_netcdf
+    USE snapshots_mod, ONLY : write_snapshots, check_snapshot_attributes, write_netcdf, &
+                              needs_displacement_gradient
     USE grad_mod, ONLY : ses3d_grad
+    USE ses3d_hip_c, ONLY : check, s3d_enable_displacement_gradient
     IMPLICIT NONE
     CHARACTER(LEN=512) :: conf_file_name
     TYPE(source_cls), ALLOCATABLE :: sources(:)
@@ -57,6 +59,8 @@ PROGRAM ses3d_nt
     ELSE
         CALL check_snapshot_attributes()
         CALL init_temp()
+        IF (needs_displacement_gradient()) &
+            CALL check(s3d_enable_displacement_gradient(ctx), 'enable_displacement_gradient')
         CALL parse_receiver_file(ctx, receivers)
         WRITE(OUTPUT_UNIT, '("Found ", I0, " receivers.")') SIZE(receivers)
         CALL parse_source_file(ctx, sources)

@@ -8,8 +8,9 @@
 !! sensitivity densities grad_rho grad_cs grad_cp grad_csh grad_csv
 !! (src/volume_snapshot.f90:281-293), and the model: rho lambda mu vs/cs
 !! vp/cp a b c q (:254-279), rebuilt on the host from the model files or the
-!! built-in model. Not provided: the displacement-gradient fields exx..ezz
-!! (not integrated on the GPU) and volume curl fields.
+!! built-in model, and the displacement gradient exx exy exz eyy eyz ezz
+!! (:220-236; integrated on the device when a snapshot asks for it). Not
+!! provided: the volume curl fields rot_vx, rot_vy, rot_vz.
 MODULE snapshots_mod
     USE, INTRINSIC :: ISO_C_BINDING
     USE ses3d_hip_c
@@ -17,7 +18,7 @@ MODULE snapshots_mod
     USE evolution_mod, ONLY : model_param
     IMPLICIT NONE
     PRIVATE
-    PUBLIC :: write_snapshots, check_snapshot_attributes, write_netcdf
+    PUBLIC :: write_snapshots, check_snapshot_attributes, write_netcdf, needs_displacement_gradient
 
 CONTAINS
 
@@ -46,7 +47,8 @@ CONTAINS
                 SELECT CASE (TRIM(configuration%outputs(i)%attributes(j)))
                 CASE ('', 'vx', 'vy', 'vz', 'sxx', 'sxy', 'sxz', 'syy', 'syz', 'szz', 'pressure', 'taper', &
                       'grad_rho', 'grad_cs', 'grad_cp', 'grad_csh', 'grad_csv', 'rho', 'lambda', 'mu', &
-                      'vs', 'cs', 'vp', 'cp', 'a', 'b', 'c', 'q')
+                      'vs', 'cs', 'vp', 'cp', 'a', 'b', 'c', 'q', &
+                      'exx', 'exy', 'exz', 'eyy', 'eyz', 'ezz')
                 CASE DEFAULT
                     CALL abort('Snapshot attribute not supported on the GPU path: ' // &
                                TRIM(configuration%outputs(i)%attributes(j)))
@@ -54,6 +56,21 @@ CONTAINS
             END DO
         END DO
     END SUBROUTINE
+
+    !> Whether a snapshot asks for the displacement gradient (exx .. ezz),
+    !! which the device then integrates from the first step on.
+    LOGICAL FUNCTION needs_displacement_gradient()
+        INTEGER :: i, j
+        needs_displacement_gradient = .FALSE.
+        DO i = 1, SIZE(configuration%outputs)
+            DO j = 1, SIZE(configuration%outputs(i)%attributes)
+                SELECT CASE (TRIM(configuration%outputs(i)%attributes(j)))
+                CASE ('exx', 'exy', 'exz', 'eyy', 'eyz', 'ezz')
+                    needs_displacement_gradient = .TRUE.
+                END SELECT
+            END DO
+        END DO
+    END FUNCTION
 
     !> rho = 1/rhoinv where rhoinv > 0, else 0 (src/model_paras.F90:398-402)
     SUBROUTINE rho_of(f)
@@ -100,6 +117,13 @@ CONTAINS
                 CASE ('syz'); CALL get(ctx, S3D_SYZ, f)
                 CASE ('szz'); CALL get(ctx, S3D_SZZ, f)
                 CASE ('taper'); CALL get(ctx, S3D_TAPER, f)
+                ! displacement gradient, symmetric part (src/volume_snapshot.f90:220-236)
+                CASE ('exx'); CALL get(ctx, S3D_EXX, f)
+                CASE ('exy'); CALL get(ctx, S3D_EXY, f)
+                CASE ('exz'); CALL get(ctx, S3D_EXZ, f)
+                CASE ('eyy'); CALL get(ctx, S3D_EYY, f)
+                CASE ('eyz'); CALL get(ctx, S3D_EYZ, f)
+                CASE ('ezz'); CALL get(ctx, S3D_EZZ, f)
                 CASE ('rho')
                     CALL model_param(1, f)
                     CALL rho_of(f)

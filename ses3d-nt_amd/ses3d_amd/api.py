@@ -27,6 +27,8 @@ VX, VY, VZ = 0, 1, 2
 SXX, SXY, SXZ, SYY, SYZ, SZZ = 3, 4, 5, 6, 7, 8
 MM, TAPER = 18, 19
 GRAD_RHO, GRAD_CS, GRAD_CP, GRAD_CSH, GRAD_CSV = 20, 21, 22, 23, 24
+# symmetric displacement gradient (after enable_displacement_gradient)
+EXX, EXY, EXZ, EYY, EYZ, EZZ = 26, 27, 28, 29, 30, 31
 MXX = 32
 CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6,
       "rot_vx": 7, "rot_vy": 8, "rot_vz": 9,
@@ -43,7 +45,7 @@ EXPORTS = [
     "s3d_synchronize", "s3d_timing_enable", "s3d_kernel_times",
     "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step", "s3d_set_tile",
     "s3d_batch_synchronize", "s3d_batch_destroy", "s3d_grad_add", "s3d_vrange_noalloc",
-    "s3d_set_lds_layout", "s3d_get_tuning",
+    "s3d_set_lds_layout", "s3d_get_tuning", "s3d_enable_displacement_gradient",
 ]
 
 
@@ -103,6 +105,7 @@ def lib():
         L.s3d_get_field.argtypes = [vp, C.c_int, dp]
         L.s3d_set_field.argtypes = [vp, C.c_int, dp]
         L.s3d_grad_add.argtypes = [vp, dp, dp, dp, C.c_int]
+        L.s3d_enable_displacement_gradient.argtypes = [vp]
         L.s3d_vrange.argtypes = [vp, dp]
         L.s3d_vrange_noalloc.argtypes = [vp, dp]
         L.s3d_kinetic_energy.argtypes = [vp, C.POINTER(C.c_double)]
@@ -286,6 +289,11 @@ class Simulation:
 
     def set(self, field, arr):
         _chk(lib().s3d_set_field(self._h, field, np.ascontiguousarray(arr, np.float64).reshape(-1)))
+
+    def enable_displacement_gradient(self):
+        """Integrate the displacement gradient from the next step on
+        (s3d_enable_displacement_gradient); read it with get(EXX .. EZZ)."""
+        _chk(lib().s3d_enable_displacement_gradient(self._h))
 
     def grad_add(self, vx_fw, vy_fw, vz_fw, increment):
         """grad_rho += increment*dt*(v_fw . v)/J with a forward velocity

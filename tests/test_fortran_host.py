@@ -54,7 +54,7 @@ def test_forward_run_matches_oracle(tmp_path, precision, tol):
     nt = 200
     case = synth.homogeneous(nt=nt, nx=8, ny=8, nz=5, n_rec=5)
     snap = ("\n&output_raw\n timestep_start = 100\n timestep_increment = 100\n"
-            " timestep_end = 200\n attributes = 'vz' 'pressure'\n override = .TRUE.\n"
+            " timestep_end = 200\n attributes = 'vz' 'pressure' 'EXY' 'ezz'\n override = .TRUE.\n"
             " prefix = '%s/'\n/\n" % str(tmp_path))
     cfg = write_config(case, str(tmp_path), precision=precision, extra=snap)
     r = run(cfg)
@@ -62,11 +62,18 @@ def test_forward_run_matches_oracle(tmp_path, precision, tol):
     assert r.returncode == 0
     assert "successfully completed" in r.stdout
     o, recs = oracle.build_case(case)
+    o.enable_displacement_gradient()
     o.step(100)
     dtype = "<f4" if precision == "fp32" else "<f8"
     vz100 = np.fromfile(os.path.join(str(tmp_path), "vz_100.vol"), dtype=dtype)
     assert rel_l2(vz100, o.get(oracle.VZ).ravel()) < tol
+    # exy = (dxuy + dyux) / 2 of the integrated gradient (src/volume_snapshot.f90:223-224)
+    exy100 = np.fromfile(os.path.join(str(tmp_path), "exy_100.vol"), dtype=dtype)
+    dxuy, dyux = o.get(oracle.DXUX + 1), o.get(oracle.DXUX + 3)
+    assert rel_l2(exy100, ((dxuy + dyux) / 2).ravel()) < tol
     o.step(nt - 100)
+    ezz200 = np.fromfile(os.path.join(str(tmp_path), "ezz_200.vol"), dtype=dtype)
+    assert rel_l2(ezz200, o.get(oracle.DXUX + 8).ravel()) < tol
     p200 = np.fromfile(os.path.join(str(tmp_path), "pressure_200.vol"), dtype=dtype)
     pref = -(o.get(oracle.SXX) + o.get(oracle.SYY) + o.get(oracle.SZZ)).ravel() / 3.0
     assert rel_l2(p200, pref) < 10 * tol

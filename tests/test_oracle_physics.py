@@ -104,6 +104,43 @@ def test_radial_shear_matches_textbook_rate():
     assert np.max(np.abs(exx)) < 1e-12 * np.max(a * r)
 
 
+def test_displacement_gradient_integrates_the_velocity_gradient():
+    # With zero moduli the stress, the force and so the velocity stay as
+    # set, and the displacement gradient after n steps is n dt times the
+    # velocity gradient in the reference's spherical form. The field
+    # v_theta = a r^2, v_phi = b r^2 theta, v_r = c r phi is differentiated
+    # exactly by the GLL basis and gives nine different components:
+    #   d_x u_x = (v_r + d_theta v_theta) / r           = c phi
+    #   d_x u_y = d_theta v_phi / r                      = b r
+    #   d_x u_z = (d_theta v_r - v_theta) / r            = -a r
+    #   d_y u_x = -v_phi cot / r + d_phi v_theta / (r sin) = -b r theta cot
+    #   d_y u_y = d_phi v_phi / (r sin) + (v_r + v_theta cot) / r = c phi + a r cot
+    #   d_y u_z = -v_phi / r + d_phi v_r / (r sin)       = -b r theta + c / sin
+    #   d_z u_x = d_r v_theta = 2 a r, d_z u_y = 2 b r theta, d_z u_z = c phi
+    # so a transposed or dropped component or a wrong time factor fails.
+    g = small_grid(nx=4, ny=3, nz=3, lpd=4, taper_width=0, nt=10)
+    o = oracle.Oracle(g)
+    o.set_model(oracle.RHOINV, np.full(o.shape, 1.0 / 2700.0))
+    o.set_model(oracle.LAMBDA, np.zeros(o.shape))
+    o.set_model(oracle.MU, np.zeros(o.shape))
+    o.init()
+    o.enable_displacement_gradient()
+    th, ph, r = node_coords(o)
+    a, b, c = 2.6e-14, 1.9e-14, 6.0e-7
+    vt, vp_, vr = a * r * r, b * r * r * th, c * r * ph
+    _set_v(o, vt, vp_, vr)
+    n = 7
+    o.step(n)
+    assert np.array_equal(o.get(oracle.VY), vp_)
+    t = n * g.dt
+    cot, sin = 1.0 / np.tan(th), np.sin(th)
+    expect = {"xx": c * ph, "xy": b * r, "xz": -a * r,
+              "yx": -b * r * th * cot, "yy": c * ph + a * r * cot, "yz": -b * r * th + c / sin,
+              "zx": 2 * a * r, "zy": 2 * b * r * th, "zz": c * ph}
+    for q, name in enumerate(("xx", "xy", "xz", "yx", "yy", "yz", "zx", "zy", "zz")):
+        assert rel_l2(o.get(oracle.DXUX + q), expect[name] * t) < 1e-11, name
+
+
 def _force(o):
     """assembled force per unique node: MM * s (s = acceleration)"""
     mm = local_to_unique(o.get(oracle.MM), o.grid)
