@@ -94,13 +94,17 @@ def elem_kernel_bytes(mode, u, visco):
     return s * (12 + 36 * visco + 3 * u) + ls * 12 + st * (2 + visco)
 
 
-def load_sims(case, modes, slab, remap=True, tune=None):
+def load_sims(case, modes, slab, remap=True, tune=None, state_buffers=0):
     """Build one Simulation per precision mode from a single pass over the
     model slabs; returns a list of (mode, sim). tune = (tile, LDS layout)
     sets the FP32 simulation's element tile and layout instead of the
-    autotuner."""
+    autotuner. The first mode gets `state_buffers`; an FP64 companion of an
+    FP32 mode updates its state in place (one state buffer), so that both
+    fit on one GPU at 192x192x96."""
     g = case.grid
-    sims = [(name, api.Simulation(g, xcd_remap=remap, **MODES[name])) for name in modes]
+    sims = [(name, api.Simulation(g, xcd_remap=remap,
+                                  state_buffers=state_buffers if q == 0 else 1, **MODES[name]))
+            for q, name in enumerate(modes)]
     t0 = time.time()
     for ez0 in range(0, g.nz, slab):
         nez = min(slab, g.nz - ez0)
@@ -381,12 +385,14 @@ def multi_source_bench(args):
     kw = dict(MODES[mode])
     if kw.pop("bf16_static"):
         raise SystemExit("BF16 static fields are not supported in a batch")
+    kw["state_buffers"] = args.state_buffers
     case = synth.multi_source(seed=5, n_src=args.sources, nt=args.warmup + 2 * args.steps + 1)
     g = case.grid
     N = g.nodes
     S = len(case.sources)
     u = g.unique_nodes / N
-    per = max(1, min(args.per_batch, S))
+    per_default = 16 if args.state_buffers == 1 else 8
+    per = max(1, min(args.per_batch or per_default, S))
     t0 = time.time()
     owner, recs = api.build_case(replace(case, sources=[]), slab=args.slab, autotune=True, **kw)
     log("[multi-source] %dx%dx%d p=%d, %d sources in batches of %d, %d receivers, %d GLL nodes "
@@ -432,6 +438,7 @@ def multi_source_bench(args):
                        "elements, GLL order %d, %s, batches of %d" % (S, g.nx, g.ny, g.nz, g.lpd, mode,
                                                                      per),
            "gll_nodes_per_source": N, "sources": S, "per_batch": per, "steps": args.steps,
+           "state_buffers": 1 if args.state_buffers == 1 else 2,
            "batched_shared": {"value": value, "ms_per_step": 1e3 * wall_b / args.steps},
            "round_robin": {"value": N * S * args.steps / wall_r,
                            "ms_per_step": 1e3 * wall_r / args.steps},
@@ -471,8 +478,11 @@ def main():
                     help="many sources sharing one model (128x128x48, p=4) instead of the single mesh")
     ap.add_argument("--sources", type=int, default=32,
                     help="sources sharing the model (config: 32)")
-    ap.add_argument("--per-batch", type=int, default=16,
-                    help="sources per shared-model batch (16 fit in HBM at 128x128x48)")
+    ap.add_argument("--per-batch", type=int, default=None,
+                    help="sources per shared-model batch (default: what fits in HBM at "
+                         "128x128x48, 8 with two state buffers, 16 with one)")
+    ap.add_argument("--state-buffers", type=int, default=0, choices=(0, 1, 2),
+                    help="state blocks per simulation: 0 library default (2), 1 in place")
     ap.add_argument("--precision-study", action="store_true",
                     help="FP32 variants vs FP64 seismogram / snapshot misfit instead of timing")
     ap.add_argument("--study-steps", type=int, default=4000)
@@ -523,7 +533,8 @@ def main():
     mode = args.mode
     modes = [mode] + ([] if args.no_fp64 or mode == "fp64" else ["fp64"])
     tune = None if args.tile is None else (args.tile, args.lds_layout)
-    sims = dict(load_sims(case, modes, args.slab, remap=not args.no_remap, tune=tune))
+    sims = dict(load_sims(case, modes, args.slab, remap=not args.no_remap, tune=tune,
+                          state_buffers=args.state_buffers))
     tile = sims[mode].tile
     wall, ms, cnt, vr = time_gpu(sims[mode], args.steps, args.warmup)
     value = N * args.steps / wall
@@ -572,7 +583,8 @@ def main():
                                "one source, 256 surface receivers and 32 on rings around "
                                "the source" % (g.nx, g.ny, g.nz, MODE_TEXT[mode]),
                    "elements": g.n_elem, "gll_nodes": N, "unique_nodes": g.unique_nodes,
-                   "precision": mode, "parallelism": "single GPU"},
+                   "precision": mode, "parallelism": "single GPU",
+                   "state_buffers": 1 if args.state_buffers == 1 else 2},
         "roofline": roof,
         "step_roofline": {"bytes_per_node": Bstep, "achieved_GBps": Bstep * value / 1e9,
                           "frac": step_frac,
@@ -585,7 +597,8 @@ def main():
         out["fp64_gpu"] = {"value": v64, "ms_per_step": 1e3 * wall64 / args.steps,
                            "elem_ms": ms64[0] / max(cnt64[0], 1),
                            "node_ms": ms64[1] / max(cnt64[1], 1),
-                           "speedup_fp32_over_fp64": value / v64}
+                           "speedup_fp32_over_fp64": value / v64,
+                           "state_buffers": 1}
         log("[bench] fp64 %.3f ms/step" % (1e3 * wall64 / args.steps))
         if misfit_steps > 0:
             t0 = time.time()

@@ -49,6 +49,11 @@ typedef struct s3d_options {
     int device;            /* HIP device ordinal                          */
     int xcd_remap;         /* 1: give each XCD a contiguous element range */
     int bf16_static;       /* FP32: store the moduli, tau and dt/M in BF16 */
+    int state_buffers;     /* stress, compensation and memory variables:
+                              2 (and 0, the default): two blocks, each step
+                              reads one and writes the other, about 10 %
+                              faster element kernel for the state's size
+                              again in memory; 1: one block updated in place */
 } s3d_options;
 
 /* Model parameter ids (reference src/model_paras.F90:39-46). */

@@ -211,11 +211,17 @@ struct s3d_ctx {
     int nrec_dev = 0;
     int it = 0;
     int *d_it = nullptr;       // device copy of it, advanced by advance_kernel
-    // captured chunk of steps (hipGraph)
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_steps = 0;
+    // captured chunk of steps (hipGraph), one per starting state block
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    int graph_steps[2] = {0, 0};
     bool use_graph = true;
+    // state blocks: with two (nbuf 2) the element kernel reads blk[cur] and
+    // writes blk[1 - cur], and cur flips every step; with one it updates
+    // blk[0] in place. sig, sig_lo and mem view blk[cur].
+    void *blk[2] = {nullptr, nullptr};
+    int nbuf = 2;
+    int cur = 0;
     int epb = 0;               // element tile (0 = default), see s3d_autotune
     int lds_layout = 0;        // LDS staging layout of the FP32 element kernel, see Stage
     bool fields_set = false;   // s3d_set_field was called (state not at rest)
@@ -258,6 +264,34 @@ size_t rsize(const s3d_ctx *c);
 
 // Per-simulation state: velocity (+ compensation), stress (+ compensation),
 // memory variables, element force and the device step counter, zeroed.
+// The parts of state block k: stress, compensation words, memory variables.
+struct StateParts {
+    char *sig, *lo, *mem;
+};
+
+StateParts state_parts(const s3d_ctx *c, int k)
+{
+    const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+    const int ncl = c->ns - 6 - (c->visco ? 18 : 0);
+    char *b = (char *)c->blk[k];
+    return {b, ncl ? b + rs * 6 * c->NP : nullptr, c->visco ? b + rs * (6 + ncl) * c->NP : nullptr};
+}
+
+void set_state_block(s3d_ctx *c, int k)
+{
+    c->cur = k;
+    const StateParts s = state_parts(c, k);
+    c->sig = s.sig;
+    c->sig_lo = s.lo;
+    c->mem = s.mem;
+}
+
+// the block the next step writes
+int next_block(const s3d_ctx *c) { return c->nbuf == 2 ? 1 - c->cur : c->cur; }
+
+// after a step's element kernel: its output block becomes the state
+void advance_state(s3d_ctx *c) { set_state_block(c, next_block(c)); }
+
 int alloc_state(s3d_ctx *c)
 {
     const size_t rs = rsize(c);
@@ -270,10 +304,9 @@ int alloc_state(s3d_ctx *c)
     // FP64 has no compensation); sig_lo and mem point at their first
     // components and are not separate allocations
     c->ns = state_components(c->o.kahan_sigma, c->visco);
-    if (dalloc(c, &c->sig, rs * c->ns * c->nn)) return 1;
-    const int ncl = c->ns - 6 - (c->visco ? 18 : 0);
-    c->sig_lo = ncl ? (char *)c->sig + rs * 6 * c->NP : nullptr;
-    c->mem = c->visco ? (char *)c->sig + rs * (6 + ncl) * c->NP : nullptr;
+    for (int k = 0; k < c->nbuf; ++k)
+        if (dalloc(c, &c->blk[k], rs * c->ns * c->nn)) return 1;
+    set_state_block(c, 0);
     if (dalloc(c, &c->force, rs * 3 * c->nn, false)) return 1;
     if (dalloc(c, (void **)&c->d_it, sizeof(int))) return 1;
     return 0;
@@ -309,9 +342,13 @@ size_t rsize(const s3d_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double)
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ---- template dispatch helpers ----
+// Element kernel arguments for a step that reads state block rd and writes
+// block wr (by default the current block and the next step's output block).
 template <int P, typename R>
-ElemArgs<R> elem_args(s3d_ctx *c)
+ElemArgs<R> elem_args(s3d_ctx *c, int rd = -1, int wr = -1)
 {
+    if (rd < 0) rd = c->cur;
+    if (wr < 0) wr = next_block(c);
     ElemArgs<R> a;
     std::memset(&a, 0, sizeof(a));
     const int P1 = P + 1;
@@ -321,12 +358,18 @@ ElemArgs<R> elem_args(s3d_ctx *c)
     a.nt = c->g.nt;
     a.kahan_sigma = c->o.kahan_sigma;
     a.vx = (decltype(a.vx))((const R *)c->v[0]); a.vy = (decltype(a.vy))((const R *)c->v[1]); a.vz = (decltype(a.vz))((const R *)c->v[2]);
-    a.sig = (decltype(a.sig))((R *)c->sig);
-    if (c->o.kahan_sigma == 2)
-        a.sig_lo2 = (decltype(a.sig_lo2))((unsigned *)c->sig_lo);
-    else
-        a.sig_lo = (decltype(a.sig_lo))((R *)c->sig_lo);
-    a.mem = (decltype(a.mem))((R *)c->mem);
+    const StateParts in = state_parts(c, rd), out = state_parts(c, wr);
+    a.sig = (decltype(a.sig))((const R *)in.sig);
+    a.sig_w = (decltype(a.sig_w))((R *)out.sig);
+    if (c->o.kahan_sigma == 2) {
+        a.sig_lo2 = (decltype(a.sig_lo2))((const unsigned *)in.lo);
+        a.sig_lo2_w = (decltype(a.sig_lo2_w))((unsigned *)out.lo);
+    } else {
+        a.sig_lo = (decltype(a.sig_lo))((const R *)in.lo);
+        a.sig_lo_w = (decltype(a.sig_lo_w))((R *)out.lo);
+    }
+    a.mem = (decltype(a.mem))((const R *)in.mem);
+    a.mem_w = (decltype(a.mem_w))((R *)out.mem);
     if (c->o.bf16_static) {
         a.s16.lam = (decltype(a.s16.lam))((const bf16 *)c->lam); a.s16.mu = (decltype(a.s16.mu))((const bf16 *)c->mu); a.s16.tau = (decltype(a.s16.tau))((const bf16 *)c->tau);
         a.s16.ca = (decltype(a.s16.ca))((const bf16 *)c->ca); a.s16.cb = (decltype(a.s16.cb))((const bf16 *)c->cb); a.s16.cc = (decltype(a.s16.cc))((const bf16 *)c->cc);
@@ -577,6 +620,7 @@ int launch_step(s3d_ctx *c)
     ev_begin(c, 0);
     if (int rc = launch_elem<P, R>(c, ea, c->stream, c->epb)) return rc;
     ev_end(c, 0);
+    advance_state(c);   // the receivers read the stress this step wrote
     NodeArgs<R> na = node_args<P, R>(c);
     const unsigned gn = nblocks(c->nuniq, 256);
     ev_begin(c, 1);
@@ -614,21 +658,30 @@ int dispatch_step(s3d_ctx *c)
 
 int launch_one(s3d_ctx *c) { return c->fp32 ? dispatch_step<float>(c) : dispatch_step<double>(c); }
 
-void drop_graph(s3d_ctx *c)
+void drop_graph_slot(s3d_ctx *c, int k)
 {
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    c->graph_exec = nullptr;
-    c->graph = nullptr;
-    c->graph_steps = 0;
+    if (c->graph_exec[k]) (void)hipGraphExecDestroy(c->graph_exec[k]);
+    if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+    c->graph_exec[k] = nullptr;
+    c->graph[k] = nullptr;
+    c->graph_steps[k] = 0;
 }
 
-// Capture `chunk` consecutive iterations into one graph. The kernels read the
-// iteration number from the device counter, so the same graph replays any
-// chunk of the run.
+void drop_graph(s3d_ctx *c)
+{
+    drop_graph_slot(c, 0);
+    drop_graph_slot(c, 1);
+}
+
+// Capture `chunk` (even) consecutive iterations into one graph, for the
+// current state block: an even number of steps ends on the block it started
+// from, so the graph replays whenever that block is current. The kernels
+// read the iteration number from the device counter, so the same graph
+// replays any chunk of the run.
 int capture_graph(s3d_ctx *c, int chunk)
 {
-    drop_graph(c);
+    const int k = c->cur;
+    drop_graph_slot(c, k);
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
     for (int s = 0; s < chunk && rc == 0; ++s) rc = launch_one(c);
@@ -636,16 +689,20 @@ int capture_graph(s3d_ctx *c, int chunk)
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
+        set_state_block(c, k);
         return rc;
     }
-    if (e != hipSuccess) return fail(std::string("graph capture: ") + hipGetErrorString(e));
-    c->graph = g;
-    e = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        drop_graph(c);
+        set_state_block(c, k);
+        return fail(std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    c->graph[k] = g;
+    e = hipGraphInstantiate(&c->graph_exec[k], g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        drop_graph_slot(c, k);
         return fail(std::string("graph instantiate: ") + hipGetErrorString(e));
     }
-    c->graph_steps = chunk;
+    c->graph_steps[k] = chunk;
     return 0;
 }
 
@@ -909,7 +966,12 @@ struct BatchGroup {
     int ks = 0;                 // stress compensation mode
     bool fp32 = false, visco = false, kahan = false;
     std::vector<s3d_ctx *> sims;
+    // element and receiver arguments in two sets of sims.size() entries: set
+    // `phase` serves the batch's even (0) or odd (1) steps since it was
+    // built, when members with two state buffers read and write the other
+    // block of their pair
     void *d_eargs = nullptr, *d_nargs = nullptr, *d_rargs = nullptr;
+    int phase = 0;
     // per workgroup of each batched launch: (simulation, its workgroup index)
     int2 *d_eown = nullptr, *d_nown = nullptr, *d_rown = nullptr;
     int ewg = 0, nwg = 0, rwg = 0;
@@ -922,9 +984,11 @@ struct s3d_batch {
     hipStream_t stream = nullptr;
     int **d_its = nullptr;
     bool built = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_steps = 0;
+    // captured chunk of steps, one per phase (see BatchGroup)
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    int graph_steps[2] = {0, 0};
+    int phase = 0;
     hipEvent_t ev = nullptr;
 };
 
@@ -935,17 +999,27 @@ int batch_build_group_e(BatchGroup &G)
 {
     using S = Shape<P, E>;
     const int n = (int)G.sims.size();
-    std::vector<ElemArgs<R>> ea(n);
+    std::vector<ElemArgs<R>> ea(2 * n);
     std::vector<NodeArgs<R>> na(n);
-    std::vector<RecArgs<R>> ra(n);
+    std::vector<RecArgs<R>> ra(2 * n);
     std::vector<int> es(n + 1, 0), ns(n + 1, 0), rs(n + 1, 0);
     for (int q = 0; q < n; ++q) {
         s3d_ctx *c = G.sims[q];
-        ea[q] = elem_args<P, R>(c);
-        ea[q].remap = 0;
+        // even steps read the current block and write the next one; odd
+        // steps read that one and write the block after it
+        const int r0 = c->cur, w0 = next_block(c);
+        const int r1 = w0, w1 = c->nbuf == 2 ? r0 : w0;
+        ea[q] = elem_args<P, R>(c, r0, w0);
+        ea[n + q] = elem_args<P, R>(c, r1, w1);
+        ea[q].remap = ea[n + q].remap = 0;
         na[q] = node_args<P, R>(c);
-        ra[q] = rec_args<R>(c);
-        ra[q].rec_rot = nullptr;   // no curl channels in a batch (refused at creation)
+        for (int h = 0; h < 2; ++h) {
+            RecArgs<R> &r = ra[h * n + q];
+            r = rec_args<R>(c);
+            // the receivers read the stress their step wrote
+            r.sig = (decltype(r.sig))((const R *)state_parts(c, h ? w1 : w0).sig);
+            r.rec_rot = nullptr;   // no curl channels in a batch (refused at creation)
+        }
         es[q + 1] = es[q] + (int)nblocks(c->nel, S::EPB);
         ns[q + 1] = ns[q] + (int)nblocks(c->nuniq, 256);
         rs[q + 1] = rs[q] + c->nrec_dev;
@@ -978,15 +1052,15 @@ int batch_build_group_e(BatchGroup &G)
     const std::vector<int2> eo = owners(es, 1, interleave), no = owners(ns, 0, interleave),
                             ro = owners(rs, 0, false);
     G.ewg = (int)eo.size();
-    HIPCHK(hipMalloc(&G.d_eargs, sizeof(ElemArgs<R>) * n));
+    HIPCHK(hipMalloc(&G.d_eargs, sizeof(ElemArgs<R>) * 2 * n));
     HIPCHK(hipMalloc(&G.d_nargs, sizeof(NodeArgs<R>) * n));
-    HIPCHK(hipMalloc(&G.d_rargs, sizeof(RecArgs<R>) * n));
+    HIPCHK(hipMalloc(&G.d_rargs, sizeof(RecArgs<R>) * 2 * n));
     HIPCHK(hipMalloc(&G.d_eown, sizeof(int2) * (eo.size() + 1)));
     HIPCHK(hipMalloc(&G.d_nown, sizeof(int2) * (no.size() + 1)));
     HIPCHK(hipMalloc(&G.d_rown, sizeof(int2) * (ro.size() + 1)));
-    HIPCHK(hipMemcpy(G.d_eargs, ea.data(), sizeof(ElemArgs<R>) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_eargs, ea.data(), sizeof(ElemArgs<R>) * 2 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(G.d_nargs, na.data(), sizeof(NodeArgs<R>) * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(G.d_rargs, ra.data(), sizeof(RecArgs<R>) * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(G.d_rargs, ra.data(), sizeof(RecArgs<R>) * 2 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(G.d_eown, eo.data(), sizeof(int2) * eo.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(G.d_nown, no.data(), sizeof(int2) * no.size(), hipMemcpyHostToDevice));
     if (!ro.empty()) HIPCHK(hipMemcpy(G.d_rown, ro.data(), sizeof(int2) * ro.size(), hipMemcpyHostToDevice));
@@ -1000,13 +1074,14 @@ int batch_launch_group_k(const BatchGroup &G, hipStream_t st)
     if constexpr (std::is_same<R, float>::value) {
         if (G.ll == 1) {
             hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 1>), dim3(G.ewg), dim3(S::NT), 0, st,
-                               (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
+                               (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size(),
+                               (const int2 *)G.d_eown);
             HIPCHK(hipGetLastError());
             return 0;
         }
     }
     hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 0>), dim3(G.ewg), dim3(S::NT), 0, st,
-                       (const ElemArgs<R> *)G.d_eargs, (const int2 *)G.d_eown);
+                       (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size(), (const int2 *)G.d_eown);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1061,7 +1136,8 @@ int batch_launch_node(const BatchGroup &G, hipStream_t st)
     HIPCHK(hipGetLastError());
     if (G.rwg > 0) {
         hipLaunchKernelGGL((record_kernel_batch<P, R>), dim3(G.rwg), dim3(64), 0, st,
-                           (const RecArgs<R> *)G.d_rargs, (const int2 *)G.d_rown);
+                           (const RecArgs<R> *)G.d_rargs + (size_t)G.phase * G.sims.size(),
+                           (const int2 *)G.d_rown);
         HIPCHK(hipGetLastError());
     }
     return 0;
@@ -1093,11 +1169,14 @@ void batch_free_groups(s3d_batch *b)
         for (void *p : bufs) dfree(p);
     }
     b->groups.clear();
-    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->graph_exec = nullptr;
-    b->graph = nullptr;
-    b->graph_steps = 0;
+    for (int k = 0; k < 2; ++k) {
+        if (b->graph_exec[k]) (void)hipGraphExecDestroy(b->graph_exec[k]);
+        if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+        b->graph_exec[k] = nullptr;
+        b->graph[k] = nullptr;
+        b->graph_steps[k] = 0;
+    }
+    b->phase = 0;
     b->built = false;
 }
 
@@ -1105,13 +1184,17 @@ void batch_free_groups(s3d_batch *b)
 // node and receiver kernels, then every counter advances
 int batch_launch_step(s3d_batch *b)
 {
-    for (auto &G : b->groups)
+    for (auto &G : b->groups) {
+        G.phase = b->phase;
         if (int rc = batch_group_call(1, G, b->stream)) return rc;
+    }
     for (auto &G : b->groups)
         if (int rc = batch_group_call(2, G, b->stream)) return rc;
     const int n = (int)b->sims.size();
     hipLaunchKernelGGL(advance_many_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_its, n);
     HIPCHK(hipGetLastError());
+    for (s3d_ctx *c : b->sims) advance_state(c);
+    b->phase ^= 1;
     return 0;
 }
 
@@ -1228,6 +1311,11 @@ int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out)
     if (!c->fp32) { c->o.kahan_v = 0; c->o.kahan_sigma = 0; c->o.bf16_static = 0; }
     c->o.kahan_v = c->o.kahan_v != 0;
     c->o.bf16_static = c->o.bf16_static != 0;
+    if (c->o.state_buffers < 0 || c->o.state_buffers > 2) {
+        delete c;
+        return fail("state_buffers must be 0 (default), 1 or 2");
+    }
+    c->nbuf = c->o.state_buffers == 1 ? 1 : 2;
     c->dev = c->o.device;
     if (c->dev < 0 || c->dev >= ndev) { delete c; return fail("bad device ordinal"); }
     hipError_t e = hipSetDevice(c->dev);
@@ -1281,8 +1369,8 @@ int s3d_destroy(s3d_ctx *c)
         c->model = nullptr;
     }
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
-    c->sig_lo = c->mem = nullptr;   // parts of the state block c->sig
-    void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->sig,
+    c->sig = c->sig_lo = c->mem = nullptr;   // views of the state blocks
+    void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->blk[0], c->blk[1],
                     c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho, c->grad_stage, c->dgrad,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
@@ -1307,6 +1395,7 @@ int s3d_create_shared(s3d_ctx *base, s3d_ctx **out)
     // mesh, options, host geometry and element tile of the model's simulation
     c->g = m->g;
     c->o = m->o;
+    c->nbuf = m->nbuf;
     c->p = m->p; c->P1 = m->P1; c->NP = m->NP; c->PB = m->PB;
     c->nel = m->nel; c->nn = m->nn; c->nblk = m->nblk; c->nuniq = m->nuniq;
     c->fp32 = m->fp32; c->visco = m->visco; c->aniso = m->aniso;
@@ -1749,13 +1838,14 @@ int s3d_step(s3d_ctx *c, int nsteps)
     // the rest (and every step while kernel timing is on) launches directly.
     constexpr int GRAPH_CHUNK = 16;
     int left = nsteps;
+    static_assert(GRAPH_CHUNK % 2 == 0, "a chunk must end on the state block it starts from");
     if (c->use_graph && !c->timing && left >= GRAPH_CHUNK) {
-        if (c->graph_steps != GRAPH_CHUNK) {
+        if (c->graph_steps[c->cur] != GRAPH_CHUNK) {
             int rc = capture_graph(c, GRAPH_CHUNK);
             if (rc) return rc;
         }
         while (left >= GRAPH_CHUNK) {
-            HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+            HIPCHK(hipGraphLaunch(c->graph_exec[c->cur], c->stream));
             c->it += GRAPH_CHUNK;
             left -= GRAPH_CHUNK;
         }
@@ -2100,8 +2190,18 @@ int s3d_batch_step(s3d_batch *b, int nsteps)
     }
     constexpr int GRAPH_CHUNK = 16;
     int left = nsteps;
+    // an even chunk ends in the phase (and on the state blocks) it started
+    // from, so the graph captured for a phase replays whenever it is current
+    static_assert(GRAPH_CHUNK % 2 == 0, "a chunk must end in the phase it starts from");
     if (left >= GRAPH_CHUNK) {
-        if (b->graph_steps != GRAPH_CHUNK) {
+        const int k = b->phase;
+        if (b->graph_steps[k] != GRAPH_CHUNK) {
+            std::vector<int> cur0;
+            for (s3d_ctx *c : b->sims) cur0.push_back(c->cur);
+            auto restore = [&]() {
+                for (size_t q = 0; q < b->sims.size(); ++q) set_state_block(b->sims[q], cur0[q]);
+                b->phase = k;
+            };
             HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             int rc = 0;
             for (int q = 0; q < GRAPH_CHUNK && rc == 0; ++q) rc = batch_launch_step(b);
@@ -2109,16 +2209,20 @@ int s3d_batch_step(s3d_batch *b, int nsteps)
             hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (rc) {
                 if (g) (void)hipGraphDestroy(g);
+                restore();
                 return rc;
             }
-            if (e != hipSuccess) return fail(std::string("batch capture: ") + hipGetErrorString(e));
-            b->graph = g;
-            e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                restore();
+                return fail(std::string("batch capture: ") + hipGetErrorString(e));
+            }
+            b->graph[k] = g;
+            e = hipGraphInstantiate(&b->graph_exec[k], g, nullptr, nullptr, 0);
             if (e != hipSuccess) return fail(std::string("batch instantiate: ") + hipGetErrorString(e));
-            b->graph_steps = GRAPH_CHUNK;
+            b->graph_steps[k] = GRAPH_CHUNK;
         }
         while (left >= GRAPH_CHUNK) {
-            HIPCHK(hipGraphLaunch(b->graph_exec, b->stream));
+            HIPCHK(hipGraphLaunch(b->graph_exec[k], b->stream));
             for (s3d_ctx *c : b->sims) c->it += GRAPH_CHUNK;
             left -= GRAPH_CHUNK;
         }

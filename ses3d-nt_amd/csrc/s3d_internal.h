@@ -103,11 +103,17 @@ struct ElemArgs {
     int nt;
     int kahan_sigma;           // 0 off, 1 FP32 compensation, 2 BF16 compensation
     gptr<const R> vx, vy, vz;  // unique-node velocity
-    gptr<R> sig;               // [6][nel][NP]
-    gptr<R> sig_lo;            // [6][nel][NP] compensation (kahan_sigma 1) or null
-    gptr<unsigned> sig_lo2;    // [3][nel][NP] compensation (kahan_sigma 2): BF16 pairs
-                               // (xx,xy) (xz,yy) (yz,zz), first in the low half
-    gptr<R> mem;               // [18][nel][NP] (visco) or null
+    // state block read by the step (parts of one element-major block, see
+    // em_index): stress, its compensation, memory variables
+    gptr<const R> sig;               // 6 components
+    gptr<const R> sig_lo;            // 6 compensation words (kahan_sigma 1) or null
+    gptr<const unsigned> sig_lo2;    // 3 words (kahan_sigma 2): BF16 pairs (xx,xy)
+                                     // (xz,yy) (yz,zz), first in the low half
+    gptr<const R> mem;               // 18 memory variables (visco) or null
+    // state block the step writes: the same block (one state buffer) or the
+    // other one of a pair (two state buffers, swapped every step)
+    gptr<R> sig_w, sig_lo_w, mem_w;
+    gptr<unsigned> sig_lo2_w;
     // [nel][NP] moduli times dt: elastic lam, mu; visco-elastic
     // lam - 2/3 tau mu and mu (1 + tau) (relaxed-to-unrelaxed form)
     gptr<const R> lam, mu, tau;

@@ -576,23 +576,23 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             if constexpr (KS == 1) {
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
-                    a.sig[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
-                    a.sig_lo[em_index(ev, NS, c, NP, node)] = lo[c] - lo[c] * dmp;
+                    a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
+                    a.sig_lo_w[em_index(ev, NS, c, NP, node)] = lo[c] - lo[c] * dmp;
                 }
             } else if constexpr (KS == 2) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) a.sig[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
+                for (int c = 0; c < 6; ++c) a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
 #pragma unroll
                 for (int q = 0; q < 3; ++q)
-                    a.sig_lo2[em_index(ev, NS, q, NP, node)] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
+                    a.sig_lo2_w[em_index(ev, NS, q, NP, node)] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
                                                        (float)(lo[2 * q + 1] - lo[2 * q + 1] * dmp));
             } else {
 #pragma unroll
-                for (int c = 0; c < 6; ++c) a.sig[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
+                for (int c = 0; c < 6; ++c) a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
             }
             if constexpr (VISCO) {
 #pragma unroll
-                for (int c = 0; c < 18; ++c) a.mem[em_index(ev, NS, c, NP, node)] = M[c];
+                for (int c = 0; c < 18; ++c) a.mem_w[em_index(ev, NS, c, NP, node)] = M[c];
             }
         }
     }
