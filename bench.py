@@ -333,7 +333,8 @@ def batch_bench(args):
     cases = synth.uneven_batch(seed=4, n_sims=args.batch_sims, nt=args.warmup + 3 * args.steps + 1)
     N = sum(c.grid.nodes for c in cases)
     t0 = time.time()
-    sims = [api.build_case(c, precision="fp32", autotune=True)[0] for c in cases]
+    sims = [api.build_case(c, precision="fp32", autotune=True, state_buffers=args.state_buffers)[0]
+            for c in cases]
     log("[batch] %d simulations, %d GLL nodes, orders %s, setup %.0f s"
         % (len(sims), N, sorted(set(c.grid.lpd for c in cases)), time.time() - t0))
     b = api.Batch(sims)
@@ -367,7 +368,8 @@ def batch_bench(args):
            "batched": {"value": N * args.steps / wall_b, "ms_per_step": 1e3 * wall_b / args.steps},
            "round_robin": {"value": N * args.steps / wall_r, "ms_per_step": 1e3 * wall_r / args.steps},
            "sequential": {"value": N * args.steps / wall_s, "ms_per_step": 1e3 * wall_s / args.steps},
-           "speedup_batched_vs_round_robin": wall_r / wall_b}
+           "speedup_batched_vs_round_robin": wall_r / wall_b,
+           "state_buffers": 1 if args.state_buffers == 1 else 2}
     log("[batch] batched %.3f ms/step, round robin %.3f ms/step, sequential %.3f ms/step"
         % (1e3 * wall_b / args.steps, 1e3 * wall_r / args.steps, 1e3 * wall_s / args.steps))
     print(json.dumps(out), flush=True)
@@ -385,13 +387,17 @@ def multi_source_bench(args):
     kw = dict(MODES[mode])
     if kw.pop("bf16_static"):
         raise SystemExit("BF16 static fields are not supported in a batch")
-    kw["state_buffers"] = args.state_buffers
+    # one state block per source unless asked otherwise: with source-innermost
+    # batches the second block does not pay (80.05 against 80.64 ms/step for
+    # 16 sources in batches of 8), and one block fits 16 sources per batch
+    nbuf = args.state_buffers or 1
+    kw["state_buffers"] = nbuf
     case = synth.multi_source(seed=5, n_src=args.sources, nt=args.warmup + 2 * args.steps + 1)
     g = case.grid
     N = g.nodes
     S = len(case.sources)
     u = g.unique_nodes / N
-    per_default = 16 if args.state_buffers == 1 else 8
+    per_default = 16 if nbuf == 1 else 8
     per = max(1, min(args.per_batch or per_default, S))
     t0 = time.time()
     owner, recs = api.build_case(replace(case, sources=[]), slab=args.slab, autotune=True, **kw)
@@ -438,7 +444,7 @@ def multi_source_bench(args):
                        "elements, GLL order %d, %s, batches of %d" % (S, g.nx, g.ny, g.nz, g.lpd, mode,
                                                                      per),
            "gll_nodes_per_source": N, "sources": S, "per_batch": per, "steps": args.steps,
-           "state_buffers": 1 if args.state_buffers == 1 else 2,
+           "state_buffers": nbuf,
            "batched_shared": {"value": value, "ms_per_step": 1e3 * wall_b / args.steps},
            "round_robin": {"value": N * S * args.steps / wall_r,
                            "ms_per_step": 1e3 * wall_r / args.steps},
@@ -480,9 +486,10 @@ def main():
                     help="sources sharing the model (config: 32)")
     ap.add_argument("--per-batch", type=int, default=None,
                     help="sources per shared-model batch (default: what fits in HBM at "
-                         "128x128x48, 8 with two state buffers, 16 with one)")
+                         "128x128x48, 16 with one state block, 8 with two)")
     ap.add_argument("--state-buffers", type=int, default=0, choices=(0, 1, 2),
-                    help="state blocks per simulation: 0 library default (2), 1 in place")
+                    help="state blocks per simulation: 0 default (2; 1 for --multi-source), "
+                         "1 updated in place, 2 read one and write the other")
     ap.add_argument("--precision-study", action="store_true",
                     help="FP32 variants vs FP64 seismogram / snapshot misfit instead of timing")
     ap.add_argument("--study-steps", type=int, default=4000)
