@@ -1209,12 +1209,14 @@ int batch_build(s3d_batch *b)
         if (upload_sources(c) || upload_receivers(c)) return 1;
         drop_graph(c);
     }
+    // one launch per kind of step (order, precision, visco, compensation);
+    // the element tile and LDS layout give bit-identical results, so a
+    // group takes those of its largest member rather than splitting by them
     for (s3d_ctx *c : b->sims) {
         bool placed = false;
         for (auto &G : b->groups)
             if (G.p == c->p && G.fp32 == c->fp32 && G.visco == c->visco &&
-                G.kahan == (c->lo[0] != nullptr) && G.epb == c->epb && G.ks == c->o.kahan_sigma &&
-                G.ll == c->lds_layout) {
+                G.kahan == (c->lo[0] != nullptr) && G.ks == c->o.kahan_sigma) {
                 G.sims.push_back(c);
                 placed = true;
                 break;
@@ -1225,12 +1227,17 @@ int batch_build(s3d_batch *b)
             G.fp32 = c->fp32;
             G.visco = c->visco;
             G.kahan = c->lo[0] != nullptr;
-            G.epb = c->epb;
-            G.ll = c->fp32 ? c->lds_layout : 0;
             G.ks = c->o.kahan_sigma;
             G.sims.push_back(c);
             b->groups.push_back(G);
         }
+    }
+    for (auto &G : b->groups) {
+        const s3d_ctx *big = G.sims[0];
+        for (const s3d_ctx *c : G.sims)
+            if (c->nel > big->nel) big = c;
+        G.epb = big->epb;
+        G.ll = big->fp32 ? big->lds_layout : 0;
     }
     for (auto &G : b->groups)
         if (int rc = batch_group_call(0, G, b->stream)) return rc;

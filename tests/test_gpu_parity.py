@@ -358,6 +358,35 @@ def test_set_tile_checks():
     assert d.tuning() == (2, 1, 0)
 
 
+def test_batch_members_with_different_tiles_and_layouts_match_solo_runs():
+    """A batch launches members of one kind together with the tile and LDS
+    layout of its largest member; every tile and layout gives the same
+    results, so each member still equals its own solo run bit for bit."""
+    cases = [synth.layered(seed=6 + q, nt=24, nx=nx, ny=6, nz=4, lpd=4, n_rec=3)
+             for q, nx in enumerate((8, 5, 9, 7))]
+    cfgs = [(1, 0), (2, 1), (4, 0), (4, 1)]
+    solo, members, recs = [], [], []
+    for case, (tile, lay) in zip(cases, cfgs):
+        for lst in (solo, members):
+            sim, rr = api.build_case(case, precision="fp32", kahan_sigma="bf16")
+            sim.set_tile(tile)
+            sim.set_lds_layout(lay)
+            lst.append(sim)
+        recs.append(rr)
+    for sim in solo:
+        sim.step(24)
+    b = api.Batch(members)
+    b.step(24)
+    b.synchronize()
+    b.close()
+    for q, (a, m) in enumerate(zip(solo, members)):
+        for r in range(len(recs[q])):
+            for ch in range(3):
+                assert np.array_equal(a.trace(r, ch), m.trace(r, ch)), (q, r, ch)
+        assert np.array_equal(a.get(api.VX), m.get(api.VX)), q
+        assert np.array_equal(a.get(api.SYZ), m.get(api.SYZ)), q
+
+
 def test_batch_member_outlives_batch():
     case = synth.homogeneous(seed=3, nt=10, nx=4, ny=4, nz=4, n_rec=1)
     s, _ = api.build_case(case, precision="fp32")
