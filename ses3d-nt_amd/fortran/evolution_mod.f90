@@ -183,6 +183,7 @@ CONTAINS
         o%kahan_sigma = MERGE(MERGE(2, 1, cf%kahan_sigma_bf16), 0, cf%kahan_sigma)
         o%bf16_static = MERGE(1, 0, cf%bf16_static)
         o%device = cf%device
+        o%state_buffers = cf%state_buffers
         CALL check(s3d_create(g, o, ctx), 'create')
         n = INT(cf%nx, 8) * cf%ny * cf%nz * (cf%lpd + 1)**3
         ALLOCATE(buf(n))

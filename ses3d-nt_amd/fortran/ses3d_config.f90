@@ -10,7 +10,9 @@
 !! kahan_sigma, kahan_sigma_bf16 (stress compensation words in BF16),
 !! bf16_static (moduli, tau and dt/M stored in BF16), device, tile (element
 !! tile of the FP32 step: -1 autotune, 0 default, 1, 2 or 4 elements per
-!! workgroup).
+!! workgroup), state_buffers (2: the step reads the state from one block and
+!! writes the other, the default; 1: one block updated in place, half the
+!! state memory).
 MODULE configuration_mod
     USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY : ERROR_UNIT, IOSTAT_END
     USE, INTRINSIC :: ISO_C_BINDING, ONLY : c_double, c_int
@@ -66,7 +68,7 @@ MODULE configuration_mod
         ! &gpu
         CHARACTER(LEN=8) :: precision
         LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
-        INTEGER :: device, tile
+        INTEGER :: device, tile, state_buffers
         TYPE(source_entry), ALLOCATABLE :: sources(:)
         TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
         TYPE(output_entry), ALLOCATABLE :: outputs(:)
@@ -127,8 +129,9 @@ CONTAINS
         ! &gpu
         CHARACTER(LEN=8) :: precision
         LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
-        INTEGER :: device, tile
-        NAMELIST /gpu/ precision, kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static, device, tile
+        INTEGER :: device, tile, state_buffers
+        NAMELIST /gpu/ precision, kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static, device, tile, &
+                       state_buffers
         INTEGER :: lun, ios, n
         TYPE(configuration_typ) :: cf
 
@@ -213,7 +216,7 @@ CONTAINS
 
         ! ---- &gpu (optional) ----
         precision = 'fp32'; kahan_v = .TRUE.; kahan_sigma = .TRUE.; device = 0
-        kahan_sigma_bf16 = .TRUE.; bf16_static = .FALSE.; tile = -1
+        kahan_sigma_bf16 = .TRUE.; bf16_static = .FALSE.; tile = -1; state_buffers = 2
         n = count_groups(lun, 'gpu')
         IF (n > 1) CALL abort('Found ' // i2c(n) // ' &gpu-groups in config-file!')
         IF (n == 1) THEN
@@ -227,6 +230,8 @@ CONTAINS
         cf%device = device
         IF (ALL(tile /= [-1, 0, 1, 2, 4])) CALL abort('tile must be -1 (autotune), 0, 1, 2 or 4')
         cf%tile = tile
+        IF (state_buffers /= 1 .AND. state_buffers /= 2) CALL abort('state_buffers must be 1 or 2')
+        cf%state_buffers = state_buffers
 
         CALL parse_sources(lun, cf)
         CALL parse_receivers(lun, cf)

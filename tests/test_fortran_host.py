@@ -289,6 +289,19 @@ def test_tile_option_checked(tmp_path):
     assert r.returncode != 0 and "tile must be" in r.stderr
 
 
+def test_state_buffers_option_checked(tmp_path):
+    case = synth.homogeneous(nt=60, nx=6, ny=6, nz=5, n_rec=2)
+    cfg = write_config(case, str(tmp_path))
+    base = open(cfg).read()
+    for value, ok in ((1, True), (2, True), (3, False)):
+        open(cfg, "w").write(base.replace("&gpu\n", "&gpu\n state_buffers = %d\n" % value, 1))
+        r = check(cfg)
+        if ok:
+            assert r.returncode == 0, r.stderr
+        else:
+            assert r.returncode != 0 and "state_buffers must be" in r.stderr
+
+
 def _raw(prefix, start, inc, end, atts):
     return ("\n&output_raw\n timestep_start = %d\n timestep_increment = %d\n"
             " timestep_end = %d\n attributes = %s\n override = .TRUE.\n prefix = '%s'\n/\n"
