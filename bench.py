@@ -483,7 +483,7 @@ def main():
     ap.add_argument("--multi-source", action="store_true",
                     help="many sources sharing one model (128x128x48, p=4) instead of the single mesh")
     ap.add_argument("--sources", type=int, default=32,
-                    help="sources sharing the model (config: 32)")
+                    help="sources sharing the model (default 32)")
     ap.add_argument("--per-batch", type=int, default=None,
                     help="sources per shared-model batch (default: what fits in HBM at "
                          "128x128x48, 16 with one state block, 8 with two)")

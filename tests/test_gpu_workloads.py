@@ -1,17 +1,17 @@
 """The five benchmark workloads at their own sizes on the GPU.
 
-1  homogeneous elastic 12x12x8, p = 4, 1000 steps, 16 receivers: FP64 and the
-   timed FP32 mode against the FP64 oracle.
-2  layered visco-elastic 96x96x32, p = 4, 4000 steps, 256 receivers: the
-   timed FP32 mode against the FP64 GPU path (traces and a snapshot).
-3  the same model at 192x192x96 (442 M GLL nodes): the timed FP32 mode against
-   the FP64 GPU path over 1500 steps, with receivers on rings around the
-   source so that the whole wavelet reaches them inside the run.
-4  64 uneven regional simulations (8x8x8 .. 48x48x24, p = 4 or 6) advanced as
-   one batch: members equal the same simulations stepped alone, bit for bit.
-5  32 sources on one 128x128x48, p = 4 model, as two batches of 16 (the state
-   of 32 sources exceeds the 288 GB of HBM): each batched source equals the
-   same source stepped alone on the shared model, bit for bit.
+* homogeneous elastic 12x12x8, p = 4, 1000 steps, 16 receivers: FP64 and the
+  timed FP32 mode against the FP64 oracle.
+* layered visco-elastic 96x96x32, p = 4, 4000 steps, 256 receivers: the
+  timed FP32 mode against the FP64 GPU path (traces and a snapshot).
+* the same model at 192x192x96 (442 M GLL nodes): the timed FP32 mode against
+  the FP64 GPU path over 1500 steps, with receivers on rings around the
+  source so that the whole wavelet reaches them inside the run.
+* 64 uneven regional simulations (8x8x8 .. 48x48x24, p = 4 or 6) advanced as
+  one batch: members equal the same simulations stepped alone, bit for bit.
+* 32 sources on one 128x128x48, p = 4 model, as two batches of 16 (the state
+  of 32 sources exceeds the 288 GB of HBM): each batched source equals the
+  same source stepped alone on the shared model, bit for bit.
 
 Misfits are relative L2 per trace over the traces with signal (norm above
 1e-3 of the largest), the quantity the benchmark's 1e-5 bound is stated in.
@@ -32,7 +32,7 @@ TIMED = dict(precision="fp32", kahan_v=True, kahan_sigma="bf16")
 
 
 def log(*a):
-    print("[configs]", *a, flush=True)
+    print("[workloads]", *a, flush=True)
 
 
 def _traces(sim, recs):
@@ -46,7 +46,7 @@ def _misfit(t, ref):
     return float(np.median(err)), float(err.max()), int(keep.sum())
 
 
-def test_config1_homogeneous_against_oracle():
+def test_homogeneous_12x12x8_1000_steps_against_oracle():
     case = synth.homogeneous()                      # 12x12x8, p=4, 1000 steps, 16 receivers
     g = case.grid
     assert (g.nx, g.ny, g.nz, g.lpd, g.nt, len(case.receivers)) == (12, 12, 8, 4, 1000, 16)
@@ -54,13 +54,13 @@ def test_config1_homogeneous_against_oracle():
     o, orecs = oracle.build_case(case)
     o.step(g.nt)
     ref = np.array([[o.trace(r, c) for c in range(3)] for r in range(len(orecs))])
-    log("config 1 oracle %.1f s" % (time.time() - t0))
+    log("homogeneous 12x12x8 oracle %.1f s" % (time.time() - t0))
     for name, kw, tol in (("fp64", dict(precision="fp64"), 1e-11), ("fp32 timed", TIMED, 1e-5)):
         sim, recs = api.build_case(case, **kw)
         assert len(recs) == len(orecs) == 16
         sim.step(g.nt)
         med, mx, n = _misfit(_traces(sim, recs), ref)
-        log("config 1 %s: trace rel L2 median %.2e max %.2e over %d traces" % (name, med, mx, n))
+        log("homogeneous 12x12x8 %s: trace rel L2 median %.2e max %.2e over %d traces" % (name, med, mx, n))
         assert mx < tol
         sim.close()
 
@@ -91,14 +91,14 @@ def _fp32_vs_fp64(case, tag, snap_tol=1e-5):
     assert snap < snap_tol
 
 
-def test_config2_layered_visco_96x96x32_4000_steps():
+def test_layered_visco_96x96x32_4000_steps():
     case = synth.layered()                          # 96x96x32, p=4, 4000 steps, 256 receivers
     g = case.grid
     assert (g.nx, g.ny, g.nz, g.nt, len(case.receivers)) == (96, 96, 32, 4000, 256)
-    _fp32_vs_fp64(case, "config 2")
+    _fp32_vs_fp64(case, "layered 96x96x32")
 
 
-def test_config3_layered_visco_192x192x96():
+def test_layered_visco_192x192x96_1500_steps():
     case = synth.layered(nx=192, ny=192, nz=96, nt=1500, n_rec=32)
     src = case.sources[0]
     ring = []
@@ -108,7 +108,7 @@ def test_config3_layered_visco_192x192x96():
             ring.append(synth.Receiver(src.lat + d * np.cos(a), src.lon + d * np.sin(a), 0.0,
                                        "R%d_%d" % (int(10 * d), az)))
     case = replace(case, receivers=ring + case.receivers)
-    _fp32_vs_fp64(case, "config 3")
+    _fp32_vs_fp64(case, "layered 192x192x96")
 
 
 def _early(case, onset=2.0, width=2.0):
@@ -117,13 +117,13 @@ def _early(case, onset=2.0, width=2.0):
     return replace(case, sources=[replace(s, onset=onset, width=width) for s in case.sources])
 
 
-def test_config4_uneven_batch_of_64_equals_solo_runs():
+def test_uneven_batch_of_64_equals_solo_runs():
     steps = 20
     cases = [_early(c) for c in synth.uneven_batch(seed=4, n_sims=64, nt=steps)]
     assert len(cases) == 64 and {c.grid.lpd for c in cases} == {4, 6}
     t0 = time.time()
     sims = [api.build_case(c, **TIMED)[0] for c in cases]
-    log("config 4: 64 simulations, %d GLL nodes, set up in %.0f s"
+    log("uneven batch: 64 simulations, %d GLL nodes, set up in %.0f s"
         % (sum(c.grid.nodes for c in cases), time.time() - t0))
     b = api.Batch(sims)
     b.step(3)
@@ -144,12 +144,12 @@ def test_config4_uneven_batch_of_64_equals_solo_runs():
             for ch in range(3):
                 assert np.array_equal(sims[q].trace(r, ch), solo.trace(r, ch)), (q, r, ch)
         solo.close()
-    log("config 4: members %s equal their solo runs" % pick)
+    log("uneven batch: members %s equal their solo runs" % pick)
     for s in sims:
         s.close()
 
 
-def test_config5_32_sources_as_two_batches_of_16():
+def test_32_sources_on_128x128x48_as_two_batches_of_16():
     steps = 20
     case = _early(synth.multi_source(n_src=32, nt=steps))
     g = case.grid
@@ -158,7 +158,7 @@ def test_config5_32_sources_as_two_batches_of_16():
     # one state buffer per source (updated in place): 16 sources and the
     # model's owner then fit on one GPU (224 GB; two buffers would need 400 GB)
     owner, recs = api.build_case(replace(case, sources=[]), slab=4, state_buffers=1, **TIMED)
-    log("config 5 model set up in %.0f s" % (time.time() - t0))
+    log("32 sources: model set up in %.0f s" % (time.time() - t0))
     batched = {}
     for half in (case.sources[:16], case.sources[16:]):
         sims = []
@@ -171,7 +171,7 @@ def test_config5_32_sources_as_two_batches_of_16():
         b.step(16)
         b.synchronize()
         b.close()
-        log("config 5: batch of %d sources, device %.0f GB" %
+        log("32 sources: batch of %d sources, device %.0f GB" %
             (len(sims), (owner.device_bytes() + sum(x.device_bytes() for x in sims)) / 1e9))
         for s, sim in zip(half, sims):
             q = case.sources.index(s)
@@ -187,5 +187,5 @@ def test_config5_32_sources_as_two_batches_of_16():
         assert np.array_equal(vz, v), q
         assert np.array_equal(tr, _traces(lone, recs)), q
         lone.close()
-    log("config 5: sources %s equal their solo runs" % sorted(batched))
+    log("32 sources: sources %s equal their solo runs" % sorted(batched))
     owner.close()
