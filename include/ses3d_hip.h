@@ -73,8 +73,12 @@ enum {
        exx .. ezz, src/volume_snapshot.f90:220-236), read only, after
        s3d_enable_displacement_gradient */
     S3D_EXX = 26, S3D_EXY, S3D_EXZ, S3D_EYY, S3D_EYZ, S3D_EZZ,
-    S3D_MXX = 32   /* memory variable: S3D_MXX + 6*mechanism + comp,
+    S3D_MXX = 32,  /* memory variable: S3D_MXX + 6*mechanism + comp,
                       comp order xx, xy, xz, yy, yz, zz; mechanism 0..2 */
+    /* volume curl fields (reference snapshot attributes rot_vx, rot_vy,
+       rot_vz, src/volume_snapshot.f90:238-245), read only, after a step
+       preceded by s3d_capture_curl_strain */
+    S3D_ROT_VX = 50, S3D_ROT_VY, S3D_ROT_VZ
 };
 
 /* Receiver channels (reference src/config/channel_mod.f90:270-336). The curl
@@ -255,6 +259,17 @@ int s3d_grad_add(s3d_ctx *ctx, const double *vx_fw6d, const double *vy_fw6d, con
  * before the first step to match the reference. Refused for a batch
  * member. */
 int s3d_enable_displacement_gradient(s3d_ctx *ctx);
+
+/* Store the strain-rate parts that the volume curl fields combine with the
+ * velocity, formed from the current velocity, i.e. the start velocity of the
+ * next step (the reference's snapshot of rot_v* after step it mixes the
+ * strain rates ses3d_evolution formed at the start of step it with the
+ * velocity at its end, src/volume_snapshot.f90:238-245). After exactly one
+ * more step, s3d_get_field(S3D_ROT_VX .. S3D_ROT_VZ) returns the fields.
+ * The first call allocates 6 x 8 B per element-local node, kept until
+ * s3d_destroy. No counterpart call in the reference, whose arrays hold the
+ * strain rates anyway. */
+int s3d_capture_curl_strain(s3d_ctx *ctx);
 
 /* min and max of vx, vy, vz (reference v_ptp_loc, src/elastic_vars.F90:179-185). */
 int s3d_vrange(s3d_ctx *ctx, double *out2);

@@ -185,6 +185,7 @@ struct s3d_ctx {
          *ca = nullptr, *cb = nullptr, *cc = nullptr, *force = nullptr, *tth = nullptr, *mat = nullptr,
          *grad_rho = nullptr, *grad_stage = nullptr,   // FP64 density; 3 snapshot components
          *dgrad = nullptr,   // FP64 displacement gradient [nel][6][NP], see s3d_enable_displacement_gradient
+         *curl_parts = nullptr,   // FP64 [nel][6][NP], see s3d_capture_curl_strain
          *tr = nullptr, *wdev = nullptr;
     double *mm64 = nullptr, *dpx = nullptr, *dpy = nullptr, *dpz = nullptr;
     double bytes = 0;
@@ -224,6 +225,7 @@ struct s3d_ctx {
     int cur = 0;
     int epb = 0;               // element tile (0 = default), see s3d_autotune
     int lds_layout = 0;        // LDS staging layout of the FP32 element kernel, see Stage
+    int curl_it = -1;          // steps completed at the last s3d_capture_curl_strain
     bool fields_set = false;   // s3d_set_field was called (state not at rest)
     // s3d_create_shared: the simulation whose static fields (moduli, tau, rho,
     // A/B/C, dt/M, FP64 mass, 1-D tables, taper profiles) this one uses, and
@@ -768,6 +770,28 @@ struct ExpandRun {
     {
         hipLaunchKernelGGL((expand_unique<P, R>), dim3(nblocks(c->nn, 256)), dim3(256), 0, c->stream,
                            (const R *)src, (const R *)lo, dst, c->g.nx, c->g.ny, c->g.nz);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+};
+
+template <int P, typename R>
+struct CurlPartsRun {
+    static int run(s3d_ctx *c)
+    {
+        hipLaunchKernelGGL((curl_parts_kernel<P, R>), dim3((unsigned)c->nel), dim3(64), 0, c->stream,
+                           curl_args<P, R>(c), c->nel, (double *)c->curl_parts);
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+};
+
+template <int P, typename R>
+struct CurlFieldRun {
+    static int run(s3d_ctx *c, int comp, double *out)
+    {
+        hipLaunchKernelGGL((curl_field_kernel<P, R>), dim3(nblocks(c->nn, 256)), dim3(256), 0, c->stream,
+                           curl_args<P, R>(c), c->g.nz, c->nn, (const double *)c->curl_parts, comp, out);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -1378,7 +1402,7 @@ int s3d_destroy(s3d_ctx *c)
     for (int q = 0; q < 7; ++q) dfree(c->stage[q]);
     c->sig = c->sig_lo = c->mem = nullptr;   // views of the state blocks
     void *bufs[] = {c->v[0], c->v[1], c->v[2], c->lo[0], c->lo[1], c->lo[2], c->dt_inv_mm, c->blk[0], c->blk[1],
-                    c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho, c->grad_stage, c->dgrad,
+                    c->mat, c->rho, c->ca, c->cb, c->cc, c->grad_rho, c->grad_stage, c->dgrad, c->curl_parts,
                     c->force, c->tth, c->tr, c->wdev, c->mm64, c->dpx, c->dpy, c->dpz,
                     c->d_slot, c->d_chan, c->d_coef, c->d_stf, c->d_rec_e, c->d_rec_nch,
                     c->d_rec_off, c->d_rec_ch, c->d_rec_vm, c->d_traces, c->d_rec_rot, c->d_rot_e, c->d_rot_buf};
@@ -1927,6 +1951,14 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
                      : with_p<ExpandRun, double>(c->p, c, (const void *)c->v[field], (const void *)nullptr, buf);
     } else if (field == S3D_MM) {
         rc = with_p<ExpandRun, double>(c->p, c, (const void *)c->mm64, (const void *)nullptr, buf);
+    } else if (field >= S3D_ROT_VX && field <= S3D_ROT_VZ) {
+        if (!c->curl_parts || c->curl_it != c->it - 1) {
+            (void)hipFree(buf);
+            return fail("volume curl fields need s3d_capture_curl_strain just before the last step");
+        }
+        const int comp = field - S3D_ROT_VX;
+        rc = c->fp32 ? with_p<CurlFieldRun, float>(c->p, c, comp, buf)
+                     : with_p<CurlFieldRun, double>(c->p, c, comp, buf);
     } else if (field >= S3D_EXX && field <= S3D_EZZ) {
         if (!c->dgrad) {
             (void)hipFree(buf);
@@ -2071,6 +2103,18 @@ int s3d_enable_displacement_gradient(s3d_ctx *c)
     HIPCHK(hipSetDevice(c->dev));
     if (dalloc(c, &c->dgrad, sizeof(double) * 6 * c->nn)) return 1;
     drop_graph(c);   // the captured launches carry the element kernel's arguments
+    return 0;
+}
+
+int s3d_capture_curl_strain(s3d_ctx *c)
+{
+    if (!c) return fail("null argument");
+    if (!c->inited) return fail("s3d_init first");
+    HIPCHK(hipSetDevice(c->dev));
+    if (!c->curl_parts && dalloc(c, &c->curl_parts, sizeof(double) * 6 * c->nn, false)) return 1;
+    const int rc = c->fp32 ? with_p<CurlPartsRun, float>(c->p, c) : with_p<CurlPartsRun, double>(c->p, c);
+    if (rc) return rc;
+    c->curl_it = c->it;
     return 0;
 }
 

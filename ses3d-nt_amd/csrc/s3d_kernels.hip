@@ -716,6 +716,18 @@ node_kernel_batch(const NodeArgs<R> *__restrict__ args, const int2 *__restrict__
 // ---------------------------------------------------------------------------
 // receivers: one wave per receiver, all its channels
 // ---------------------------------------------------------------------------
+// The curl formulas (src/config/channel_mod.f90:620-774 for receivers,
+// src/volume_snapshot.f90:238-245 for snapshots) from the parts of a step's
+// start velocity and the velocity vx, vy at its end; comp 0, 1, 2 =
+// rot_vx, rot_vy, rot_vz.
+__device__ inline double curl_formula(int comp, const double part[6], double vxn, double vyn, double r,
+                                      double rs, double ct, double st)
+{
+    if (comp == 0) return -vyn / r + part[0] / rs + part[3];
+    if (comp == 1) return -part[4] + (vxn - part[1]) / r;
+    return (part[2] + vyn * ct - part[5] / st) / r;
+}
+
 template <typename R>
 struct RecArgs {
     int nx, ny, nel, nt;
@@ -775,11 +787,12 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
             // curl channels (src/config/channel_mod.f90:620-774): strain-rate
             // parts of the step's start velocity, this velocity at its end
             const double *b = a.rot_buf + (long)rs * 10 * NP + n;
-            const double eyz = b[0], ezy = b[NP], ezx = b[2 * NP], exz = b[3 * NP], exy = b[4 * NP],
-                         eyx = b[5 * NP], rr = b[6 * NP], rst = b[7 * NP], ct = b[8 * NP], st = b[9 * NP];
-            acc[4] += w * (-vyn / rr + eyz / rst + ezy);
-            acc[5] += w * (-ezx + (vxn - exz) / rr);
-            acc[6] += w * ((exy + vyn * ct - eyx / st) / rr);
+            double part[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) part[c] = b[c * NP];
+            const double rr = b[6 * NP], rst = b[7 * NP], ct = b[8 * NP], st = b[9 * NP];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) acc[4 + c] += w * curl_formula(c, part, vxn, vyn, rr, rst, ct, st);
         }
         if (slot >= 0) {
 #pragma unroll
@@ -845,6 +858,52 @@ __device__ __forceinline__ double curl_vel(gptr<const R> v, gptr<const R> l, lon
     return (double)v[u] - (l ? (double)l[u] : 0.0);
 }
 
+// The strain-rate parts the curl formulas use, at node (i, j, k) of element
+// (ex, ey, ez), from the current velocity: eyz, exz, exy as the step's
+// symmetrisation leaves them and ezy, ezx, eyx as its spherical completion
+// leaves them (src/evolution.f90:165-208), in that order in part[0..5].
+template <int P, typename R>
+__device__ inline void curl_parts(const CurlArgs<R> &a, int ex, int ey, int ez, int i, int j, int k,
+                                  double part[6])
+{
+    constexpr int P1 = P + 1;
+    double exy = 0.0, exz = 0.0, eyx = 0.0, eyz = 0.0, ezx = 0.0, ezy = 0.0;
+    for (int q = 0; q < P1; ++q) {
+        const double d1 = a.Dx[i * P1 + q];
+        const long u = unique_index<P>(ex, ey, ez, q, j, k, a.nx, a.ny);
+        exy = exy + curl_vel(a.vy, a.ly, u) * d1;
+        exz = exz + curl_vel(a.vz, a.lz, u) * d1;
+    }
+    for (int q = 0; q < P1; ++q) {
+        const double d2 = a.Dy[j * P1 + q];
+        const long u = unique_index<P>(ex, ey, ez, i, q, k, a.nx, a.ny);
+        eyx = eyx + curl_vel(a.vx, a.lx, u) * d2;
+        eyz = eyz + curl_vel(a.vz, a.lz, u) * d2;
+    }
+    for (int q = 0; q < P1; ++q) {
+        const double d3 = a.Dz[k * P1 + q];
+        const long u = unique_index<P>(ex, ey, ez, i, j, q, a.nx, a.ny);
+        ezx = ezx + curl_vel(a.vx, a.lx, u) * d3;
+        ezy = ezy + curl_vel(a.vy, a.ly, u) * d3;
+    }
+    const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
+    const double vxn = curl_vel(a.vx, a.lx, u), vyn = curl_vel(a.vy, a.ly, u);
+    const double r = a.tr[ez * P1 + k].r, st = a.tth[ex * P1 + i].sin_t, ct = a.tth[ex * P1 + i].cot;
+    const double rs = r * st;
+    ezx = -ezx;
+    exz = (exz - vxn) / r;
+    ezy = -ezy;
+    eyz = -vyn / r + eyz / rs;
+    exy = exy / r;
+    eyx = -(vyn * ct) / r + eyx / rs;
+    part[0] = (eyz + ezy) / 2.0;
+    part[1] = (exz + ezx) / 2.0;
+    part[2] = (exy + eyx) / 2.0;
+    part[3] = ezy;
+    part[4] = ezx;
+    part[5] = eyx;
+}
+
 template <int P, typename R>
 __global__ void __launch_bounds__(64) curl_strain_kernel(const CurlArgs<R> a)
 {
@@ -855,47 +914,60 @@ __global__ void __launch_bounds__(64) curl_strain_kernel(const CurlArgs<R> a)
     const int ex = e % a.nx, ey = (e / a.nx) % a.ny, ez = e / (a.nx * a.ny);
     for (int n = threadIdx.x; n < NP; n += 64) {
         const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
-        double exy = 0.0, exz = 0.0, eyx = 0.0, eyz = 0.0, ezx = 0.0, ezy = 0.0;
-        for (int q = 0; q < P1; ++q) {
-            const double d1 = a.Dx[i * P1 + q];
-            const long u = unique_index<P>(ex, ey, ez, q, j, k, a.nx, a.ny);
-            exy = exy + curl_vel(a.vy, a.ly, u) * d1;
-            exz = exz + curl_vel(a.vz, a.lz, u) * d1;
-        }
-        for (int q = 0; q < P1; ++q) {
-            const double d2 = a.Dy[j * P1 + q];
-            const long u = unique_index<P>(ex, ey, ez, i, q, k, a.nx, a.ny);
-            eyx = eyx + curl_vel(a.vx, a.lx, u) * d2;
-            eyz = eyz + curl_vel(a.vz, a.lz, u) * d2;
-        }
-        for (int q = 0; q < P1; ++q) {
-            const double d3 = a.Dz[k * P1 + q];
-            const long u = unique_index<P>(ex, ey, ez, i, j, q, a.nx, a.ny);
-            ezx = ezx + curl_vel(a.vx, a.lx, u) * d3;
-            ezy = ezy + curl_vel(a.vy, a.ly, u) * d3;
-        }
-        const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
-        const double vxn = curl_vel(a.vx, a.lx, u), vyn = curl_vel(a.vy, a.ly, u);
+        double part[6];
+        curl_parts<P, R>(a, ex, ey, ez, i, j, k, part);
         const double r = a.tr[ez * P1 + k].r, st = a.tth[ex * P1 + i].sin_t, ct = a.tth[ex * P1 + i].cot;
-        const double rs = r * st;
-        ezx = -ezx;
-        exz = (exz - vxn) / r;
-        ezy = -ezy;
-        eyz = -vyn / r + eyz / rs;
-        exy = exy / r;
-        eyx = -(vyn * ct) / r + eyx / rs;
         double *b = a.buf + (long)s * 10 * NP + n;
-        b[0] = (eyz + ezy) / 2.0;
-        b[NP] = ezy;
-        b[2 * NP] = ezx;
-        b[3 * NP] = (exz + ezx) / 2.0;
-        b[4 * NP] = (exy + eyx) / 2.0;
-        b[5 * NP] = eyx;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) b[c * NP] = part[c];
         b[6 * NP] = r;
-        b[7 * NP] = rs;
+        b[7 * NP] = r * st;
         b[8 * NP] = ct;
         b[9 * NP] = st;
     }
+}
+
+// Volume curl fields, first half: the parts of every element-local node from
+// the current velocity (the start velocity of the next step), element-major
+// parts[e][6][NP]. One workgroup per element.
+template <int P, typename R>
+__global__ void __launch_bounds__(64) curl_parts_kernel(const CurlArgs<R> a, long nel, double *parts)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1;
+    const long e = blockIdx.x;
+    if (e >= nel) return;
+    const int ex = (int)(e % a.nx), ey = (int)((e / a.nx) % a.ny), ez = (int)(e / ((long)a.nx * a.ny));
+    for (int n = threadIdx.x; n < NP; n += 64) {
+        const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
+        double part[6];
+        curl_parts<P, R>(a, ex, ey, ez, i, j, k, part);
+#pragma unroll
+        for (int c = 0; c < 6; ++c) parts[(e * 6 + c) * NP + n] = part[c];
+    }
+}
+
+// Volume curl fields, second half: rot_v{x,y,z} (comp 0, 1, 2) of every
+// element-local node from the stored parts and the current velocity, in the
+// reference's 6-D layout out[ex + nx (ey + ny (ez + nz (i + P1 (j + P1 k))))].
+template <int P, typename R>
+__global__ void __launch_bounds__(256) curl_field_kernel(const CurlArgs<R> a, int nz, long nn,
+                                                         const double *parts, int comp, double *out)
+{
+    constexpr int P1 = P + 1, NP = P1 * P1 * P1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn) return;
+    const long e = t / NP;
+    const int n = (int)(t - e * NP);
+    const int ex = (int)(e % a.nx), ey = (int)((e / a.nx) % a.ny), ez = (int)(e / ((long)a.nx * a.ny));
+    const int i = n % P1, j = (n / P1) % P1, k = n / (P1 * P1);
+    double part[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) part[c] = parts[(e * 6 + c) * NP + n];
+    const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
+    const double vxn = curl_vel(a.vx, a.lx, u), vyn = curl_vel(a.vy, a.ly, u);
+    const double r = a.tr[ez * P1 + k].r, st = a.tth[ex * P1 + i].sin_t, ct = a.tth[ex * P1 + i].cot;
+    const long o = ex + (long)a.nx * (ey + (long)a.ny * (ez + (long)nz * (i + (long)P1 * (j + (long)P1 * k))));
+    out[o] = curl_formula(comp, part, vxn, vyn, r, r * st, ct, st);
 }
 
 template <int P, typename R>

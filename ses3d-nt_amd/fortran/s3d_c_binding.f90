@@ -38,7 +38,8 @@ MODULE ses3d_hip_c
         S3D_SXX = 3, S3D_SXY = 4, S3D_SXZ = 5, S3D_SYY = 6, S3D_SYZ = 7, S3D_SZZ = 8, &
         S3D_MM = 18, S3D_TAPER = 19, S3D_GRAD_RHO = 20, S3D_GRAD_CS = 21, &
         S3D_GRAD_CP = 22, S3D_GRAD_CSH = 23, S3D_GRAD_CSV = 24, &
-        S3D_EXX = 26, S3D_EXY = 27, S3D_EXZ = 28, S3D_EYY = 29, S3D_EYZ = 30, S3D_EZZ = 31
+        S3D_EXX = 26, S3D_EXY = 27, S3D_EXZ = 28, S3D_EYY = 29, S3D_EYZ = 30, S3D_EZZ = 31, &
+        S3D_ROT_VX = 50, S3D_ROT_VY = 51, S3D_ROT_VZ = 52
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_CH_VX = 0, S3D_CH_VY = 1, &
         S3D_CH_VZ = 2, S3D_CH_N = 3, S3D_CH_E = 4, S3D_CH_Z = 5, &
         S3D_CH_PRESSURE = 6, S3D_CH_ROT_VX = 7, S3D_CH_ROT_VY = 8, S3D_CH_ROT_VZ = 9, &
@@ -56,7 +57,7 @@ MODULE ses3d_hip_c
               s3d_synchronize, s3d_autotune, s3d_set_tile, s3d_create_shared, &
               s3d_batch_create, s3d_batch_step, s3d_batch_synchronize, s3d_batch_destroy, &
               s3d_grad_add, s3d_vrange_noalloc, s3d_set_lds_layout, s3d_get_tuning, &
-              s3d_enable_displacement_gradient, &
+              s3d_enable_displacement_gradient, s3d_capture_curl_strain, &
               s3d_timing_enable, s3d_kernel_times, s3d_device_bytes
     PUBLIC :: check, s3d_error_message
 
@@ -215,6 +216,10 @@ MODULE ses3d_hip_c
         END FUNCTION
         INTEGER(c_int) FUNCTION s3d_enable_displacement_gradient(ctx) &
                 BIND(C, NAME='s3d_enable_displacement_gradient')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_capture_curl_strain(ctx) BIND(C, NAME='s3d_capture_curl_strain')
             IMPORT :: c_int, c_ptr
             TYPE(c_ptr), VALUE :: ctx
         END FUNCTION

@@ -20,9 +20,9 @@ PROGRAM ses3d_nt
     USE channels_mod, ONLY : source_cls, receiver_cls, parse_source_file, parse_receiver_file, &
                              count_receiver_channels
     USE snapshots_mod, ONLY : write_snapshots, check_snapshot_attributes, write_netcdf, &
-                              needs_displacement_gradient
+                              needs_displacement_gradient, curl_snapshot_due
     USE grad_mod, ONLY : ses3d_grad
-    USE ses3d_hip_c, ONLY : check, s3d_enable_displacement_gradient
+    USE ses3d_hip_c, ONLY : check, s3d_enable_displacement_gradient, s3d_capture_curl_strain
     IMPLICIT NONE
     CHARACTER(LEN=512) :: conf_file_name
     TYPE(source_cls), ALLOCATABLE :: sources(:)
@@ -70,6 +70,7 @@ PROGRAM ses3d_nt
 
         every = MAX(1, configuration%nt / 10)
         DO it = 1, configuration%nt
+            IF (curl_snapshot_due(it)) CALL check(s3d_capture_curl_strain(ctx), 'capture_curl_strain')
             CALL ses3d_evolution()
             DO i = 1, SIZE(sources)
                 CALL sources(i)%add(it)

@@ -8,9 +8,11 @@
 !! sensitivity densities grad_rho grad_cs grad_cp grad_csh grad_csv
 !! (src/volume_snapshot.f90:281-293), and the model: rho lambda mu vs/cs
 !! vp/cp a b c q (:254-279), rebuilt on the host from the model files or the
-!! built-in model, and the displacement gradient exx exy exz eyy eyz ezz
-!! (:220-236; integrated on the device when a snapshot asks for it). Not
-!! provided: the volume curl fields rot_vx, rot_vy, rot_vz.
+!! built-in model, the displacement gradient exx exy exz eyy eyz ezz
+!! (:220-236; integrated on the device when a snapshot asks for it) and the
+!! curl fields rot_vx rot_vy rot_vz (:238-245; the strain-rate parts of the
+!! step's start velocity are captured on the device before a step whose
+!! snapshot asks for them).
 MODULE snapshots_mod
     USE, INTRINSIC :: ISO_C_BINDING
     USE ses3d_hip_c
@@ -18,7 +20,8 @@ MODULE snapshots_mod
     USE evolution_mod, ONLY : model_param
     IMPLICIT NONE
     PRIVATE
-    PUBLIC :: write_snapshots, check_snapshot_attributes, write_netcdf, needs_displacement_gradient
+    PUBLIC :: write_snapshots, check_snapshot_attributes, write_netcdf, needs_displacement_gradient, &
+              curl_snapshot_due
 
 CONTAINS
 
@@ -48,7 +51,7 @@ CONTAINS
                 CASE ('', 'vx', 'vy', 'vz', 'sxx', 'sxy', 'sxz', 'syy', 'syz', 'szz', 'pressure', 'taper', &
                       'grad_rho', 'grad_cs', 'grad_cp', 'grad_csh', 'grad_csv', 'rho', 'lambda', 'mu', &
                       'vs', 'cs', 'vp', 'cp', 'a', 'b', 'c', 'q', &
-                      'exx', 'exy', 'exz', 'eyy', 'eyz', 'ezz')
+                      'exx', 'exy', 'exz', 'eyy', 'eyz', 'ezz', 'rot_vx', 'rot_vy', 'rot_vz')
                 CASE DEFAULT
                     CALL abort('Snapshot attribute not supported on the GPU path: ' // &
                                TRIM(configuration%outputs(i)%attributes(j)))
@@ -67,6 +70,24 @@ CONTAINS
                 SELECT CASE (TRIM(configuration%outputs(i)%attributes(j)))
                 CASE ('exx', 'exy', 'exz', 'eyy', 'eyz', 'ezz')
                     needs_displacement_gradient = .TRUE.
+                END SELECT
+            END DO
+        END DO
+    END FUNCTION
+
+    !> Whether a snapshot due at step it asks for a curl field (rot_v*):
+    !! the strain-rate parts of the step's start velocity are then captured
+    !! before the step.
+    LOGICAL FUNCTION curl_snapshot_due(it)
+        INTEGER, INTENT(IN) :: it
+        INTEGER :: i, j
+        curl_snapshot_due = .FALSE.
+        DO i = 1, SIZE(configuration%outputs)
+            IF (.NOT. due(configuration%outputs(i), it)) CYCLE
+            DO j = 1, SIZE(configuration%outputs(i)%attributes)
+                SELECT CASE (TRIM(configuration%outputs(i)%attributes(j)))
+                CASE ('rot_vx', 'rot_vy', 'rot_vz')
+                    curl_snapshot_due = .TRUE.
                 END SELECT
             END DO
         END DO
@@ -124,6 +145,10 @@ CONTAINS
                 CASE ('eyy'); CALL get(ctx, S3D_EYY, f)
                 CASE ('eyz'); CALL get(ctx, S3D_EYZ, f)
                 CASE ('ezz'); CALL get(ctx, S3D_EZZ, f)
+                ! curl of the velocity as the reference forms it (src/volume_snapshot.f90:238-245)
+                CASE ('rot_vx'); CALL get(ctx, S3D_ROT_VX, f)
+                CASE ('rot_vy'); CALL get(ctx, S3D_ROT_VY, f)
+                CASE ('rot_vz'); CALL get(ctx, S3D_ROT_VZ, f)
                 CASE ('rho')
                     CALL model_param(1, f)
                     CALL rho_of(f)

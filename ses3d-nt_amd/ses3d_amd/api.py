@@ -29,6 +29,8 @@ MM, TAPER = 18, 19
 GRAD_RHO, GRAD_CS, GRAD_CP, GRAD_CSH, GRAD_CSV = 20, 21, 22, 23, 24
 # symmetric displacement gradient (after enable_displacement_gradient)
 EXX, EXY, EXZ, EYY, EYZ, EZZ = 26, 27, 28, 29, 30, 31
+# volume curl fields (after capture_curl_strain and one step)
+ROT_VX, ROT_VY, ROT_VZ = 50, 51, 52
 MXX = 32
 CH = {"vx": 0, "vy": 1, "vz": 2, "n": 3, "e": 4, "z": 5, "pressure": 6,
       "rot_vx": 7, "rot_vy": 8, "rot_vz": 9,
@@ -46,6 +48,7 @@ EXPORTS = [
     "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step", "s3d_set_tile",
     "s3d_batch_synchronize", "s3d_batch_destroy", "s3d_grad_add", "s3d_vrange_noalloc",
     "s3d_set_lds_layout", "s3d_get_tuning", "s3d_enable_displacement_gradient",
+    "s3d_capture_curl_strain",
 ]
 
 
@@ -107,6 +110,7 @@ def lib():
         L.s3d_set_field.argtypes = [vp, C.c_int, dp]
         L.s3d_grad_add.argtypes = [vp, dp, dp, dp, C.c_int]
         L.s3d_enable_displacement_gradient.argtypes = [vp]
+        L.s3d_capture_curl_strain.argtypes = [vp]
         L.s3d_vrange.argtypes = [vp, dp]
         L.s3d_vrange_noalloc.argtypes = [vp, dp]
         L.s3d_kinetic_energy.argtypes = [vp, C.POINTER(C.c_double)]
@@ -297,6 +301,12 @@ class Simulation:
         """Integrate the displacement gradient from the next step on
         (s3d_enable_displacement_gradient); read it with get(EXX .. EZZ)."""
         _chk(lib().s3d_enable_displacement_gradient(self._h))
+
+    def capture_curl_strain(self):
+        """Store the strain-rate parts of the current velocity for the volume
+        curl fields; after one more step get(ROT_VX .. ROT_VZ) returns them
+        (s3d_capture_curl_strain)."""
+        _chk(lib().s3d_capture_curl_strain(self._h))
 
     def grad_add(self, vx_fw, vy_fw, vz_fw, increment):
         """grad_rho += increment*dt*(v_fw . v)/J with a forward velocity

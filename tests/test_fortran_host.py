@@ -54,7 +54,8 @@ def test_forward_run_matches_oracle(tmp_path, precision, tol):
     nt = 200
     case = synth.homogeneous(nt=nt, nx=8, ny=8, nz=5, n_rec=5)
     snap = ("\n&output_raw\n timestep_start = 100\n timestep_increment = 100\n"
-            " timestep_end = 200\n attributes = 'vz' 'pressure' 'EXY' 'ezz'\n override = .TRUE.\n"
+            " timestep_end = 200\n attributes = 'vz' 'pressure' 'EXY' 'ezz' 'rot_vz'\n"
+            " override = .TRUE.\n"
             " prefix = '%s/'\n/\n" % str(tmp_path))
     cfg = write_config(case, str(tmp_path), precision=precision, extra=snap)
     r = run(cfg)
@@ -71,6 +72,12 @@ def test_forward_run_matches_oracle(tmp_path, precision, tol):
     exy100 = np.fromfile(os.path.join(str(tmp_path), "exy_100.vol"), dtype=dtype)
     dxuy, dyux = o.get(oracle.DXUX + 1), o.get(oracle.DXUX + 3)
     assert rel_l2(exy100, ((dxuy + dyux) / 2).ravel()) < tol
+    # rot_vz = (exy + vy cot - eyx / sin) / r with the strain rates of the
+    # step's start velocity and vy at its end (src/volume_snapshot.f90:244-245)
+    th, r, vy = o.get(oracle.THETA), o.get(oracle.R), o.get(oracle.VY)
+    rot_vz = (o.get(oracle.EXY) + vy / np.tan(th) - o.get(oracle.EYX) / np.sin(th)) / r
+    rz100 = np.fromfile(os.path.join(str(tmp_path), "rot_vz_100.vol"), dtype=dtype)
+    assert rel_l2(rz100, rot_vz.ravel()) < 10 * tol
     o.step(nt - 100)
     ezz200 = np.fromfile(os.path.join(str(tmp_path), "ezz_200.vol"), dtype=dtype)
     assert rel_l2(ezz200, o.get(oracle.DXUX + 8).ravel()) < tol

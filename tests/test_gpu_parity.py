@@ -422,6 +422,34 @@ def test_batch_with_bf16_compensation_matches_individual_runs():
     b.close()
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-4)])
+def test_volume_curl_fields_match_oracle(precision, tol):
+    """Snapshot attributes rot_vx, rot_vy, rot_vz (src/volume_snapshot.f90:
+    238-245): the strain rates the step formed from its start velocity (the
+    oracle's strain-rate fields after the step, pinned by the closed forms
+    of test_oracle_physics.py) combined with the velocity at its end."""
+    grid = small_grid(nx=4, ny=3, nz=3, taper_width=1)
+    rng = np.random.default_rng(31)
+    o, s = _pair(grid, random_model(grid, rng, visco=True), precision=precision)
+    _random_state(o, s, grid, rng, visco=True)
+    o.step(3)
+    s.step(3)
+    s.capture_curl_strain()
+    o.step(1)
+    s.step(1)
+    th, r = o.get(oracle.THETA), o.get(oracle.R)
+    vx, vy = o.get(oracle.VX), o.get(oracle.VY)
+    rs, ct = r * np.sin(th), 1.0 / np.tan(th)
+    ref = {api.ROT_VX: -vy / r + o.get(oracle.EYZ) / rs + o.get(oracle.EZY),
+           api.ROT_VY: -o.get(oracle.EZX) + (vx - o.get(oracle.EXZ)) / r,
+           api.ROT_VZ: (o.get(oracle.EXY) + vy * ct - o.get(oracle.EYX) / np.sin(th)) / r}
+    for f, want in ref.items():
+        assert rel_l2(s.get(f), want) < tol, f
+    s.step(1)   # the captured parts belong to the previous step now
+    with pytest.raises(api.S3dError, match="capture_curl_strain"):
+        s.get(api.ROT_VX)
+
+
 @pytest.mark.parametrize("precision,tol,lpd", [("fp64", 1e-10, 4), ("fp32", 1e-4, 4),
                                                ("fp64", 1e-10, 2), ("fp64", 1e-10, 7)])
 def test_curl_channels_match_oracle(precision, tol, lpd):
