@@ -84,7 +84,7 @@ enum {
 /* Receiver channels (reference src/config/channel_mod.f90:270-336). The curl
  * channels rot_vx/vy/vz (:620-774) combine the strain rates the step forms
  * from its start velocity with the velocity at its end, as the reference's
- * records do; they are not available in batches. The source-echo channels
+ * records do. The source-echo channels
  * mxx .. mzz and fx, fy, fz (record_m**, record_f*, :871-1085) sample the
  * moment-tensor glut and the single force that the step's add(it) wrote into
  * the receiver's element, i.e. sum(vm * src_**) with the source-time sample

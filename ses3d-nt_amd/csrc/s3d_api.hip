@@ -776,6 +776,16 @@ struct ExpandRun {
 };
 
 template <int P, typename R>
+struct CurlStrainRun {
+    static int run(s3d_ctx *c, hipStream_t st)
+    {
+        hipLaunchKernelGGL((curl_strain_kernel<P, R>), dim3(c->nrot), dim3(64), 0, st, curl_args<P, R>(c));
+        HIPCHK(hipGetLastError());
+        return 0;
+    }
+};
+
+template <int P, typename R>
 struct CurlPartsRun {
     static int run(s3d_ctx *c)
     {
@@ -1042,7 +1052,6 @@ int batch_build_group_e(BatchGroup &G)
             r = rec_args<R>(c);
             // the receivers read the stress their step wrote
             r.sig = (decltype(r.sig))((const R *)state_parts(c, h ? w1 : w0).sig);
-            r.rec_rot = nullptr;   // no curl channels in a batch (refused at creation)
         }
         es[q + 1] = es[q] + (int)nblocks(c->nel, S::EPB);
         ns[q + 1] = ns[q] + (int)nblocks(c->nuniq, 256);
@@ -1208,6 +1217,14 @@ void batch_free_groups(s3d_batch *b)
 // node and receiver kernels, then every counter advances
 int batch_launch_step(s3d_batch *b)
 {
+    // members with curl receivers: the strain parts of the start velocity,
+    // before the element kernels change it
+    for (s3d_ctx *c : b->sims)
+        if (c->nrot > 0) {
+            const int rc = c->fp32 ? with_p<CurlStrainRun, float>(c->p, c, b->stream)
+                                   : with_p<CurlStrainRun, double>(c->p, c, b->stream);
+            if (rc) return rc;
+        }
     for (auto &G : b->groups) {
         G.phase = b->phase;
         if (int rc = batch_group_call(1, G, b->stream)) return rc;
@@ -2198,9 +2215,6 @@ int s3d_batch_create(int n, s3d_ctx *const *sims, s3d_batch **out)
         if (!sims[q] || !sims[q]->inited) return fail("batch members must be initialised (s3d_init)");
         if (sims[q]->dev != sims[0]->dev) return fail("batch members must share one device");
         if (sims[q]->o.bf16_static) return fail("BF16 static fields are not supported in a batch");
-        for (int ch : sims[q]->rec_ch)
-            if (ch >= S3D_CH_ROT_VX && ch <= S3D_CH_ROT_VZ)
-                return fail("curl receiver channels are not supported in a batch");
         for (int r = 0; r < q; ++r)
             if (sims[r] == sims[q]) return fail("a simulation appears twice in the batch");
     }
