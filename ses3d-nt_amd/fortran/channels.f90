@@ -110,6 +110,44 @@ CONTAINS
     !> Locate every source, build its signature and hand it to the device
     !! (src/config/source.f90:274-327, 449-536). At least one source must
     !! lie in the domain.
+    !> Locate a point (lat, lon in degrees, depth in m) with s3d_locate and
+    !! warn as the reference's point constructor does
+    !! (src/config/point.f90:207-236): a point outside the domain is dropped,
+    !! a point in the tapering boundaries (in_boundary, :686-712: within
+    !! taper_width elements of the four sides or the bottom) is kept.
+    LOGICAL FUNCTION located(ctx, lat, lon, depth, exyz, vm)
+        TYPE(c_ptr), INTENT(IN) :: ctx
+        REAL(c_double), INTENT(IN) :: lat, lon, depth
+        INTEGER(c_int), INTENT(OUT) :: exyz(3)
+        REAL(c_double), INTENT(OUT) :: vm(:)
+        REAL(c_double), PARAMETER :: pi = 3.141592653589793238_c_double, &
+            earth_radius = 6371000.0_c_double
+        REAL(c_double) :: x, y, z, x0, x1, y0, y1, z0, z1, tw
+        CHARACTER(LEN=256) :: msg
+        WRITE(msg, '("Point (", G0, ", ", G0, ", ", G0, ")")') lat, lon, depth
+        located = s3d_locate(ctx, lat, lon, depth, exyz, vm) == 0
+        IF (.NOT. located) THEN
+            CALL warn(TRIM(msg) // ' not in computational domain!')
+            RETURN
+        END IF
+        ASSOCIATE (cf => configuration)
+        x = (90.0_c_double - lat) * pi / 180.0_c_double
+        y = lon * pi / 180.0_c_double
+        z = earth_radius - depth
+        x0 = (90.0_c_double - cf%lat_max) * pi / 180.0_c_double
+        x1 = (90.0_c_double - cf%lat_min) * pi / 180.0_c_double
+        y0 = cf%lon_min * pi / 180.0_c_double
+        y1 = cf%lon_max * pi / 180.0_c_double
+        z0 = cf%rad_min
+        z1 = cf%rad_max
+        tw = REAL(cf%taper_width, c_double)
+        IF (x0 + tw * (x1 - x0) / cf%nx > x .OR. x > x1 - tw * (x1 - x0) / cf%nx .OR. &
+            y0 + tw * (y1 - y0) / cf%ny > y .OR. y > y1 - tw * (y1 - y0) / cf%ny .OR. &
+            z0 + tw * (z1 - z0) / cf%nz > z) &
+            CALL warn(TRIM(msg) // ' located in tapering boundaries!')
+        END ASSOCIATE
+    END FUNCTION
+
     SUBROUTINE parse_source_file(ctx, sources)
         TYPE(c_ptr), INTENT(IN) :: ctx
         TYPE(source_cls), ALLOCATABLE, INTENT(OUT) :: sources(:)
@@ -132,9 +170,7 @@ CONTAINS
                     CYCLE
                 END IF
             END IF
-            IF (s3d_locate(ctx, s%lat, s%lon, s%depth, exyz, vm) /= 0) THEN
-                WRITE(*, '("WARNING: source (", G0, ", ", G0, ", ", G0, ") not in computational domain!")') &
-                    s%lat, s%lon, s%depth
+            IF (.NOT. located(ctx, s%lat, s%lon, s%depth, exyz, vm)) THEN
                 IF (ALLOCATED(comp)) DEALLOCATE(comp)
                 IF (ALLOCATED(comp3)) DEALLOCATE(comp3)
                 CYCLE
@@ -252,11 +288,7 @@ CONTAINS
             r%info = configuration%receivers(i)
             CALL receiver_channels(r)
             IF (r%nch == 0) CYCLE
-            IF (s3d_locate(ctx, r%info%lat, r%info%lon, r%info%depth, exyz, vm) /= 0) THEN
-                WRITE(*, '("WARNING: receiver ", A, " not in computational domain!")') &
-                    TRIM(r%info%station)
-                CYCLE
-            END IF
+            IF (.NOT. located(ctx, r%info%lat, r%info%lon, r%info%depth, exyz, vm)) CYCLE
             CALL check(s3d_add_receiver(ctx, exyz, vm, INT(r%nch, c_int), r%codes, id), &
                        'add_receiver')
             r%id = id

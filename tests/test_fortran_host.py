@@ -287,6 +287,23 @@ def test_reference_example_configs_parse_unchanged(cfg):
     assert (" %d receivers with" % n_rec) in r.stdout
 
 
+@pytest.mark.gpu
+def test_point_location_warnings(tmp_path):
+    """As the reference's point constructor (src/config/point.f90:207-236): a
+    receiver outside the section is dropped with a warning, one inside the
+    tapering boundaries is kept with a warning, interior ones pass silently."""
+    from dataclasses import replace
+    case = synth.homogeneous(nt=20, nx=6, ny=6, nz=5, n_rec=2)
+    extra = [synth.Receiver(4.0, 0.0, 0.0, "TAPR"), synth.Receiver(6.0, 0.0, 0.0, "OUTS")]
+    cfg = write_config(replace(case, receivers=case.receivers + extra), str(tmp_path))
+    r = run(cfg)
+    print(r.stdout[-800:], r.stderr[-400:])
+    assert r.returncode == 0
+    assert r.stdout.count("located in tapering boundaries!") == 1
+    assert r.stdout.count("not in computational domain!") == 1
+    assert "Found 3 receivers." in r.stdout
+
+
 def test_tile_option_checked(tmp_path):
     case = synth.homogeneous(nt=60, nx=6, ny=6, nz=5, n_rec=2)
     cfg = write_config(case, str(tmp_path))
