@@ -126,8 +126,9 @@ def load_sims(case, modes, slab, remap=True, tune=None, state_buffers=0):
             sim.set_tile(tune[0])
             sim.set_lds_layout(tune[1])
         sim.tile, sim.remap, sim.layout = sim.tuning()
-        log("[bench] %s tuning: %d elements per workgroup, XCD order %d, LDS layout %d"
-            % (name, sim.tile, sim.remap, sim.layout))
+        sim.early = sim.store_order()
+        log("[bench] %s tuning: %d elements per workgroup, XCD order %d, LDS layout %d, "
+            "early stores %d" % (name, sim.tile, sim.remap, sim.layout, sim.early))
         nrec = 0
         for s_ in case.sources:
             loc = sim.locate(s_.lat, s_.lon, s_.depth)
@@ -557,9 +558,10 @@ def main():
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": None,
             "kernel": "elem_kernel<4,%s,visco>, %d elements per workgroup, %s LDS layout, %s "
-                      "workgroup order" % (rtype, tile,
-                                           "node-interleaved" if sims[mode].layout else "planar",
-                                           "XCD-aware" if sims[mode].remap else "plain"),
+                      "workgroup order, state stored %s the divergence"
+                      % (rtype, tile, "node-interleaved" if sims[mode].layout else "planar",
+                         "XCD-aware" if sims[mode].remap else "plain",
+                         "before" if sims[mode].early else "after"),
             "algorithmic_bytes_per_launch": eb,
             "algorithmic_bytes_per_node": eb / N,
             # the element force the element kernel writes for the node kernel

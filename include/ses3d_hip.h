@@ -151,7 +151,8 @@ int s3d_init(s3d_ctx *ctx);
 int s3d_cfl(s3d_ctx *ctx, double *courant);
 
 /* Autotuner: times the element-kernel tile variants (elements per
- * workgroup), the XCD-aware block order and the LDS staging layout on this
+ * workgroup), the XCD-aware block order, the LDS staging layout and the
+ * store order (s3d_set_store_order) on this
  * mesh and keeps the fastest; *tile gets the elements per workgroup chosen
  * and *ms the element kernel time (s3d_get_tuning gives the rest). Needs the state at rest (after s3d_init, before s3d_set_field
  * and s3d_step): the trial launches then leave every field unchanged. No
@@ -169,6 +170,16 @@ int s3d_set_tile(s3d_ctx *ctx, int tile);
  * results; FP64 and BF16-static simulations use 0. s3d_autotune searches
  * tile x block order x layout. No counterpart in the reference. */
 int s3d_set_lds_layout(s3d_ctx *ctx, int layout);
+
+/* Where the FP32 element kernel stores the new stress, its compensation and
+ * the memory variables: 0 after the weak divergence (default), 1 before it
+ * (their registers are then free during the divergence, which raises the
+ * occupancy of some GLL orders: order 6 with FP32 compensation of the
+ * velocity only runs about 6 % faster). Bit-identical results; planar LDS
+ * layout only; FP64 and BF16-static simulations use 0. s3d_autotune
+ * searches it with the layout. No counterpart in the reference. */
+int s3d_set_store_order(s3d_ctx *ctx, int early);
+int s3d_get_store_order(s3d_ctx *ctx, int *early);
 
 /* The element tile (elements per workgroup), block order (1: XCD-aware) and
  * LDS layout in use, e.g. as chosen by s3d_autotune. */

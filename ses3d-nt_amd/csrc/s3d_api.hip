@@ -225,6 +225,7 @@ struct s3d_ctx {
     int cur = 0;
     int epb = 0;               // element tile (0 = default), see s3d_autotune
     int lds_layout = 0;        // LDS staging layout of the FP32 element kernel, see Stage
+    bool early_stores = false; // FP32 element kernel stores the state before the divergence
     int curl_it = -1;          // steps completed at the last s3d_capture_curl_strain
     bool fields_set = false;   // s3d_set_field was called (state not at rest)
     // s3d_create_shared: the simulation whose static fields (moduli, tau, rho,
@@ -340,6 +341,9 @@ int upload_tables(s3d_ctx *c, const std::vector<double> &sin1)
 }
 
 size_t rsize(const s3d_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double); }
+
+// the element kernel's variant (elem_body's LL) for the layout and store order
+int kernel_variant(const s3d_ctx *c) { return c->early_stores ? 2 : c->lds_layout; }
 
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -478,8 +482,10 @@ void launch_elem_ll(int ks, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
         hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
 }
 
-// stress-compensation mode and LDS layout are kernel template parameters
-// (FP32 only; FP64 and the BF16-static kernels use the planar layout)
+// stress-compensation mode and kernel variant (elem_body's LL: planar,
+// node-interleaved, planar with the state stored before the divergence) are
+// kernel template parameters (FP32 only; FP64 and the BF16-static kernels
+// use variant 0)
 template <int P, int E, typename R, bool VISCO, bool S16>
 int launch_elem_ks(int ks, int ll, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
 {
@@ -489,6 +495,9 @@ int launch_elem_ks(int ks, int ll, const ElemArgs<R> &ea, hipStream_t st, unsign
         if constexpr (!S16) {
             if (ll == 1) {
                 launch_elem_ll<P, E, R, VISCO, S16, 1>(ks, ea, st, ge);
+                done = true;
+            } else if (ll == 2) {
+                launch_elem_ll<P, E, R, VISCO, S16, 2>(ks, ea, st, ge);
                 done = true;
             }
         }
@@ -505,7 +514,7 @@ int launch_elem_e(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st)
 {
     using S = Shape<P, E>;
     const unsigned ge = nblocks(c->nel, S::EPB);
-    const int ks = c->o.kahan_sigma, ll = c->lds_layout;
+    const int ks = c->o.kahan_sigma, ll = kernel_variant(c);
     if constexpr (std::is_same<R, float>::value && E == 0) {
         if (c->o.bf16_static)   // default tile only
             return c->visco ? launch_elem_ks<P, 0, float, true, true>(ks, 0, ea, st, ge)
@@ -554,13 +563,14 @@ int autotune_p(s3d_ctx *c, int *tile_out, double *ms_out)
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     double best = 1e300;
-    int best_tile = c->epb, best_remap = c->o.xcd_remap, best_ll = c->lds_layout;
-    const int nll = (c->fp32 && !c->o.bf16_static) ? 2 : 1;
+    int best_tile = c->epb, best_remap = c->o.xcd_remap, best_ll = kernel_variant(c);
+    const int nll = (c->fp32 && !c->o.bf16_static) ? 3 : 1;
     for (int tile : tile_candidates<P>(c->fp32, c->o.bf16_static))
         for (int ll = 0; ll < nll; ++ll)
             for (int remap = 0; remap < 2; ++remap) {
                 c->o.xcd_remap = remap;
-                c->lds_layout = ll;
+                c->lds_layout = ll == 1 ? 1 : 0;
+                c->early_stores = ll == 2;
                 ElemArgs<R> ea = elem_args<P, R>(c);
                 if (int rc = launch_elem<P, R>(c, ea, c->stream, tile)) return rc;   // warm-up
                 HIPCHK(hipEventRecord(e0, c->stream));
@@ -581,7 +591,8 @@ int autotune_p(s3d_ctx *c, int *tile_out, double *ms_out)
     (void)hipEventDestroy(e1);
     c->epb = best_tile;
     c->o.xcd_remap = best_remap;
-    c->lds_layout = best_ll;
+    c->lds_layout = best_ll == 1 ? 1 : 0;
+    c->early_stores = best_ll == 2;
     drop_graph(c);
     if (tile_out) *tile_out = best_tile == 0 ? default_epb<P>() : best_tile;
     if (ms_out) *ms_out = best;
@@ -996,7 +1007,7 @@ int upload_receivers(s3d_ctx *c)
 struct BatchGroup {
     int p = 0;
     int epb = 0;
-    int ll = 0;                 // LDS layout
+    int ll = 0;                 // element kernel variant (elem_body's LL)
     int ks = 0;                 // stress compensation mode
     bool fp32 = false, visco = false, kahan = false;
     std::vector<s3d_ctx *> sims;
@@ -1105,10 +1116,14 @@ int batch_launch_group_k(const BatchGroup &G, hipStream_t st)
 {
     using S = Shape<P, E>;
     if constexpr (std::is_same<R, float>::value) {
-        if (G.ll == 1) {
-            hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 1>), dim3(G.ewg), dim3(S::NT), 0, st,
-                               (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size(),
-                               (const int2 *)G.d_eown);
+        if (G.ll == 1 || G.ll == 2) {
+            const ElemArgs<R> *args = (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size();
+            if (G.ll == 1)
+                hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 1>), dim3(G.ewg), dim3(S::NT), 0, st,
+                                   args, (const int2 *)G.d_eown);
+            else
+                hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 2>), dim3(G.ewg), dim3(S::NT), 0, st,
+                                   args, (const int2 *)G.d_eown);
             HIPCHK(hipGetLastError());
             return 0;
         }
@@ -1278,7 +1293,7 @@ int batch_build(s3d_batch *b)
         for (const s3d_ctx *c : G.sims)
             if (c->nel > big->nel) big = c;
         G.epb = big->epb;
-        G.ll = big->fp32 ? big->lds_layout : 0;
+        G.ll = big->fp32 ? kernel_variant(big) : 0;
     }
     for (auto &G : b->groups)
         if (int rc = batch_group_call(0, G, b->stream)) return rc;
@@ -1464,6 +1479,7 @@ int s3d_create_shared(s3d_ctx *base, s3d_ctx **out)
     c->courant = m->courant;
     c->epb = m->epb;
     c->lds_layout = m->lds_layout;
+    c->early_stores = m->early_stores;
     // static fields: the model's, not copied
     c->dt_inv_mm = m->dt_inv_mm; c->mm64 = m->mm64; c->lam = m->lam; c->mu = m->mu; c->tau = m->tau;
     c->rho = m->rho; c->ca = m->ca; c->cb = m->cb; c->cc = m->cc; c->tth = m->tth; c->tr = m->tr;
@@ -1717,8 +1733,30 @@ int s3d_set_lds_layout(s3d_ctx *c, int layout)
     if (layout != 0 && layout != 1) return fail("LDS layout must be 0 (planar) or 1 (node-interleaved)");
     if (layout != 0 && (!c->fp32 || c->o.bf16_static))
         return fail("FP64 and BF16-static simulations use the planar LDS layout");
+    if (layout == 1 && c->early_stores)
+        return fail("the node-interleaved LDS layout stores the state after the divergence");
     c->lds_layout = layout;
     drop_graph(c);
+    return 0;
+}
+
+int s3d_set_store_order(s3d_ctx *c, int early)
+{
+    if (!c) return fail("null argument");
+    if (early != 0 && early != 1) return fail("store order must be 0 (after the divergence) or 1 (before)");
+    if (early && (!c->fp32 || c->o.bf16_static))
+        return fail("FP64 and BF16-static simulations store the state after the divergence");
+    if (early && c->lds_layout == 1)
+        return fail("the node-interleaved LDS layout stores the state after the divergence");
+    c->early_stores = early != 0;
+    drop_graph(c);
+    return 0;
+}
+
+int s3d_get_store_order(s3d_ctx *c, int *early)
+{
+    if (!c || !early) return fail("null argument");
+    *early = c->early_stores ? 1 : 0;
     return 0;
 }
 

@@ -244,7 +244,12 @@ enum { G_SIN = 0, G_ISIN, G_COT, G_R, G_IR, G_R2, G_PX, G_PY, G_PZ, G_N };
 
 // Element kernel body: the workgroup advances tile `blk` (EPB elements, one
 // lane per element-local node). S16: the static per-node fields are stored
-// in BF16 (ElemArgs::s16). KS: stress compensation (see load_tile).
+// in BF16 (ElemArgs::s16). KS: stress compensation (see load_tile). LL: the
+// kernel variant, 0 planar LDS staging, 1 node-interleaved staging (see
+// Stage), 2 planar staging with the new stress, compensation and memory
+// variables stored before the weak divergence instead of after it (their
+// registers are then free during the divergence, which raises the occupancy
+// of some orders and compensation modes); the arithmetic is the same.
 template <int P, int E, typename R, bool VISCO, bool S16, int KS, int LL>
 __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
@@ -252,7 +257,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     using C = coef_t<R>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB, NE = EPB;
     constexpr int NS = state_components(KS, VISCO);
-    __shared__ Stage<R, LL, EPB * NP> lds;
+    constexpr bool EARLY = LL == 2;
+    __shared__ Stage<R, LL == 1 ? 1 : 0, EPB * NP> lds;
     __shared__ C cf[4][P1 * P1];     // Dx, Dy, Dz, dl*w
     __shared__ C cw[P1];             // GLL weights
     __shared__ C gtab[NE][G_N * P1]; // geometry rows of the tile's elements
@@ -528,6 +534,34 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             if (tx + ty + tz != C(0)) dmp = (R)taper_damp(tx, ty, tz, a.alpha);
         }
 
+        // ---- store tapered stress (K13) and memory variables ----
+        auto store_state = [&]() {
+            if constexpr (KS == 1) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
+                    a.sig_lo_w[em_index(ev, NS, c, NP, node)] = lo[c] - lo[c] * dmp;
+                }
+            } else if constexpr (KS == 2) {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+                    a.sig_lo2_w[em_index(ev, NS, q, NP, node)] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
+                                                       (float)(lo[2 * q + 1] - lo[2 * q + 1] * dmp));
+            } else {
+#pragma unroll
+                for (int c = 0; c < 6; ++c) a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
+            }
+            if constexpr (VISCO) {
+#pragma unroll
+                for (int c = 0; c < 18; ++c) a.mem_w[em_index(ev, NS, c, NP, node)] = M[c];
+            }
+        };
+        if constexpr (EARLY) {
+            if (valid) store_state();
+        }
+
         __syncthreads();
 
         // ---- K8: weak divergence ----
@@ -572,28 +606,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             for (int c = 0; c < 3; ++c)
                 a.force[em_index(ev, 3, c, NP, node)] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
 
-            // ---- store tapered stress (K13) and memory variables ----
-            if constexpr (KS == 1) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
-                    a.sig_lo_w[em_index(ev, NS, c, NP, node)] = lo[c] - lo[c] * dmp;
-                }
-            } else if constexpr (KS == 2) {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
-#pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    a.sig_lo2_w[em_index(ev, NS, q, NP, node)] = pack_bf16((float)(lo[2 * q] - lo[2 * q] * dmp),
-                                                       (float)(lo[2 * q + 1] - lo[2 * q + 1] * dmp));
-            } else {
-#pragma unroll
-                for (int c = 0; c < 6; ++c) a.sig_w[em_index(ev, NS, c, NP, node)] = sg[c] - sg[c] * dmp;
-            }
-            if constexpr (VISCO) {
-#pragma unroll
-                for (int c = 0; c < 18; ++c) a.mem_w[em_index(ev, NS, c, NP, node)] = M[c];
-            }
+            if constexpr (!EARLY) store_state();
         }
     }
 }

@@ -58,6 +58,7 @@ MODULE ses3d_hip_c
               s3d_batch_create, s3d_batch_step, s3d_batch_synchronize, s3d_batch_destroy, &
               s3d_grad_add, s3d_vrange_noalloc, s3d_set_lds_layout, s3d_get_tuning, &
               s3d_enable_displacement_gradient, s3d_capture_curl_strain, &
+              s3d_set_store_order, s3d_get_store_order, &
               s3d_timing_enable, s3d_kernel_times, s3d_device_bytes
     PUBLIC :: check, s3d_error_message
 
@@ -222,6 +223,16 @@ MODULE ses3d_hip_c
         INTEGER(c_int) FUNCTION s3d_capture_curl_strain(ctx) BIND(C, NAME='s3d_capture_curl_strain')
             IMPORT :: c_int, c_ptr
             TYPE(c_ptr), VALUE :: ctx
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_set_store_order(ctx, early) BIND(C, NAME='s3d_set_store_order')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int), VALUE :: early
+        END FUNCTION
+        INTEGER(c_int) FUNCTION s3d_get_store_order(ctx, early) BIND(C, NAME='s3d_get_store_order')
+            IMPORT :: c_int, c_ptr
+            TYPE(c_ptr), VALUE :: ctx
+            INTEGER(c_int), INTENT(OUT) :: early
         END FUNCTION
         ! the v range without any allocation (no counterpart in the reference)
         INTEGER(c_int) FUNCTION s3d_vrange_noalloc(ctx, out2) BIND(C, NAME='s3d_vrange_noalloc')

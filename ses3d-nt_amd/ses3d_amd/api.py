@@ -48,7 +48,7 @@ EXPORTS = [
     "s3d_device_bytes", "s3d_batch_create", "s3d_batch_step", "s3d_set_tile",
     "s3d_batch_synchronize", "s3d_batch_destroy", "s3d_grad_add", "s3d_vrange_noalloc",
     "s3d_set_lds_layout", "s3d_get_tuning", "s3d_enable_displacement_gradient",
-    "s3d_capture_curl_strain",
+    "s3d_capture_curl_strain", "s3d_set_store_order", "s3d_get_store_order",
 ]
 
 
@@ -111,6 +111,8 @@ def lib():
         L.s3d_grad_add.argtypes = [vp, dp, dp, dp, C.c_int]
         L.s3d_enable_displacement_gradient.argtypes = [vp]
         L.s3d_capture_curl_strain.argtypes = [vp]
+        L.s3d_set_store_order.argtypes = [vp, C.c_int]
+        L.s3d_get_store_order.argtypes = [vp, C.POINTER(C.c_int)]
         L.s3d_vrange.argtypes = [vp, dp]
         L.s3d_vrange_noalloc.argtypes = [vp, dp]
         L.s3d_kinetic_energy.argtypes = [vp, C.POINTER(C.c_double)]
@@ -235,6 +237,16 @@ class Simulation:
         """LDS staging layout of the FP32 element kernel (s3d_set_lds_layout):
         0 planar, 1 node-interleaved."""
         _chk(lib().s3d_set_lds_layout(self._h, int(layout)))
+
+    def set_store_order(self, early):
+        """Store order of the FP32 element kernel (s3d_set_store_order): 0
+        the new state after the weak divergence, 1 before it."""
+        _chk(lib().s3d_set_store_order(self._h, int(early)))
+
+    def store_order(self):
+        x = C.c_int()
+        _chk(lib().s3d_get_store_order(self._h, C.byref(x)))
+        return x.value
 
     def tuning(self):
         """(elements per workgroup, XCD-aware block order, LDS layout) in use."""

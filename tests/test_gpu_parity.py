@@ -340,6 +340,53 @@ def test_lds_layouts_identical(tile):
     assert t in (1, 2, 4) and lay in (0, 1)
 
 
+@pytest.mark.parametrize("lpd,mode", [(4, {"kahan_sigma": "bf16"}), (6, {}), (3, {"kahan_sigma": True})])
+def test_early_state_stores_identical(lpd, mode):
+    """Storing the new state before the weak divergence changes where the
+    stores sit, not what they store: traces and fields equal the default
+    order bit for bit, with two state blocks, with one, and in a batch."""
+    case = synth.layered(seed=12, nt=20, nx=5, ny=4, nz=3, lpd=lpd, n_rec=3)
+    for nbuf in (2, 1):
+        ref, recs = api.build_case(case, precision="fp32", state_buffers=nbuf, **mode)
+        ref.step(20)
+        s, _ = api.build_case(case, precision="fp32", state_buffers=nbuf, **mode)
+        s.set_store_order(1)
+        assert s.store_order() == 1
+        s.step(20)
+        for r in range(len(recs)):
+            for ch in range(3):
+                assert np.array_equal(s.trace(r, ch), ref.trace(r, ch)), (nbuf, r, ch)
+        for f in (api.VX, api.SXZ, api.MXX + 11):
+            assert np.array_equal(s.get(f), ref.get(f)), (nbuf, f)
+    m, _ = api.build_case(case, precision="fp32", **mode)
+    m.set_store_order(1)
+    b = api.Batch([m])
+    b.step(20)
+    b.synchronize()
+    b.close()
+    for r in range(len(recs)):
+        assert np.array_equal(m.trace(r, 2), ref.trace(r, 2)), r
+
+
+def test_store_order_checks():
+    case = synth.homogeneous(nt=4, nx=4, ny=4, nz=3, n_rec=1)
+    s, _ = api.build_case(case, precision="fp32")
+    with pytest.raises(api.S3dError, match="store order must be"):
+        s.set_store_order(2)
+    s.set_lds_layout(1)
+    with pytest.raises(api.S3dError, match="node-interleaved"):
+        s.set_store_order(1)
+    s.set_lds_layout(0)
+    s.set_store_order(1)
+    with pytest.raises(api.S3dError, match="node-interleaved"):
+        s.set_lds_layout(1)
+    d, _ = api.build_case(case, precision="fp64")
+    with pytest.raises(api.S3dError, match="FP64"):
+        d.set_store_order(1)
+    t, _ = api.build_case(case, precision="fp32", autotune=True)
+    assert t.store_order() in (0, 1)
+
+
 def test_set_tile_checks():
     elastic = synth.homogeneous(nt=10, nx=5, ny=5, nz=4, n_rec=1)
     s, _ = api.build_case(elastic, precision="fp32")
