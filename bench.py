@@ -30,6 +30,17 @@ from dataclasses import replace  # noqa: E402
 from ses3d_amd import api, synth  # noqa: E402
 
 HBM_PEAK = 8.0e12          # MI355X HBM3E, bytes/s
+FP64_PEAK = 78.6e12        # MI355X vector FP64, flop/s
+FP32_PEAK = 157.3e12       # MI355X vector FP32, flop/s
+# Arithmetic per element-local node of the visco-elastic element kernel at
+# order 4 with compensated stress (counted from the kernel source; an FMA
+# counts 2): FP64 90 for the strain-rate contraction (45 FMA), 24 for the
+# spherical completion and symmetrisation, 90 for the divergence contraction
+# and 37 for its coefficients and the force; FP32 44 for the stress rate, 24
+# for its compensated integration, 6 for the effective stress, 73 for the
+# memory variables and 24 for the tapered stores.
+ELEM_FP64_FLOP = 241
+ELEM_FP32_FLOP = 171
 CPU_SAMPLE = dict(nx=24, ny=24, nz=12)   # oracle sample of the same workload
 MISFIT_STEPS = 1200        # FP32 vs FP64 misfit leg after the timing
 
@@ -604,6 +615,17 @@ def main():
                           "elem_ms": elem_ms, "node_ms": ms[1] / max(cnt[1], 1),
                           "record_ms": ms[2] / max(cnt[2], 1)},
     }
+    if mode != "fp64":
+        # the element kernel's arithmetic against the vector peaks: far from
+        # both, the kernel is bound by HBM
+        f64 = ELEM_FP64_FLOP * N / (elem_ms * 1e-3)
+        f32 = ELEM_FP32_FLOP * N / (elem_ms * 1e-3)
+        out["flop_rate"] = {"kernel": "element kernel", "unit": "TFLOP/s",
+                            "fp64_flop_per_node": ELEM_FP64_FLOP, "fp32_flop_per_node": ELEM_FP32_FLOP,
+                            "fp64_achieved": f64 / 1e12, "fp64_peak": FP64_PEAK / 1e12,
+                            "fp64_frac": f64 / FP64_PEAK,
+                            "fp32_achieved": f32 / 1e12, "fp32_peak": FP32_PEAK / 1e12,
+                            "fp32_frac": f32 / FP32_PEAK}
     if "fp64" in modes and mode != "fp64":
         t32 = traces_at(sims[mode], misfit_steps) if misfit_steps > 0 else None
         for sim in sims.values():
