@@ -85,36 +85,40 @@ def mode_sizes(mode):
             2 if m["bf16_static"] else s)
 
 
-def step_bytes(mode, u, visco, L=3, S=1):
+def step_bytes(mode, u, visco, L=3, S=1, nm=6):
     """Algorithmic HBM bytes per element-local node per step:
-    s_state*[12 + 12 L visco + 6 u] + s_lo_v*6 u + s_lo_sigma*12
-    + s_static*[(2 + visco) + u]/S"""
+    s_state*[12 + 2 nm L visco + 6 u] + s_lo_v*6 u + s_lo_sigma*12
+    + s_static*[(2 + visco) + u]/S, with nm memory words per mechanism (6,
+    or 5 for the deviatoric form)"""
     s, lv, ls, st = mode_sizes(mode)
-    return s * (12 + 12 * L * visco + 6 * u) + lv * 6 * u + ls * 12 + st * ((2 + visco) + u) / S
+    return s * (12 + 2 * nm * L * visco + 6 * u) + lv * 6 * u + ls * 12 + st * ((2 + visco) + u) / S
 
 
-def elem_kernel_bytes(mode, u, visco):
+def elem_kernel_bytes(mode, u, visco, nm=6):
     """The element kernel's share of the step's algorithmic bytes per
     element-local node: stress r+w (12 words, +12 compensation words),
-    memory variables r+w (36 words), lambda, mu (+tau) reads and the
+    memory variables r+w (2 x 3 nm words: 36, or 30 in the five-word
+    deviatoric form), lambda, mu (+tau) reads and the
     velocity read once per unique node (3u words); the rest of step_bytes
     (velocity write, its compensation r+w, dt/M) is the node kernel's. The
     element force that the two kernels exchange (3 words written here, read
     by the node kernel) is not among the algorithmic bytes."""
     s, _, ls, st = mode_sizes(mode)
-    return s * (12 + 36 * visco + 3 * u) + ls * 12 + st * (2 + visco)
+    return s * (12 + 6 * nm * visco + 3 * u) + ls * 12 + st * (2 + visco)
 
 
-def load_sims(case, modes, slab, remap=True, tune=None, state_buffers=0):
+def load_sims(case, modes, slab, remap=True, tune=None, state_buffers=0, memory_vars=0):
     """Build one Simulation per precision mode from a single pass over the
     model slabs; returns a list of (mode, sim). tune = (tile, LDS layout)
     sets the FP32 simulation's element tile and layout instead of the
-    autotuner. The first mode gets `state_buffers`; further modes update
-    their state in place (one state buffer), so that an FP32 and an FP64
-    simulation fit on one GPU together at 192x192x96."""
+    autotuner. The first mode gets `state_buffers` and `memory_vars`;
+    further modes update their state in place (one state buffer), so that
+    an FP32 and an FP64 simulation fit on one GPU together at 192x192x96,
+    and keep six memory words per mechanism."""
     g = case.grid
     sims = [(name, api.Simulation(g, xcd_remap=remap,
-                                  state_buffers=state_buffers if q == 0 else 1, **MODES[name]))
+                                  state_buffers=state_buffers if q == 0 else 1,
+                                  memory_vars=memory_vars if q == 0 else 0, **MODES[name]))
             for q, name in enumerate(modes)]
     t0 = time.time()
     for ez0 in range(0, g.nz, slab):
@@ -347,7 +351,8 @@ def batch_bench(args):
     cases = synth.uneven_batch(seed=4, n_sims=args.batch_sims, nt=args.warmup + 3 * args.steps + 1)
     N = sum(c.grid.nodes for c in cases)
     t0 = time.time()
-    sims = [api.build_case(c, precision="fp32", autotune=True, state_buffers=args.state_buffers)[0]
+    sims = [api.build_case(c, precision="fp32", autotune=True, state_buffers=args.state_buffers,
+                           memory_vars=args.memory_vars)[0]
             for c in cases]
     log("[batch] %d simulations, %d GLL nodes, orders %s, setup %.0f s"
         % (len(sims), N, sorted(set(c.grid.lpd for c in cases)), time.time() - t0))
@@ -383,7 +388,8 @@ def batch_bench(args):
            "round_robin": {"value": N * args.steps / wall_r, "ms_per_step": 1e3 * wall_r / args.steps},
            "sequential": {"value": N * args.steps / wall_s, "ms_per_step": 1e3 * wall_s / args.steps},
            "speedup_batched_vs_round_robin": wall_r / wall_b,
-           "state_buffers": 1 if args.state_buffers == 1 else 2}
+           "state_buffers": 1 if args.state_buffers == 1 else 2,
+           "memory_vars": args.memory_vars}
     log("[batch] batched %.3f ms/step, round robin %.3f ms/step, sequential %.3f ms/step"
         % (1e3 * wall_b / args.steps, 1e3 * wall_r / args.steps, 1e3 * wall_s / args.steps))
     print(json.dumps(out), flush=True)
@@ -406,6 +412,7 @@ def multi_source_bench(args):
     # 16 sources in batches of 8), and one block fits 16 sources per batch
     nbuf = args.state_buffers or 1
     kw["state_buffers"] = nbuf
+    kw["memory_vars"] = args.memory_vars
     case = synth.multi_source(seed=5, n_src=args.sources, nt=args.warmup + 2 * args.steps + 1)
     g = case.grid
     N = g.nodes
@@ -451,14 +458,14 @@ def multi_source_bench(args):
         for s_ in sims:
             s_.close()
     owner.close()
-    B = step_bytes(mode, u, True, S=per)
+    B = step_bytes(mode, u, True, S=per, nm=args.memory_vars)
     value = N * S * args.steps / wall_b
     out = {"metric": "GLL-node updates/s (node x source)",
            "workload": "%d moment-tensor sources on one layered visco-elastic model, %dx%dx%d "
                        "elements, GLL order %d, %s, batches of %d" % (S, g.nx, g.ny, g.nz, g.lpd, mode,
                                                                      per),
            "gll_nodes_per_source": N, "sources": S, "per_batch": per, "steps": args.steps,
-           "state_buffers": nbuf,
+           "state_buffers": nbuf, "memory_vars": args.memory_vars,
            "batched_shared": {"value": value, "ms_per_step": 1e3 * wall_b / args.steps},
            "round_robin": {"value": N * S * args.steps / wall_r,
                            "ms_per_step": 1e3 * wall_r / args.steps},
@@ -504,6 +511,10 @@ def main():
     ap.add_argument("--state-buffers", type=int, default=0, choices=(0, 1, 2),
                     help="state blocks per simulation: 0 default (2; 1 for --multi-source), "
                          "1 updated in place, 2 read one and write the other")
+    ap.add_argument("--memory-vars", type=int, default=5, choices=(5, 6),
+                    help="memory words per relaxation mechanism of the FP32 timed mode: 5, "
+                         "the deviatoric part (default), or 6, the reference's form; the "
+                         "FP64 leg keeps 6")
     ap.add_argument("--precision-study", action="store_true",
                     help="FP32 variants vs FP64 seismogram / snapshot misfit instead of timing")
     ap.add_argument("--study-steps", type=int, default=4000)
@@ -556,8 +567,9 @@ def main():
     tune = None if args.tile is None else (args.tile, args.lds_layout)
     # the FP64 path runs after the timed mode, each alone on the GPU with its
     # default state blocks (both at once would not fit at 192x192x96)
+    nm = args.memory_vars if mode != "fp64" else 6
     sims = dict(load_sims(case, modes[:1], args.slab, remap=not args.no_remap, tune=tune,
-                          state_buffers=args.state_buffers))
+                          state_buffers=args.state_buffers, memory_vars=nm))
     tile = sims[mode].tile
     wall, ms, cnt, vr = time_gpu(sims[mode], args.steps, args.warmup)
     value = N * args.steps / wall
@@ -565,9 +577,9 @@ def main():
         % (mode, 1e3 * wall / args.steps, ms[0] / cnt[0], ms[1] / cnt[1], vr))
 
     elem_ms = ms[0] / max(cnt[0], 1)
-    eb = elem_kernel_bytes(mode, u, 1) * N
+    eb = elem_kernel_bytes(mode, u, 1, nm) * N
     achieved = eb / (elem_ms * 1e-3)
-    Bstep = step_bytes(mode, u, 1)
+    Bstep = step_bytes(mode, u, 1, nm=nm)
     rtype = "double" if mode == "fp64" else "float"
     s_state = mode_sizes(mode)[0]
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
@@ -603,12 +615,16 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if mode == "fp64" else "f32",
         "data": "synthetic (seeded layered visco-elastic model, GRF vs perturbation)",
         "config": {"workload": "layered visco-elastic section 40x40 deg x 1400 km, "
-                               "%dx%dx%d elements, GLL order 4, 3 SLS mechanisms, %s, "
+                               "%dx%dx%d elements, GLL order 4, 3 SLS mechanisms%s, %s, "
                                "one source, 256 surface receivers and 32 on rings around "
-                               "the source" % (g.nx, g.ny, g.nz, MODE_TEXT[mode]),
+                               "the source" % (g.nx, g.ny, g.nz,
+                                               " (memory variables held as their deviatoric "
+                                               "part, 5 words per mechanism)" if nm == 5 else "",
+                                               MODE_TEXT[mode]),
                    "elements": g.n_elem, "gll_nodes": N, "unique_nodes": g.unique_nodes,
                    "precision": mode, "parallelism": "single GPU",
-                   "state_buffers": 1 if args.state_buffers == 1 else 2},
+                   "state_buffers": 1 if args.state_buffers == 1 else 2,
+                   "memory_vars": nm},
         "roofline": roof,
         "step_roofline": {"bytes_per_node": Bstep, "achieved_GBps": Bstep * value / 1e9,
                           "frac": step_frac,

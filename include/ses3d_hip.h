@@ -54,6 +54,16 @@ typedef struct s3d_options {
                               reads one and writes the other, about 10 %
                               faster element kernel for the state's size
                               again in memory; 1: one block updated in place */
+    int memory_vars;       /* visco-elastic memory variables per relaxation
+                              mechanism: 6 (and 0, the default) the six
+                              components xx xy xz yy yz zz as the reference
+                              holds them (src/evolution.f90:95); 5 their
+                              deviatoric part xx xy xz yy yz, zz = -(xx + yy):
+                              only that part enters the stress, so the
+                              stress and velocity follow the same equations
+                              with 24 B per node less state traffic in FP32;
+                              the diagonal S3D_MXX fields are then the
+                              deviatoric parts and zz is read only         */
 } s3d_options;
 
 /* Model parameter ids (reference src/model_paras.F90:39-46). */

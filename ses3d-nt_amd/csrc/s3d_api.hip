@@ -226,6 +226,7 @@ struct s3d_ctx {
     int epb = 0;               // element tile (0 = default), see s3d_autotune
     int lds_layout = 0;        // LDS staging layout of the FP32 element kernel, see Stage
     bool early_stores = false; // FP32 element kernel stores the state before the divergence
+    int nm = 6;                // memory words per relaxation mechanism (6, or 5: deviatoric)
     int curl_it = -1;          // steps completed at the last s3d_capture_curl_strain
     bool fields_set = false;   // s3d_set_field was called (state not at rest)
     // s3d_create_shared: the simulation whose static fields (moduli, tau, rho,
@@ -275,7 +276,7 @@ struct StateParts {
 StateParts state_parts(const s3d_ctx *c, int k)
 {
     const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
-    const int ncl = c->ns - 6 - (c->visco ? 18 : 0);
+    const int ncl = c->ns - 6 - (c->visco ? NRDISS * c->nm : 0);
     char *b = (char *)c->blk[k];
     return {b, ncl ? b + rs * 6 * c->NP : nullptr, c->visco ? b + rs * (6 + ncl) * c->NP : nullptr};
 }
@@ -306,7 +307,7 @@ int alloc_state(s3d_ctx *c)
     // block of ns words (FP32: 4-byte words, the BF16 compensation in pairs;
     // FP64 has no compensation); sig_lo and mem point at their first
     // components and are not separate allocations
-    c->ns = state_components(c->o.kahan_sigma, c->visco);
+    c->ns = state_components(c->o.kahan_sigma, c->visco ? c->nm : 0);
     for (int k = 0; k < c->nbuf; ++k)
         if (dalloc(c, &c->blk[k], rs * c->ns * c->nn)) return 1;
     set_state_block(c, 0);
@@ -470,23 +471,23 @@ void ev_end(s3d_ctx *c, int k) { ev_begin(c, k); }
 // 2 and 4 elements per workgroup (where that fits 1024 lanes) besides the
 // default; FP64 uses the default tile only.
 // the stress-compensation mode is a kernel template parameter (FP32 only)
-template <int P, int E, typename R, bool VISCO, bool S16, int LL>
+template <int P, int E, typename R, int VM, bool S16, int LL>
 void launch_elem_ll(int ks, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
 {
     using S = Shape<P, E>;
     if (ks == 1)
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 1, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VM, S16, 1, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
     else if (ks == 2)
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 2, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VM, S16, 2, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
     else
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VM, S16, 0, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
 }
 
 // stress-compensation mode and kernel variant (elem_body's LL: planar,
 // node-interleaved, planar with the state stored before the divergence) are
 // kernel template parameters (FP32 only; FP64 and the BF16-static kernels
 // use variant 0)
-template <int P, int E, typename R, bool VISCO, bool S16>
+template <int P, int E, typename R, int VM, bool S16>
 int launch_elem_ks(int ks, int ll, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
 {
     using S = Shape<P, E>;
@@ -494,16 +495,16 @@ int launch_elem_ks(int ks, int ll, const ElemArgs<R> &ea, hipStream_t st, unsign
         bool done = false;
         if constexpr (!S16) {
             if (ll == 1) {
-                launch_elem_ll<P, E, R, VISCO, S16, 1>(ks, ea, st, ge);
+                launch_elem_ll<P, E, R, VM, S16, 1>(ks, ea, st, ge);
                 done = true;
             } else if (ll == 2) {
-                launch_elem_ll<P, E, R, VISCO, S16, 2>(ks, ea, st, ge);
+                launch_elem_ll<P, E, R, VM, S16, 2>(ks, ea, st, ge);
                 done = true;
             }
         }
-        if (!done) launch_elem_ll<P, E, R, VISCO, S16, 0>(ks, ea, st, ge);
+        if (!done) launch_elem_ll<P, E, R, VM, S16, 0>(ks, ea, st, ge);
     } else {
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VISCO, S16, 0, 0>), dim3(ge), dim3(S::NT), 0, st, ea);
+        hipLaunchKernelGGL((elem_kernel<P, E, R, VM, S16, 0, 0>), dim3(ge), dim3(S::NT), 0, st, ea);
     }
     HIPCHK(hipGetLastError());
     return 0;
@@ -515,13 +516,16 @@ int launch_elem_e(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st)
     using S = Shape<P, E>;
     const unsigned ge = nblocks(c->nel, S::EPB);
     const int ks = c->o.kahan_sigma, ll = kernel_variant(c);
+    const int vm = c->visco ? c->nm : 0;
     if constexpr (std::is_same<R, float>::value && E == 0) {
         if (c->o.bf16_static)   // default tile only
-            return c->visco ? launch_elem_ks<P, 0, float, true, true>(ks, 0, ea, st, ge)
-                            : launch_elem_ks<P, 0, float, false, true>(ks, 0, ea, st, ge);
+            return vm == 6 ? launch_elem_ks<P, 0, float, 6, true>(ks, 0, ea, st, ge)
+                 : vm == 5 ? launch_elem_ks<P, 0, float, 5, true>(ks, 0, ea, st, ge)
+                           : launch_elem_ks<P, 0, float, 0, true>(ks, 0, ea, st, ge);
     }
-    return c->visco ? launch_elem_ks<P, E, R, true, false>(ks, ll, ea, st, ge)
-                    : launch_elem_ks<P, E, R, false, false>(ks, ll, ea, st, ge);
+    return vm == 6 ? launch_elem_ks<P, E, R, 6, false>(ks, ll, ea, st, ge)
+         : vm == 5 ? launch_elem_ks<P, E, R, 5, false>(ks, ll, ea, st, ge)
+                   : launch_elem_ks<P, E, R, 0, false>(ks, ll, ea, st, ge);
 }
 
 template <int P>
@@ -1000,7 +1004,8 @@ int upload_receivers(s3d_ctx *c)
 
 // ---------------------------------------------------------------------------
 // Batches: simulations of one device advanced together. Simulations with the
-// same (order, precision, visco, compensation) form a group; each group costs
+// same (order, precision, memory-variable form, compensation) form a group;
+// each group costs
 // one element, one node and one receiver launch per step, with per-simulation
 // arguments in device arrays and workgroups partitioned by prefix sums.
 // ---------------------------------------------------------------------------
@@ -1009,6 +1014,7 @@ struct BatchGroup {
     int epb = 0;
     int ll = 0;                 // element kernel variant (elem_body's LL)
     int ks = 0;                 // stress compensation mode
+    int vm = 0;                 // memory words per mechanism (0 elastic, 6, 5)
     bool fp32 = false, visco = false, kahan = false;
     std::vector<s3d_ctx *> sims;
     // element and receiver arguments in two sets of sims.size() entries: set
@@ -1111,7 +1117,7 @@ int batch_build_group_e(BatchGroup &G)
     return 0;
 }
 
-template <int P, int E, typename R, bool VISCO, int KS>
+template <int P, int E, typename R, int VM, int KS>
 int batch_launch_group_k(const BatchGroup &G, hipStream_t st)
 {
     using S = Shape<P, E>;
@@ -1119,35 +1125,37 @@ int batch_launch_group_k(const BatchGroup &G, hipStream_t st)
         if (G.ll == 1 || G.ll == 2) {
             const ElemArgs<R> *args = (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size();
             if (G.ll == 1)
-                hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 1>), dim3(G.ewg), dim3(S::NT), 0, st,
+                hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VM, KS, 1>), dim3(G.ewg), dim3(S::NT), 0, st,
                                    args, (const int2 *)G.d_eown);
             else
-                hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 2>), dim3(G.ewg), dim3(S::NT), 0, st,
+                hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VM, KS, 2>), dim3(G.ewg), dim3(S::NT), 0, st,
                                    args, (const int2 *)G.d_eown);
             HIPCHK(hipGetLastError());
             return 0;
         }
     }
-    hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VISCO, KS, 0>), dim3(G.ewg), dim3(S::NT), 0, st,
+    hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VM, KS, 0>), dim3(G.ewg), dim3(S::NT), 0, st,
                        (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size(), (const int2 *)G.d_eown);
     HIPCHK(hipGetLastError());
     return 0;
 }
 
-template <int P, int E, typename R, bool VISCO>
+template <int P, int E, typename R, int VM>
 int batch_launch_group_v(const BatchGroup &G, hipStream_t st)
 {
     if constexpr (std::is_same<R, float>::value) {
-        if (G.ks == 1) return batch_launch_group_k<P, E, R, VISCO, 1>(G, st);
-        if (G.ks == 2) return batch_launch_group_k<P, E, R, VISCO, 2>(G, st);
+        if (G.ks == 1) return batch_launch_group_k<P, E, R, VM, 1>(G, st);
+        if (G.ks == 2) return batch_launch_group_k<P, E, R, VM, 2>(G, st);
     }
-    return batch_launch_group_k<P, E, R, VISCO, 0>(G, st);
+    return batch_launch_group_k<P, E, R, VM, 0>(G, st);
 }
 
 template <int P, int E, typename R>
 int batch_launch_group_e(const BatchGroup &G, hipStream_t st)
 {
-    return G.visco ? batch_launch_group_v<P, E, R, true>(G, st) : batch_launch_group_v<P, E, R, false>(G, st);
+    return G.vm == 6 ? batch_launch_group_v<P, E, R, 6>(G, st)
+         : G.vm == 5 ? batch_launch_group_v<P, E, R, 5>(G, st)
+                     : batch_launch_group_v<P, E, R, 0>(G, st);
 }
 
 template <int P, typename R>
@@ -1265,13 +1273,15 @@ int batch_build(s3d_batch *b)
         if (upload_sources(c) || upload_receivers(c)) return 1;
         drop_graph(c);
     }
-    // one launch per kind of step (order, precision, visco, compensation);
+    // one launch per kind of step (order, precision, memory variables,
+    // compensation);
     // the element tile and LDS layout give bit-identical results, so a
     // group takes those of its largest member rather than splitting by them
     for (s3d_ctx *c : b->sims) {
         bool placed = false;
         for (auto &G : b->groups)
             if (G.p == c->p && G.fp32 == c->fp32 && G.visco == c->visco &&
+                G.vm == (c->visco ? c->nm : 0) &&
                 G.kahan == (c->lo[0] != nullptr) && G.ks == c->o.kahan_sigma) {
                 G.sims.push_back(c);
                 placed = true;
@@ -1282,6 +1292,7 @@ int batch_build(s3d_batch *b)
             G.p = c->p;
             G.fp32 = c->fp32;
             G.visco = c->visco;
+            G.vm = c->visco ? c->nm : 0;
             G.kahan = c->lo[0] != nullptr;
             G.ks = c->o.kahan_sigma;
             G.sims.push_back(c);
@@ -1379,6 +1390,11 @@ int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out)
         return fail("state_buffers must be 0 (default), 1 or 2");
     }
     c->nbuf = c->o.state_buffers == 1 ? 1 : 2;
+    if (c->o.memory_vars != 0 && c->o.memory_vars != 5 && c->o.memory_vars != 6) {
+        delete c;
+        return fail("memory_vars must be 0 (default, 6), 6 or 5");
+    }
+    c->nm = c->o.memory_vars == 5 ? 5 : 6;
     c->dev = c->o.device;
     if (c->dev < 0 || c->dev >= ndev) { delete c; return fail("bad device ordinal"); }
     hipError_t e = hipSetDevice(c->dev);
@@ -1459,6 +1475,7 @@ int s3d_create_shared(s3d_ctx *base, s3d_ctx **out)
     c->g = m->g;
     c->o = m->o;
     c->nbuf = m->nbuf;
+    c->nm = m->nm;
     c->p = m->p; c->P1 = m->P1; c->NP = m->NP; c->PB = m->PB;
     c->nel = m->nel; c->nn = m->nn; c->nblk = m->nblk; c->nuniq = m->nuniq;
     c->fp32 = m->fp32; c->visco = m->visco; c->aniso = m->aniso;
@@ -1964,6 +1981,14 @@ int s3d_get_trace(s3d_ctx *c, int rec, int ch, double *out)
     return 0;
 }
 
+// Component of the memory-variable part of the state block that holds field
+// S3D_MXX + 6 * mechanism + comp (comp < 5 in the five-word form).
+int mem_word(const s3d_ctx *c, int field)
+{
+    const int f = field - S3D_MXX;
+    return c->nm == 5 ? 5 * (f / 6) + f % 6 : f;
+}
+
 int s3d_get_field(s3d_ctx *c, int field, double *out)
 {
     if (!c || !out) return fail("null argument");
@@ -2022,6 +2047,16 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
         }
         hipLaunchKernelGGL(from_device_layout<double>, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
                            (const double *)c->dgrad + (long)c->NP * (field - S3D_EXX), buf, c->nel, c->NP, 6);
+    } else if (field >= S3D_MXX && field < S3D_MXX + 18 && c->mem && c->nm == 5 &&
+               (field - S3D_MXX) % 6 == ZZ) {
+        // five-word form: zz is not stored, it is -(xx + yy)
+        (void)hipFree(buf);
+        std::vector<double> xx(nn), yy(nn);
+        const int m0 = S3D_MXX + 6 * ((field - S3D_MXX) / 6);
+        if (int rc = s3d_get_field(c, m0 + XX, xx.data())) return rc;
+        if (int rc = s3d_get_field(c, m0 + YY, yy.data())) return rc;
+        for (long q = 0; q < nn; ++q) out[q] = -(xx[q] + yy[q]);
+        return 0;
     } else if ((field >= S3D_SXX && field <= S3D_SZZ) || (field >= S3D_MXX && field < S3D_MXX + 18)) {
         const void *src = nullptr;
         const void *lo = nullptr;
@@ -2031,7 +2066,7 @@ int s3d_get_field(s3d_ctx *c, int field, double *out)
             if (c->sig_lo && c->o.kahan_sigma == 1) lo = (const char *)c->sig_lo + rsize(c) * c->NP * (field - S3D_SXX);
             if (c->sig_lo && c->o.kahan_sigma == 2) lo = (const char *)c->sig_lo + 4 * c->NP * ((field - S3D_SXX) / 2);
         } else if (c->mem) {
-            src = (const char *)c->mem + rsize(c) * c->NP * (field - S3D_MXX);
+            src = (const char *)c->mem + rsize(c) * c->NP * mem_word(c, field);
         }
         if (!src) {
             HIPCHK(hipMemsetAsync(buf, 0, sizeof(double) * nn, c->stream));
@@ -2098,8 +2133,11 @@ int s3d_set_field(s3d_ctx *c, int field, const double *in)
                 hipLaunchKernelGGL(lo_pair_clear, dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
                                    (unsigned *)c->sig_lo + c->NP * ((field - S3D_SXX) / 2), (field - S3D_SXX) % 2,
                                    c->nel, c->NP, nc);
+        } else if (c->mem && c->nm == 5 && (field - S3D_MXX) % 6 == ZZ) {
+            rc = fail("with five memory words per mechanism the zz memory variables are -(xx + yy) "
+                      "and cannot be set");
         } else if (c->mem) {
-            dst = (char *)c->mem + rsize(c) * c->NP * (field - S3D_MXX);
+            dst = (char *)c->mem + rsize(c) * c->NP * mem_word(c, field);
         } else {
             rc = fail("memory variables exist only for a visco-elastic model");
         }

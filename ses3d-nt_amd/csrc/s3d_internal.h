@@ -41,11 +41,13 @@ constexpr int MAXP1 = 8;       // p <= 7
 constexpr int NRDISS = 3;      // relaxation mechanisms
 
 // Components of the element-major state block: stress (6), its compensation
-// (6 FP32 words for kahan_sigma 1, 3 packed BF16 pairs for 2) and the 18
-// memory variables of a visco-elastic model, in that order.
-constexpr int state_components(int kahan_sigma, bool visco)
+// (6 FP32 words for kahan_sigma 1, 3 packed BF16 pairs for 2) and the memory
+// variables of a visco-elastic model, in that order. vm: memory words per
+// relaxation mechanism, 0 (elastic), 6 (xx xy xz yy yz zz, the reference's
+// form) or 5 (the deviatoric part, xx xy xz yy yz; zz = -(xx + yy)).
+constexpr int state_components(int kahan_sigma, int vm)
 {
-    return 6 + (kahan_sigma == 1 ? 6 : kahan_sigma == 2 ? 3 : 0) + (visco ? 18 : 0);
+    return 6 + (kahan_sigma == 1 ? 6 : kahan_sigma == 2 ? 3 : 0) + NRDISS * vm;
 }
 
 // strain-rate / stress component order used everywhere: xx xy xz yy yz zz

@@ -18,7 +18,8 @@
 //   record_kernel one wave per receiver: sum vm * field over the element.
 //
 // HBM bytes per element-local node and step (FP32, Kahan on v, visco):
-//   stress 24 r + 24 w, memory variables 72 r + 72 w, lambda/mu/tau 12 r,
+//   stress 24 r + 24 w, memory variables 72 r + 72 w (60 + 60 in the
+//   five-word deviatoric form), lambda/mu/tau 12 r,
 //   force 12 w + 12 r, velocity ~(3 r gather) + unique-node v, lo, 1/M
 //   (~24*u r + 24*u w with u = unique/element-local ~ 0.52).
 #include "s3d_internal.h"
@@ -163,13 +164,13 @@ struct Stage {
 };
 
 // Per-lane state of one element tile (EPB elements, one lane per
-// element-local node).
-template <typename R, bool VISCO>
+// element-local node). VM: memory words per mechanism (state_components).
+template <typename R, int VM>
 struct TileState {
     R v[3];             // velocity gathered from the unique nodes
     R sg[6];            // stress
     R lam, mu, tau;     // dt-scaled moduli, tau
-    R M[VISCO ? 18 : 1];
+    R M[VM ? NRDISS * VM : 1];
     R lo[6];            // stress compensation, kahan_sigma 1
     unsigned lo2[3];    // stress compensation, kahan_sigma 2 (packed BF16 pairs)
 };
@@ -177,7 +178,7 @@ struct TileState {
 // Element-major block fields (the state block, element force, lam/mu/tau):
 // component c of element e occupies the NP entries at (e * nc + c) * NP.
 // The state block's parts are addressed through pointers at their first
-// components (sig, sig_lo, mem) with nc = state_components(KS, VISCO).
+// components (sig, sig_lo, mem) with nc = state_components(KS, VM).
 __device__ __forceinline__ long em_index(long e, int nc, int c, int NP, int node)
 {
     return (e * nc + c) * NP + node;
@@ -189,12 +190,13 @@ __device__ __forceinline__ long em_index(long e, int nc, int c, int NP, int node
 // compensation 0 off, 1 FP32 words, 2 BF16 words in pairs (a template
 // parameter: a run-time choice leaves register joins that wait for every
 // load in flight).
-template <int P, int E, typename R, bool VISCO, bool S16, bool LOADM, int KS>
-__device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileState<R, VISCO> &s)
+template <int P, int E, typename R, int VM, bool S16, bool LOADM, int KS>
+__device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileState<R, VM> &s)
 {
     using S = Shape<P, E>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB;
-    constexpr int NS = state_components(KS, VISCO);
+    constexpr int NS = state_components(KS, VM);
+    constexpr bool VISCO = VM != 0;
     const int t = threadIdx.x;
     const int slot = t / NP;
     const int node = t - slot * NP;
@@ -223,7 +225,7 @@ __device__ __forceinline__ void load_tile(const ElemArgs<R> &a, int tile, TileSt
     }
     if constexpr (VISCO && LOADM) {
 #pragma unroll
-        for (int c = 0; c < 18; ++c) s.M[c] = a.mem[em_index(ee, NS, c, NP, node)];
+        for (int c = 0; c < NRDISS * VM; ++c) s.M[c] = a.mem[em_index(ee, NS, c, NP, node)];
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) s.lo[c] = R(0);
@@ -249,14 +251,18 @@ enum { G_SIN = 0, G_ISIN, G_COT, G_R, G_IR, G_R2, G_PX, G_PY, G_PZ, G_N };
 // Stage), 2 planar staging with the new stress, compensation and memory
 // variables stored before the weak divergence instead of after it (their
 // registers are then free during the divergence, which raises the occupancy
-// of some orders and compensation modes); the arithmetic is the same.
-template <int P, int E, typename R, bool VISCO, bool S16, int KS, int LL>
+// of some orders and compensation modes); the arithmetic is the same. VM:
+// memory words per mechanism, 0 elastic, 6 the six components, 5 their
+// deviatoric part (see the stress rate below).
+template <int P, int E, typename R, int VM, bool S16, int KS, int LL>
 __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 {
     using S = Shape<P, E>;
     using C = coef_t<R>;
     constexpr int P1 = S::P1, NP = S::NP, EPB = S::EPB, NE = EPB;
-    constexpr int NS = state_components(KS, VISCO);
+    constexpr int NS = state_components(KS, VM);
+    constexpr bool VISCO = VM != 0;
+    constexpr int NM = NRDISS * VM;   // memory words per node
     constexpr bool EARLY = LL == 2;
     __shared__ Stage<R, LL == 1 ? 1 : 0, EPB * NP> lds;
     __shared__ C cf[4][P1 * P1];     // Dx, Dy, Dz, dl*w
@@ -276,7 +282,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     // (lane t's words at mbuf[c][t]), so they hold no registers until the
     // stress rate; issued before every other load ----
     constexpr bool MDMA = VISCO && sizeof(R) == 4;
-    __shared__ R mbuf[MDMA ? 18 : 1][MDMA ? S::NT : 1];
+    __shared__ R mbuf[MDMA ? NM : 1][MDMA ? S::NT : 1];
     if constexpr (MDMA) {
         // operands in registers first: each asm statement clobbers memory,
         // and argument blocks read through a pointer would be reloaded
@@ -284,7 +290,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         const R *mem = a.mem + em_index((inside && e < a.nel) ? e : 0, NS, 0, NP, node);
         const unsigned base = (unsigned)(size_t)&mbuf[0][(t >> 6) << 6];
 #pragma unroll
-        for (int c = 0; c < 18; ++c)
+        for (int c = 0; c < NM; ++c)
             lds_dma_dword(mem + c * NP, base + (unsigned)(c * S::NT * sizeof(R)));
     }
 
@@ -310,8 +316,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     if (a.src_slot) sslot = a.src_slot[se < a.nel ? se : a.nel - 1];
 
     // ---- state of the tile ----
-    TileState<R, VISCO> cur;
-    load_tile<P, E, R, VISCO, S16, !MDMA, KS>(a, tile0, cur);
+    TileState<R, VM> cur;
+    load_tile<P, E, R, VM, S16, !MDMA, KS>(a, tile0, cur);
 
     if (t < P1 * P1) {
         cf[0][t] = tD0;
@@ -431,23 +437,43 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
                 // this lane's words were written by its own wave's loads
                 const int z = wait_vmem_zero();
 #pragma unroll
-                for (int c = 0; c < 18; ++c) cur.M[c] = mbuf[c][t + z];
+                for (int c = 0; c < NM; ++c) cur.M[c] = mbuf[c][t + z];
             }
-            R Md[6];
-#pragma unroll
-            for (int c = 0; c < 6; ++c) Md[c] = cur.M[c] + cur.M[6 + c] + cur.M[12 + c];
-            const R Msum = Md[XX] + Md[YY] + Md[ZZ];
             // lam holds kappa - 2 mu_tau / 3 and mu holds mu_tau = mu (1 + tau),
             // both formed in FP64; 2 mu itself only scales the memory variables
             const R tmut = tmu;                                    // 2 mu_tau
             const R tm = tmut / (R(1) + cur.tau);                  // 2 mu
-            const R Lr = cur.lam * (exx + eyy + ezz) - tm * Msum * R(1.0 / 3.0);
-            rt[XX] = Lr + tmut * exx + tm * Md[XX];
-            rt[YY] = Lr + tmut * eyy + tm * Md[YY];
-            rt[ZZ] = Lr + tmut * ezz + tm * Md[ZZ];
-            rt[XY] = tmut * exy + tm * Md[XY];
-            rt[XZ] = tmut * exz + tm * Md[XZ];
-            rt[YZ] = tmut * eyz + tm * Md[YZ];
+            if constexpr (VM == 6) {
+                R Md[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) Md[c] = cur.M[c] + cur.M[6 + c] + cur.M[12 + c];
+                const R Msum = Md[XX] + Md[YY] + Md[ZZ];
+                const R Lr = cur.lam * (exx + eyy + ezz) - tm * Msum * R(1.0 / 3.0);
+                rt[XX] = Lr + tmut * exx + tm * Md[XX];
+                rt[YY] = Lr + tmut * eyy + tm * Md[YY];
+                rt[ZZ] = Lr + tmut * ezz + tm * Md[ZZ];
+                rt[XY] = tmut * exy + tm * Md[XY];
+                rt[XZ] = tmut * exz + tm * Md[XZ];
+                rt[YZ] = tmut * eyz + tm * Md[YZ];
+            } else {
+                // The memory variables enter the stress rate only as
+                // Md - tr(Md)/3 I, and every component relaxes at the same
+                // rate with the same forcing (t3 e), so their deviatoric
+                // part evolves on its own, driven by the deviatoric strain
+                // rate: five words per mechanism hold it (xx xy xz yy yz,
+                // zz = -(xx + yy)), and the trace is never formed.
+                R Dd[5];
+#pragma unroll
+                for (int c = 0; c < 5; ++c) Dd[c] = cur.M[c] + cur.M[5 + c] + cur.M[10 + c];
+                const R Dzz = -(Dd[XX] + Dd[YY]);
+                const R Lr = cur.lam * (exx + eyy + ezz);
+                rt[XX] = Lr + tmut * exx + tm * Dd[XX];
+                rt[YY] = Lr + tmut * eyy + tm * Dd[YY];
+                rt[ZZ] = Lr + tmut * ezz + tm * Dzz;
+                rt[XY] = tmut * exy + tm * Dd[XY];
+                rt[XZ] = tmut * exz + tm * Dd[XZ];
+                rt[YZ] = tmut * eyz + tm * Dd[YZ];
+            }
         }
         if (S16 ? a.s16.ca != nullptr : a.ca != nullptr) {   // anisotropy (src/evolution.f90:290-296)
             R pa, pb, pc;
@@ -512,18 +538,23 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         }
 
         // ---- memory variables (loop form of src/evolution.f90:415-442) ----
-        R M[VISCO ? 18 : 1];
+        R M[VISCO ? NM : 1];
         if constexpr (VISCO) {
-            const R ev[6] = {exx, exy, exz, eyy, eyz, ezz};
             // M (1 - dt/tau_p) - dt tau / (3 tau_p) e  ==  M - dt/tau_p (M + tau/3 e):
             // the decrement form keeps the FP32 relaxation rate exact to one
             // rounding of dt/tau_p (1 - dt/tau_p would lose its low bits)
             const R t3 = cur.tau * R(1.0 / 3.0);
+            R ev[6] = {exx, exy, exz, eyy, eyz, ezz};
+            if constexpr (VM == 5) {   // deviatoric strain rate
+                const R tr3 = (exx + eyy + ezz) * R(1.0 / 3.0);
+                ev[XX] = exx - tr3;
+                ev[YY] = eyy - tr3;
+            }
 #pragma unroll
             for (int m = 0; m < NRDISS; ++m) {
 #pragma unroll
-                for (int c = 0; c < 6; ++c)
-                    M[6 * m + c] = cur.M[6 * m + c] - a.rate[m] * (cur.M[6 * m + c] + t3 * ev[c]);
+                for (int c = 0; c < VM; ++c)
+                    M[VM * m + c] = cur.M[VM * m + c] - a.rate[m] * (cur.M[VM * m + c] + t3 * ev[c]);
             }
         }
 
@@ -555,7 +586,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
             }
             if constexpr (VISCO) {
 #pragma unroll
-                for (int c = 0; c < 18; ++c) a.mem_w[em_index(ev, NS, c, NP, node)] = M[c];
+                for (int c = 0; c < NM; ++c) a.mem_w[em_index(ev, NS, c, NP, node)] = M[c];
             }
         };
         if constexpr (EARLY) {
@@ -611,14 +642,14 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
     }
 }
 
-template <int P, int E, typename R, bool VISCO, bool S16, int KS, int LL>
+template <int P, int E, typename R, int VM, bool S16, int KS, int LL>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel(const ElemArgs<R> a)
 {
-    elem_body<P, E, R, VISCO, S16, KS, LL>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
+    elem_body<P, E, R, VM, S16, KS, LL>(a, remap_block(blockIdx.x, gridDim.x, a.remap));
 }
 
-template <int P, int E, typename R, bool VISCO, int KS, int LL>
+template <int P, int E, typename R, int VM, int KS, int LL>
 __global__ void __launch_bounds__((Shape<P, E>::NT))
 elem_kernel_batch(const ElemArgs<R> *__restrict__ args, const int2 *__restrict__ own)
 {
@@ -626,7 +657,7 @@ elem_kernel_batch(const ElemArgs<R> *__restrict__ args, const int2 *__restrict__
     // batch: each XCD works through a contiguous range of simulations
     const int2 o = own[remap_block(blockIdx.x, gridDim.x, 1)];
     if (o.x < 0) return;   // padding workgroup
-    elem_body<P, E, R, VISCO, false, KS, LL>(args[o.x], o.y);
+    elem_body<P, E, R, VM, false, KS, LL>(args[o.x], o.y);
 }
 
 // ---------------------------------------------------------------------------

@@ -184,6 +184,7 @@ CONTAINS
         o%bf16_static = MERGE(1, 0, cf%bf16_static)
         o%device = cf%device
         o%state_buffers = cf%state_buffers
+        o%memory_vars = cf%memory_vars
         CALL check(s3d_create(g, o, ctx), 'create')
         n = INT(cf%nx, 8) * cf%ny * cf%nz * (cf%lpd + 1)**3
         ALLOCATE(buf(n))

@@ -29,6 +29,7 @@ MODULE ses3d_hip_c
         INTEGER(c_int) :: xcd_remap = 1
         INTEGER(c_int) :: bf16_static = 0
         INTEGER(c_int) :: state_buffers = 0
+        INTEGER(c_int) :: memory_vars = 0
     END TYPE
 
     INTEGER(c_int), PARAMETER, PUBLIC :: S3D_FP64 = 0, S3D_FP32 = 1

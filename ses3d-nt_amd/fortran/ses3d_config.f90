@@ -12,7 +12,10 @@
 !! tile of the FP32 step: -1 autotune, 0 default, 1, 2 or 4 elements per
 !! workgroup), state_buffers (2: the step reads the state from one block and
 !! writes the other, the default; 1: one block updated in place, half the
-!! state memory).
+!! state memory), memory_vars (5: the deviatoric part of the memory
+!! variables, which is all that enters the stress, five words per relaxation
+!! mechanism, the default; 6: all six components, as the reference holds
+!! them; no output of this program reads the memory variables).
 MODULE configuration_mod
     USE, INTRINSIC :: ISO_FORTRAN_ENV, ONLY : ERROR_UNIT, IOSTAT_END
     USE, INTRINSIC :: ISO_C_BINDING, ONLY : c_double, c_int
@@ -68,7 +71,7 @@ MODULE configuration_mod
         ! &gpu
         CHARACTER(LEN=8) :: precision
         LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
-        INTEGER :: device, tile, state_buffers
+        INTEGER :: device, tile, state_buffers, memory_vars
         TYPE(source_entry), ALLOCATABLE :: sources(:)
         TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
         TYPE(output_entry), ALLOCATABLE :: outputs(:)
@@ -129,9 +132,9 @@ CONTAINS
         ! &gpu
         CHARACTER(LEN=8) :: precision
         LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
-        INTEGER :: device, tile, state_buffers
+        INTEGER :: device, tile, state_buffers, memory_vars
         NAMELIST /gpu/ precision, kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static, device, tile, &
-                       state_buffers
+                       state_buffers, memory_vars
         INTEGER :: lun, ios, n
         TYPE(configuration_typ) :: cf
 
@@ -217,6 +220,7 @@ CONTAINS
         ! ---- &gpu (optional) ----
         precision = 'fp32'; kahan_v = .TRUE.; kahan_sigma = .TRUE.; device = 0
         kahan_sigma_bf16 = .TRUE.; bf16_static = .FALSE.; tile = -1; state_buffers = 2
+        memory_vars = 5
         n = count_groups(lun, 'gpu')
         IF (n > 1) CALL abort('Found ' // i2c(n) // ' &gpu-groups in config-file!')
         IF (n == 1) THEN
@@ -232,6 +236,8 @@ CONTAINS
         cf%tile = tile
         IF (state_buffers /= 1 .AND. state_buffers /= 2) CALL abort('state_buffers must be 1 or 2')
         cf%state_buffers = state_buffers
+        IF (memory_vars /= 5 .AND. memory_vars /= 6) CALL abort('memory_vars must be 5 or 6')
+        cf%memory_vars = memory_vars
 
         CALL parse_sources(lun, cf)
         CALL parse_receivers(lun, cf)

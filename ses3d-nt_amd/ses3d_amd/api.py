@@ -66,7 +66,7 @@ class S3dOptions(C.Structure):
     _fields_ = [("precision", C.c_int), ("kahan_v", C.c_int),
                 ("kahan_sigma", C.c_int), ("device", C.c_int),
                 ("xcd_remap", C.c_int), ("bf16_static", C.c_int),
-                ("state_buffers", C.c_int)]
+                ("state_buffers", C.c_int), ("memory_vars", C.c_int)]
 
 
 class S3dError(RuntimeError):
@@ -156,19 +156,23 @@ class Simulation:
     """Device state of one simulation (one mesh, one model)."""
 
     def __init__(self, grid, precision="fp32", kahan_v=True, kahan_sigma=False,
-                 device=0, xcd_remap=True, bf16_static=False, state_buffers=0):
+                 device=0, xcd_remap=True, bf16_static=False, state_buffers=0,
+                 memory_vars=0):
         """kahan_sigma: False/0 off, True/1 FP32 compensation words,
         2 or "bf16" BF16 compensation words. bf16_static: store the moduli,
         tau and dt/M in BF16 (FP32 only). state_buffers: 2 (or 0, the
         default) read the state from one block and write it to another, 1
-        update one block in place (half the state memory)."""
+        update one block in place (half the state memory). memory_vars: 6
+        (or 0, the default) memory variables per relaxation mechanism, or 5,
+        their deviatoric part (the zz field is then -(xx + yy), read only)."""
         g = S3dGrid(grid.nx, grid.ny, grid.nz, grid.lpd, grid.lat_min,
                     grid.lat_max, grid.lon_min, grid.lon_max, grid.rad_min,
                     grid.rad_max, grid.taper_width, grid.taper_slope, grid.dt,
                     grid.nt)
         ks = 2 if kahan_sigma == "bf16" else int(kahan_sigma)
         o = S3dOptions(FP32 if precision == "fp32" else FP64, int(kahan_v),
-                       ks, int(device), int(xcd_remap), int(bf16_static), int(state_buffers))
+                       ks, int(device), int(xcd_remap), int(bf16_static), int(state_buffers),
+                       int(memory_vars))
         h = C.c_void_p()
         _chk(lib().s3d_create(C.byref(g), C.byref(o), C.byref(h)))
         self._h = h
@@ -391,13 +395,15 @@ class Batch:
 
 
 def build_case(case, precision="fp32", kahan_v=True, kahan_sigma=False,
-               slab=None, xcd_remap=True, autotune=False, bf16_static=False, state_buffers=0):
+               slab=None, xcd_remap=True, autotune=False, bf16_static=False, state_buffers=0,
+               memory_vars=0):
     """Simulation loaded with a synth.Case (model, sources, receivers).
     The model is uploaded `slab` element layers at a time (all at once when
     None). Returns (simulation, list of receivers that were placed)."""
     g = case.grid
     sim = Simulation(g, precision, kahan_v, kahan_sigma, xcd_remap=xcd_remap,
-                     bf16_static=bf16_static, state_buffers=state_buffers)
+                     bf16_static=bf16_static, state_buffers=state_buffers,
+                     memory_vars=memory_vars)
     nz = g.nz
     step = slab or nz
     for ez0 in range(0, nz, step):
@@ -436,14 +442,14 @@ def _add_channels(sim, case, sources):
 
 
 def build_shared_sources(case, precision="fp32", kahan_v=True, kahan_sigma=False,
-                         slab=None, autotune=False, state_buffers=0):
+                         slab=None, autotune=False, state_buffers=0, memory_vars=0):
     """One simulation per source of `case`, all on one copy of the case's
     model (the first owns it, the others are Simulation.shared): the
     many-sources-one-model workload. Returns (simulations, placed receivers)."""
     from dataclasses import replace
     first, placed = build_case(replace(case, sources=case.sources[:1]), precision, kahan_v,
                                kahan_sigma, slab=slab, autotune=autotune,
-                               state_buffers=state_buffers)
+                               state_buffers=state_buffers, memory_vars=memory_vars)
     sims = [first]
     for s in case.sources[1:]:
         sim = Simulation.shared(first)

@@ -93,6 +93,30 @@ def test_forward_run_matches_oracle(tmp_path, precision, tol):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("memory_vars", [None, 6])
+def test_visco_forward_run_matches_oracle(tmp_path, memory_vars):
+    """A layered visco-elastic model with Q files, run by the host in FP64
+    with its default five memory words per mechanism (the deviatoric part)
+    and with all six, against the oracle's six-word run."""
+    import oracle
+    case = synth.layered(seed=3, nt=160, nx=6, ny=6, nz=4, n_rec=5)
+    cfg = write_config(case, str(tmp_path), precision="fp64")
+    if memory_vars is not None:
+        text = open(cfg).read()
+        open(cfg, "w").write(text.replace("&gpu\n", "&gpu\n memory_vars = %d\n" % memory_vars, 1))
+    r = run(cfg)
+    print(r.stdout[-600:], r.stderr[-600:])
+    assert r.returncode == 0
+    o, recs = oracle.build_case(case)
+    assert o.visco
+    o.step(case.grid.nt)
+    for ri, rec in enumerate(recs):
+        for ci, ch in enumerate(rec.channels):
+            _, _, data = read_sac(os.path.join(str(tmp_path), "XX.00.%s.%s.sac" % (rec.station, ch)))
+            assert rel_l2(data, o.trace(ri, ci)) < 1e-6, (rec.station, ch)
+
+
+@pytest.mark.gpu
 def test_forward_builtin_prem_matches_oracle(tmp_path):
     """model_type 3 (PREM, crust without ocean) built by the host and uploaded
     in FP64, against the oracle run on the model that workflow 'model' writes
@@ -324,6 +348,19 @@ def test_state_buffers_option_checked(tmp_path):
             assert r.returncode == 0, r.stderr
         else:
             assert r.returncode != 0 and "state_buffers must be" in r.stderr
+
+
+def test_memory_vars_option_checked(tmp_path):
+    case = synth.homogeneous(nt=60, nx=6, ny=6, nz=5, n_rec=2)
+    cfg = write_config(case, str(tmp_path))
+    base = open(cfg).read()
+    for value, ok in ((5, True), (6, True), (4, False)):
+        open(cfg, "w").write(base.replace("&gpu\n", "&gpu\n memory_vars = %d\n" % value, 1))
+        r = check(cfg)
+        if ok:
+            assert r.returncode == 0, r.stderr
+        else:
+            assert r.returncode != 0 and "memory_vars must be" in r.stderr
 
 
 def _raw(prefix, start, inc, end, atts):
