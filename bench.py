@@ -38,9 +38,12 @@ FP32_PEAK = 157.3e12       # MI355X vector FP32, flop/s
 # spherical completion and symmetrisation, 90 for the divergence contraction
 # and 37 for its coefficients and the force; FP32 44 for the stress rate, 24
 # for its compensated integration, 6 for the effective stress, 73 for the
-# memory variables and 24 for the tapered stores.
+# memory variables and 24 for the tapered stores. With five memory words per
+# mechanism (the deviatoric part) the stress rate takes 39 and the memory
+# variables 66 (60 for the 15 words, 6 for tau/3 and the deviatoric strain
+# rate).
 ELEM_FP64_FLOP = 241
-ELEM_FP32_FLOP = 171
+ELEM_FP32_FLOP = {6: 171, 5: 159}
 CPU_SAMPLE = dict(nx=24, ny=24, nz=12)   # oracle sample of the same workload
 MISFIT_STEPS = 1200        # FP32 vs FP64 misfit leg after the timing
 
@@ -600,8 +603,9 @@ def main():
     # HBM traffic per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE,
     # calibrated on a plain copy kernel) for this workload, when recorded
     tf = os.path.join(ROOT, "profiles", "traffic_layered_192x192x96_p4_%s.json" % mode)
-    if (g.nx, g.ny, g.nz) == (192, 192, 96) and os.path.exists(tf):
-        roof["traffic"] = json.load(open(tf))["traffic_bytes_per_launch"]["elem_kernel"]
+    tj = json.load(open(tf)) if (g.nx, g.ny, g.nz) == (192, 192, 96) and os.path.exists(tf) else None
+    if tj is not None and tj.get("memory_vars", 6) == nm:
+        roof["traffic"] = tj["traffic_bytes_per_launch"]["elem_kernel"]
         roof["traffic_kind"] = ("FETCH_SIZE + WRITE_SIZE of the element kernel (memory-side "
                                 "bytes below L2, including what the Infinity Cache serves), "
                                 "calibrated on a plain copy kernel; includes the force write")
@@ -635,9 +639,9 @@ def main():
         # the element kernel's arithmetic against the vector peaks: far from
         # both, the kernel is bound by HBM
         f64 = ELEM_FP64_FLOP * N / (elem_ms * 1e-3)
-        f32 = ELEM_FP32_FLOP * N / (elem_ms * 1e-3)
+        f32 = ELEM_FP32_FLOP[nm] * N / (elem_ms * 1e-3)
         out["flop_rate"] = {"kernel": "element kernel", "unit": "TFLOP/s",
-                            "fp64_flop_per_node": ELEM_FP64_FLOP, "fp32_flop_per_node": ELEM_FP32_FLOP,
+                            "fp64_flop_per_node": ELEM_FP64_FLOP, "fp32_flop_per_node": ELEM_FP32_FLOP[nm],
                             "fp64_achieved": f64 / 1e12, "fp64_peak": FP64_PEAK / 1e12,
                             "fp64_frac": f64 / FP64_PEAK,
                             "fp32_achieved": f32 / 1e12, "fp32_peak": FP32_PEAK / 1e12,
