@@ -161,17 +161,19 @@ int s3d_init(s3d_ctx *ctx);
 int s3d_cfl(s3d_ctx *ctx, double *courant);
 
 /* Autotuner: times the element-kernel tile variants (elements per
- * workgroup), the XCD-aware block order, the LDS staging layout and the
- * store order (s3d_set_store_order) on this
+ * workgroup; FP32 and FP64), the XCD-aware block order, and for FP32 the
+ * LDS staging layout and the store order (s3d_set_store_order) on this
  * mesh and keeps the fastest; *tile gets the elements per workgroup chosen
- * and *ms the element kernel time (s3d_get_tuning gives the rest). Needs the state at rest (after s3d_init, before s3d_set_field
- * and s3d_step): the trial launches then leave every field unchanged. No
- * counterpart in the reference. */
+ * and *ms the element kernel time (s3d_get_tuning gives the rest). Needs
+ * the state at rest (after s3d_init, before s3d_set_field and s3d_step):
+ * the trial launches then leave every field unchanged. No counterpart in
+ * the reference. */
 int s3d_autotune(s3d_ctx *ctx, int *tile, double *ms);
 
-/* Element tile of the FP32 step without timing: 0 default, or 1, 2 or 4
- * elements per workgroup (where that fits 1024 lanes). No counterpart in the
- * reference. */
+/* Element tile of the step without timing: 0 default, or 1, 2 or 4
+ * elements per workgroup (where that fits 1024 lanes); FP32 and FP64, not
+ * BF16-static simulations (default tile only). Every tile gives
+ * bit-identical results. No counterpart in the reference. */
 int s3d_set_tile(s3d_ctx *ctx, int tile);
 
 /* LDS layout of the FP32 element kernel without timing: 0 planar (one array

@@ -6,7 +6,9 @@
 #include "../../include/ses3d_hip.h"
 #include "s3d_internal.h"
 #include "s3d_kernels.hip"
+#include "s3d_launch.h"
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 #include <cstdio>
@@ -467,90 +469,31 @@ void ev_begin(s3d_ctx *c, int k)
 }
 void ev_end(s3d_ctx *c, int k) { ev_begin(c, k); }
 
-// Element-tile variants offered to the autotuner: FP32 kernels exist for 1,
-// 2 and 4 elements per workgroup (where that fits 1024 lanes) besides the
-// default; FP64 uses the default tile only.
-// the stress-compensation mode is a kernel template parameter (FP32 only)
-template <int P, int E, typename R, int VM, bool S16, int LL>
-void launch_elem_ll(int ks, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
-{
-    using S = Shape<P, E>;
-    if (ks == 1)
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VM, S16, 1, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
-    else if (ks == 2)
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VM, S16, 2, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
-    else
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VM, S16, 0, LL>), dim3(ge), dim3(S::NT), 0, st, ea);
-}
-
-// stress-compensation mode and kernel variant (elem_body's LL: planar,
-// node-interleaved, planar with the state stored before the divergence) are
-// kernel template parameters (FP32 only; FP64 and the BF16-static kernels
-// use variant 0)
-template <int P, int E, typename R, int VM, bool S16>
-int launch_elem_ks(int ks, int ll, const ElemArgs<R> &ea, hipStream_t st, unsigned ge)
-{
-    using S = Shape<P, E>;
-    if constexpr (std::is_same<R, float>::value) {
-        bool done = false;
-        if constexpr (!S16) {
-            if (ll == 1) {
-                launch_elem_ll<P, E, R, VM, S16, 1>(ks, ea, st, ge);
-                done = true;
-            } else if (ll == 2) {
-                launch_elem_ll<P, E, R, VM, S16, 2>(ks, ea, st, ge);
-                done = true;
-            }
-        }
-        if (!done) launch_elem_ll<P, E, R, VM, S16, 0>(ks, ea, st, ge);
-    } else {
-        hipLaunchKernelGGL((elem_kernel<P, E, R, VM, S16, 0, 0>), dim3(ge), dim3(S::NT), 0, st, ea);
-    }
-    HIPCHK(hipGetLastError());
-    return 0;
-}
-
-template <int P, int E, typename R>
-int launch_elem_e(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st)
-{
-    using S = Shape<P, E>;
-    const unsigned ge = nblocks(c->nel, S::EPB);
-    const int ks = c->o.kahan_sigma, ll = kernel_variant(c);
-    const int vm = c->visco ? c->nm : 0;
-    if constexpr (std::is_same<R, float>::value && E == 0) {
-        if (c->o.bf16_static)   // default tile only
-            return vm == 6 ? launch_elem_ks<P, 0, float, 6, true>(ks, 0, ea, st, ge)
-                 : vm == 5 ? launch_elem_ks<P, 0, float, 5, true>(ks, 0, ea, st, ge)
-                           : launch_elem_ks<P, 0, float, 0, true>(ks, 0, ea, st, ge);
-    }
-    return vm == 6 ? launch_elem_ks<P, E, R, 6, false>(ks, ll, ea, st, ge)
-         : vm == 5 ? launch_elem_ks<P, E, R, 5, false>(ks, ll, ea, st, ge)
-                   : launch_elem_ks<P, E, R, 0, false>(ks, ll, ea, st, ge);
-}
-
-template <int P>
-constexpr bool tile_ok(int e) { return e * (P + 1) * (P + 1) * (P + 1) <= 1024; }
-
+// One element-kernel launch; the instantiation (tile, memory words, stress
+// compensation, kernel variant, BF16 statics) is chosen in s3d_elem.hip.
 template <int P, typename R>
 int launch_elem(s3d_ctx *c, const ElemArgs<R> &ea, hipStream_t st, int epb)
 {
-    if constexpr (std::is_same<R, float>::value) {
-        if (c->o.bf16_static) epb = 0;   // the BF16-static kernel exists for the default tile
-        if (epb == 1) return launch_elem_e<P, 1, R>(c, ea, st);
-        if constexpr (tile_ok<P>(2)) if (epb == 2) return launch_elem_e<P, 2, R>(c, ea, st);
-        if constexpr (tile_ok<P>(4)) if (epb == 4) return launch_elem_e<P, 4, R>(c, ea, st);
-    }
-    return launch_elem_e<P, 0, R>(c, ea, st);
+    ElemLaunch l;
+    l.nel = c->nel;
+    l.vm = c->visco ? c->nm : 0;
+    l.ks = c->o.kahan_sigma;
+    l.ll = kernel_variant(c);
+    l.bf16_static = c->o.bf16_static;
+    l.epb = epb;
+    const hipError_t e = launch_elem_kernel<P, R>(l, ea, st);
+    if (e != hipSuccess) return fail(std::string("element kernel launch: ") + hipGetErrorString(e));
+    return 0;
 }
 
-// Element tiles offered to the autotuner: FP32 kernels exist for 1, 2 and 4
+// Element tiles offered to the autotuner: kernels exist for 1, 2 and 4
 // elements per workgroup (where that fits 1024 lanes) besides the default;
-// FP64 and BF16-static use the default tile only.
+// BF16-static uses the default tile only.
 template <int P>
-std::vector<int> tile_candidates(bool fp32, bool bf16_static)
+std::vector<int> tile_candidates(bool bf16_static)
 {
     std::vector<int> v = {0};
-    if (!fp32 || bf16_static) return v;
+    if (bf16_static) return v;
     for (int e : {1, 2, 4})
         if (e * (P + 1) * (P + 1) * (P + 1) <= 1024 && e != default_epb<P>()) v.push_back(e);
     return v;
@@ -569,7 +512,7 @@ int autotune_p(s3d_ctx *c, int *tile_out, double *ms_out)
     double best = 1e300;
     int best_tile = c->epb, best_remap = c->o.xcd_remap, best_ll = kernel_variant(c);
     const int nll = (c->fp32 && !c->o.bf16_static) ? 3 : 1;
-    for (int tile : tile_candidates<P>(c->fp32, c->o.bf16_static))
+    for (int tile : tile_candidates<P>(c->o.bf16_static))
         for (int ll = 0; ll < nll; ++ll)
             for (int remap = 0; remap < 2; ++remap) {
                 c->o.xcd_remap = remap;
@@ -1117,54 +1060,15 @@ int batch_build_group_e(BatchGroup &G)
     return 0;
 }
 
-template <int P, int E, typename R, int VM, int KS>
-int batch_launch_group_k(const BatchGroup &G, hipStream_t st)
-{
-    using S = Shape<P, E>;
-    if constexpr (std::is_same<R, float>::value) {
-        if (G.ll == 1 || G.ll == 2) {
-            const ElemArgs<R> *args = (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size();
-            if (G.ll == 1)
-                hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VM, KS, 1>), dim3(G.ewg), dim3(S::NT), 0, st,
-                                   args, (const int2 *)G.d_eown);
-            else
-                hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VM, KS, 2>), dim3(G.ewg), dim3(S::NT), 0, st,
-                                   args, (const int2 *)G.d_eown);
-            HIPCHK(hipGetLastError());
-            return 0;
-        }
-    }
-    hipLaunchKernelGGL((elem_kernel_batch<P, E, R, VM, KS, 0>), dim3(G.ewg), dim3(S::NT), 0, st,
-                       (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size(), (const int2 *)G.d_eown);
-    HIPCHK(hipGetLastError());
-    return 0;
-}
-
-template <int P, int E, typename R, int VM>
-int batch_launch_group_v(const BatchGroup &G, hipStream_t st)
-{
-    if constexpr (std::is_same<R, float>::value) {
-        if (G.ks == 1) return batch_launch_group_k<P, E, R, VM, 1>(G, st);
-        if (G.ks == 2) return batch_launch_group_k<P, E, R, VM, 2>(G, st);
-    }
-    return batch_launch_group_k<P, E, R, VM, 0>(G, st);
-}
-
-template <int P, int E, typename R>
-int batch_launch_group_e(const BatchGroup &G, hipStream_t st)
-{
-    return G.vm == 6 ? batch_launch_group_v<P, E, R, 6>(G, st)
-         : G.vm == 5 ? batch_launch_group_v<P, E, R, 5>(G, st)
-                     : batch_launch_group_v<P, E, R, 0>(G, st);
-}
-
+// The group's workgroup map for its tile: FP32 groups take their largest
+// member's tile, FP64 groups the default (as launch_elem_batch does).
 template <int P, typename R>
 int batch_build_group(BatchGroup &G)
 {
     if constexpr (std::is_same<R, float>::value) {
-        if (G.epb == 1) return batch_build_group_e<P, 1, R>(G);
-        if constexpr (tile_ok<P>(2)) if (G.epb == 2) return batch_build_group_e<P, 2, R>(G);
-        if constexpr (tile_ok<P>(4)) if (G.epb == 4) return batch_build_group_e<P, 4, R>(G);
+        if constexpr (extra_tile<P, 1>()) if (G.epb == 1) return batch_build_group_e<P, 1, R>(G);
+        if constexpr (extra_tile<P, 2>()) if (G.epb == 2) return batch_build_group_e<P, 2, R>(G);
+        if constexpr (extra_tile<P, 4>()) if (G.epb == 4) return batch_build_group_e<P, 4, R>(G);
     }
     return batch_build_group_e<P, 0, R>(G);
 }
@@ -1172,12 +1076,17 @@ int batch_build_group(BatchGroup &G)
 template <int P, typename R>
 int batch_launch_group(const BatchGroup &G, hipStream_t st)
 {
-    if constexpr (std::is_same<R, float>::value) {
-        if (G.epb == 1) return batch_launch_group_e<P, 1, R>(G, st);
-        if constexpr (tile_ok<P>(2)) if (G.epb == 2) return batch_launch_group_e<P, 2, R>(G, st);
-        if constexpr (tile_ok<P>(4)) if (G.epb == 4) return batch_launch_group_e<P, 4, R>(G, st);
-    }
-    return batch_launch_group_e<P, 0, R>(G, st);
+    ElemBatchLaunch l;
+    l.vm = G.vm;
+    l.ks = G.ks;
+    l.ll = G.ll;
+    l.epb = G.epb;
+    l.nwg = (unsigned)G.ewg;
+    l.args = (const ElemArgs<R> *)G.d_eargs + (size_t)G.phase * G.sims.size();
+    l.own = (const int2 *)G.d_eown;
+    const hipError_t e = launch_elem_batch<P, R>(l, st);
+    if (e != hipSuccess) return fail(std::string("batched element kernel launch: ") + hipGetErrorString(e));
+    return 0;
 }
 
 template <int P, typename R>
@@ -1735,11 +1644,11 @@ int s3d_set_tile(s3d_ctx *c, int tile)
 {
     if (!c) return fail("null argument");
     if (tile != 0 && tile != 1 && tile != 2 && tile != 4) return fail("tile must be 0, 1, 2 or 4");
-    if (tile != 0 && (!c->fp32 || c->o.bf16_static))
-        return fail("FP64 and BF16-static simulations use the default tile");
+    if (tile != 0 && c->o.bf16_static) return fail("BF16-static simulations use the default tile");
     const int P1 = c->p + 1;
     if (tile * P1 * P1 * P1 > 1024) return fail("tile does not fit 1024 lanes at this order");
-    c->epb = tile;
+    const int dflt = std::max(1, 256 / (P1 * P1 * P1));   // default_epb
+    c->epb = tile == dflt ? 0 : tile;
     drop_graph(c);
     return 0;
 }

@@ -1,7 +1,9 @@
 """Build the gfx950 HIP library and the Fortran host program in-tree.
 
-libses3d_hip.so   hipcc --offload-arch=gfx950, one translation unit
-                  (csrc/s3d_api.hip includes csrc/s3d_kernels.hip)
+libses3d_hip.so   hipcc --offload-arch=gfx950: csrc/s3d_api.hip (the C ABI and
+                  every kernel but the element kernel, csrc/s3d_kernels.hip)
+                  and csrc/s3d_elem.hip once per GLL order 2..7 (the element
+                  kernel's instantiations), compiled in parallel and linked
 bin/copy_probe, bin/contraction_probe
                   stand-alone measurement programs (HBM streams; the
                   derivative contraction on the VALU and on the matrix cores)
@@ -11,6 +13,7 @@ bin/ses3d_nt      flang: the Fortran host (fortran/*.f90) linked against the
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ROOT = os.path.dirname(PKG)
@@ -39,11 +42,48 @@ def _run(cmd, cwd=None):
         raise RuntimeError("command failed: %s" % " ".join(cmd))
 
 
+ELEM_ORDERS = range(2, 8)   # GLL orders with element-kernel instantiations
+
+
+def _run_parallel(cmds):
+    """Run the commands concurrently (at most MAX_JOBS or the CPU count at
+    once); raise if any fails, after all have ended."""
+    def one(cmd):
+        print("+", " ".join(cmd), flush=True)
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        return cmd, r
+    workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    failed = []
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for cmd, r in ex.map(one, cmds):
+            out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+            if out.strip():
+                print(out)
+            if r.returncode != 0:
+                failed.append(" ".join(cmd))
+    if failed:
+        raise RuntimeError("commands failed: %s" % "; ".join(failed))
+
+
 def build_hip():
-    src = os.path.join(PKG, "csrc", "s3d_api.hip")
+    csrc = os.path.join(PKG, "csrc")
+    odir = os.path.join(PKG, "_build", "hip")
+    os.makedirs(odir, exist_ok=True)
+    cflags = [f for f in HIP_FLAGS if f != "-shared"]
+    objs = [os.path.join(odir, "s3d_api.o")]
+    cmds = [[HIPCC] + cflags + ["-c", os.path.join(csrc, "s3d_api.hip"), "-o", objs[0]]]
+    for p in ELEM_ORDERS:
+        o = os.path.join(odir, "s3d_elem_p%d.o" % p)
+        cmds.append([HIPCC] + cflags + ["-DS3D_ELEM_P=%d" % p, "-c", os.path.join(csrc, "s3d_elem.hip"),
+                                        "-o", o])
+        objs.append(o)
+    for o in objs:   # no stale object can be linked if a compile fails
+        if os.path.exists(o):
+            os.remove(o)
+    _run_parallel(cmds)
     out = os.path.join(PKG, "libses3d_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC] + HIP_FLAGS + [src, "-o", tmp])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp])
     os.replace(tmp, out)
     return out
 

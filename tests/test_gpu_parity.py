@@ -308,6 +308,34 @@ def test_set_tile_identical(tile, shape):
         assert np.array_equal(s.get(f), ref.get(f)), (tile, f)
 
 
+@pytest.mark.parametrize("tile", [1, 4])
+@pytest.mark.parametrize("lpd", [2, 4])
+def test_fp64_tiles_identical(tile, lpd):
+    """FP64 element tiles (1 and 4 elements per workgroup besides the
+    default) give the default tile's traces and fields bit for bit, solo
+    and in a batch, whose FP64 groups launch the default tile."""
+    case = synth.layered(seed=9, nt=24, nx=5, ny=5, nz=3, lpd=lpd, n_rec=4)
+    ref, recs = api.build_case(case, precision="fp64")
+    ref.step(24)
+    s, _ = api.build_case(case, precision="fp64")
+    s.set_tile(tile)
+    assert s.tuning()[0] == tile
+    s.step(24)
+    for r in range(len(recs)):
+        for ch in range(3):
+            assert np.array_equal(s.trace(r, ch), ref.trace(r, ch)), (tile, r, ch)
+    for f in (api.VX, api.SXX, api.MXX + 17):
+        assert np.array_equal(s.get(f), ref.get(f)), (tile, f)
+    m, _ = api.build_case(case, precision="fp64")
+    m.set_tile(tile)
+    b = api.Batch([m])
+    b.step(24)
+    b.synchronize()
+    b.close()
+    for r in range(len(recs)):
+        assert np.array_equal(m.trace(r, 2), ref.trace(r, 2)), (tile, r)
+
+
 @pytest.mark.parametrize("tile", [1, 2, 4])
 def test_lds_layouts_identical(tile):
     """The node-interleaved LDS layout stages the same values and contracts
@@ -396,8 +424,12 @@ def test_set_tile_checks():
         s.set_tile(104)
     s.set_tile(2)
     d, _ = api.build_case(elastic, precision="fp64")
+    d.set_tile(4)
+    assert d.tuning()[0] == 4
+    d.set_tile(0)
+    h, _ = api.build_case(elastic, precision="fp32", bf16_static=True)
     with pytest.raises(api.S3dError, match="default tile"):
-        d.set_tile(4)
+        h.set_tile(4)
     with pytest.raises(api.S3dError, match="planar LDS layout"):
         d.set_lds_layout(1)
     with pytest.raises(api.S3dError, match="LDS layout must be"):
