@@ -168,6 +168,19 @@ __device__ __forceinline__ long em_index(long e, int nc, int c, int NP, int node
     return (e * nc + c) * NP + node;
 }
 
+// Element force layout: first the nodes with k < p of every element,
+// [e][c][(p+1)^2 p], then the k = p layers, [e][c][(p+1)^2]. The node kernel
+// reads each part once: an element's k = p layer is the z-1 copy of the
+// owned block above it, gathered a plane of blocks after the rest of the
+// element, and in the plain element-major layout it shared cache lines with
+// what had already been read.
+template <int P>
+__device__ __forceinline__ long force_index(long e, int c, int node, long nel)
+{
+    constexpr int NL = (P + 1) * (P + 1), NM = NL * P;
+    return node < NM ? (e * 3 + c) * NM + node : nel * 3 * NM + (e * 3 + c) * NL + (node - NM);
+}
+
 // Issue the loads of tile `tile` into `s` (no waits). Lanes without an
 // element (the tail of the last tile) read element 0's addresses, so every
 // load is unconditional and in bounds; such lanes store nothing. KS: stress
@@ -619,7 +632,7 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
         if (valid) {
 #pragma unroll
             for (int c = 0; c < 3; ++c)
-                a.force[em_index(ev, 3, c, NP, node)] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
+                a.force[force_index<P>(ev, c, node, a.nel)] = (R)((C)src[6 + c] + Cz * Sz[c] + Cy * Sy[c] + Cx * Sx[c]);
 
             if constexpr (!EARLY) store_state();
         }

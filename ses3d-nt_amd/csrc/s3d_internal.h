@@ -4,7 +4,9 @@
 //   element-local fields, e = ex + nx*(ey + ny*ez), n = i + P1*(j + P1*k):
 //     block fields, element-major f[e][c][n]: the state block (stress, its
 //       compensation and the memory variables in one block, see
-//       state_components), the element force and the moduli lam/mu/tau.
+//       state_components) and the moduli lam/mu/tau; the element force in
+//       two element-major parts, the k < p nodes and the k = p layers
+//       (force_index).
 //       One element's components are one
 //       contiguous run, so the workgroup of a tile of consecutive elements
 //       reads and writes one contiguous region per block field instead of one
@@ -121,7 +123,7 @@ struct ElemArgs {
     gptr<const R> lam, mu, tau;
     gptr<const R> ca, cb, cc;  // anisotropy dt*A, dt*B, dt*C or null
     Static16 s16;              // the same fields in BF16 when lam is null
-    gptr<R> force;             // [3][nel][NP]
+    gptr<R> force;             // element force, k < p nodes then k = p layers (force_index)
     gptr<double> dgrad;        // [nel][6][NP] displacement gradient (xx xy xz yy yz zz) or null
     double dt;                 // time step (displacement gradient increment)
     gptr<const ThTab<coef_t<R>>> tth;   // [nx*P1]
@@ -148,7 +150,7 @@ struct NodeArgs {
     long nblk;                 // (nx+1)(ny+1)(nz+1)
     gptr<R> vx, vy, vz;
     gptr<R> lx, ly, lz;        // compensation terms or null
-    gptr<const R> force;       // [3][nel][NP]
+    gptr<const R> force;       // element force (force_index)
     gptr<const R> dt_inv_mm;   // unique, dt / M
     gptr<const bf16> dt_inv_mm16;  // the same in BF16 when dt_inv_mm is null
     gptr<const double> px, py, pz;

@@ -32,7 +32,8 @@ namespace s3d {
 template <int P, typename R, bool KV>
 __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
 {
-    constexpr int P1 = P + 1, NP = P1 * P1 * P1, PB = P * P * P;
+    constexpr int P1 = P + 1, PB = P * P * P;
+    constexpr int NL = P1 * P1, NM = NL * P;   // force layout, see force_index
     const long b = tid / PB;
     const int o = (int)(tid - b * PB);
     if (b >= a.nblk) return;
@@ -42,37 +43,40 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
     if ((bx == a.nx && li) || (by == a.ny && lj) || (bz == a.nz && lk)) return;
 
     // Copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P,
-    // lk+dz*P), from clamped (always valid) addresses. Only present copies
-    // are loaded (lane-predicated loads: an interior node, the common case,
-    // issues one load instead of eight); absent ones are +0. Adding +0 is
-    // exact: the sum equals the reference's theta -> phi -> r sweep.
+    // lk+dz*P), from clamped (always valid) addresses: dz = 0 in the k < p
+    // part of the force, dz = 1 (lk = 0, local k = p) in the k = p layers
+    // (force_index). Only present copies are loaded (lane-predicated loads:
+    // an interior node, the common case, issues one load instead of eight);
+    // absent ones are +0. Adding +0 is exact: the sum equals the reference's
+    // theta -> phi -> r sweep.
     const bool px0 = bx < a.nx, px1 = li == 0 && bx > 0;
     const bool py0 = by < a.ny, py1 = lj == 0 && by > 0;
     const bool pz0 = bz < a.nz, pz1 = lk == 0 && bz > 0;
     const int ex0 = bx < a.nx ? bx : a.nx - 1, ex1 = bx > 0 ? bx - 1 : 0;
     const int ey0 = by < a.ny ? by : a.ny - 1, ey1 = by > 0 ? by - 1 : 0;
     const int ez0 = bz < a.nz ? bz : a.nz - 1, ez1 = bz > 0 ? bz - 1 : 0;
+    const long top = (long)a.nel * 3 * NM;   // start of the k = p layers
     long addr[8];
     bool pres[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
         const long e = (long)(dx ? ex1 : ex0) + (long)a.nx * ((long)(dy ? ey1 : ey0) + (long)a.ny * (dz ? ez1 : ez0));
-        const int n = (li + dx * P) + P1 * ((lj + dy * P) + P1 * (lk + dz * P));
-        addr[m] = e * 3 * NP + (n < NP ? n : NP - 1);
+        const int i2 = li + dx * P < P ? li + dx * P : P;
+        const int j2 = lj + dy * P < P ? lj + dy * P : P;
+        addr[m] = dz ? top + e * 3 * NL + i2 + P1 * j2 : e * 3 * NM + i2 + P1 * (j2 + P1 * lk);
         pres[m] = (dx ? px1 : px0) && (dy ? py1 : py0) && (dz ? pz1 : pz0);
     }
     R acc[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        const R *f = a.force + c * NP;
         R v[8];
 #pragma unroll
         for (int m = 0; m < 8; ++m) v[m] = R(0);
-        v[0] = f[addr[0]];   // clamped: in bounds also for a padding block
+        v[0] = a.force[addr[0] + c * NM];   // clamped: in bounds also for a padding block
 #pragma unroll
         for (int m = 1; m < 8; ++m)
-            if (pres[m]) v[m] = f[addr[m]];
+            if (pres[m]) v[m] = a.force[addr[m] + c * (m < 4 ? NM : NL)];
         v[0] = pres[0] ? v[0] : R(0);
         acc[c] = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
     }
