@@ -1282,6 +1282,14 @@ int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out)
     if (grid->nx < 1 || grid->ny < 1 || grid->nz < 1) return fail("element counts must be >= 1");
     if (grid->nt < 1) return fail("nt must be >= 1");
     if (grid->taper_width > 0 && grid->taper_slope == 0.0) return fail("taper_slope must be nonzero");
+    {
+        // the kernels index elements in 31 bits and unique nodes in 32
+        const long pb = (long)grid->lpd * grid->lpd * grid->lpd;
+        const long nel = (long)grid->nx * grid->ny * grid->nz;
+        const long nuniq = (long)(grid->nx + 1) * (grid->ny + 1) * (grid->nz + 1) * pb;
+        if (nel > 0x7fffffffL || nuniq > 0xffffffffL)
+            return fail("mesh too large: more than 2^31 - 1 elements or 2^32 - 1 unique nodes");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device available");
     s3d_ctx *c = new s3d_ctx();

@@ -34,13 +34,19 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
 {
     constexpr int P1 = P + 1, PB = P * P * P;
     constexpr int NL = P1 * P1, NM = NL * P;   // force layout, see force_index
-    const long b = tid / PB;
-    const int o = (int)(tid - b * PB);
-    if (b >= a.nblk) return;
-    const int li = o % P, lj = (o / P) % P, lk = o / (P * P);
-    const int nbx = a.nx + 1, nby = a.ny + 1;
-    const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long)nbx * nby));
-    if ((bx == a.nx && li) || (by == a.ny && lj) || (bz == a.nz && lk)) return;
+    if (tid >= a.nblk * PB) return;
+    // 32-bit index arithmetic (s3d_create keeps unique nodes below 2^32 and
+    // elements below 2^31): 64-bit divisions by the run-time block counts
+    // cost the kernel about half of its VALU instructions
+    const unsigned t = (unsigned)tid;
+    const unsigned b = t / PB;
+    const unsigned o = t - b * PB;
+    const unsigned li = o % P, lj = (o / P) % P, lk = o / (P * P);
+    const unsigned nx = a.nx, ny = a.ny, nz = a.nz;
+    const unsigned nbx = nx + 1, nby = ny + 1;
+    const unsigned bq = b / nbx;
+    const unsigned bx = b - bq * nbx, bz = bq / nby, by = bq - bz * nby;
+    if ((bx == nx && li) || (by == ny && lj) || (bz == nz && lk)) return;
 
     // Copies: element (bx-dx, by-dy, bz-dz) at local (li+dx*P, lj+dy*P,
     // lk+dz*P), from clamped (always valid) addresses: dz = 0 in the k < p
@@ -49,22 +55,22 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
     // an interior node, the common case, issues one load instead of eight);
     // absent ones are +0. Adding +0 is exact: the sum equals the reference's
     // theta -> phi -> r sweep.
-    const bool px0 = bx < a.nx, px1 = li == 0 && bx > 0;
-    const bool py0 = by < a.ny, py1 = lj == 0 && by > 0;
-    const bool pz0 = bz < a.nz, pz1 = lk == 0 && bz > 0;
-    const int ex0 = bx < a.nx ? bx : a.nx - 1, ex1 = bx > 0 ? bx - 1 : 0;
-    const int ey0 = by < a.ny ? by : a.ny - 1, ey1 = by > 0 ? by - 1 : 0;
-    const int ez0 = bz < a.nz ? bz : a.nz - 1, ez1 = bz > 0 ? bz - 1 : 0;
+    const bool px0 = bx < nx, px1 = li == 0 && bx > 0;
+    const bool py0 = by < ny, py1 = lj == 0 && by > 0;
+    const bool pz0 = bz < nz, pz1 = lk == 0 && bz > 0;
+    const unsigned ex0 = bx < nx ? bx : nx - 1, ex1 = bx > 0 ? bx - 1 : 0;
+    const unsigned ey0 = by < ny ? by : ny - 1, ey1 = by > 0 ? by - 1 : 0;
+    const unsigned ez0 = bz < nz ? bz : nz - 1, ez1 = bz > 0 ? bz - 1 : 0;
     const long top = (long)a.nel * 3 * NM;   // start of the k = p layers
     long addr[8];
     bool pres[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
         const int dx = m & 1, dy = (m >> 1) & 1, dz = m >> 2;
-        const long e = (long)(dx ? ex1 : ex0) + (long)a.nx * ((long)(dy ? ey1 : ey0) + (long)a.ny * (dz ? ez1 : ez0));
-        const int i2 = li + dx * P < P ? li + dx * P : P;
-        const int j2 = lj + dy * P < P ? lj + dy * P : P;
-        addr[m] = dz ? top + e * 3 * NL + i2 + P1 * j2 : e * 3 * NM + i2 + P1 * (j2 + P1 * lk);
+        const unsigned e = (dx ? ex1 : ex0) + nx * ((dy ? ey1 : ey0) + ny * (dz ? ez1 : ez0));
+        const unsigned i2 = li + dx * P < P ? li + dx * P : P;
+        const unsigned j2 = lj + dy * P < P ? lj + dy * P : P;
+        addr[m] = dz ? top + (long)e * (3 * NL) + (i2 + P1 * j2) : (long)e * (3 * NM) + (i2 + P1 * (j2 + P1 * lk));
         pres[m] = (dx ? px1 : px0) && (dy ? py1 : py0) && (dz ? pz1 : pz0);
     }
     R acc[3];
@@ -88,9 +94,9 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
         im = (R)(float)a.dt_inv_mm16[u];
     R dmp = R(0);   // 1 - taper
     if (a.taper_on) {
-        const int ex = bx < a.nx ? bx : a.nx - 1, i = bx < a.nx ? li : P;
-        const int ey = by < a.ny ? by : a.ny - 1, j = by < a.ny ? lj : P;
-        const int ez = bz < a.nz ? bz : a.nz - 1, k = bz < a.nz ? lk : P;
+        const unsigned ex = bx < nx ? bx : nx - 1, i = bx < nx ? li : P;
+        const unsigned ey = by < ny ? by : ny - 1, j = by < ny ? lj : P;
+        const unsigned ez = bz < nz ? bz : nz - 1, k = bz < nz ? lk : P;
         dmp = (R)taper_damp(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
     }
     R *v[3] = {a.vx, a.vy, a.vz};
