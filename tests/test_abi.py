@@ -72,6 +72,10 @@ def test_bad_arguments_are_rejected():
     bad = synth.Grid(**{**g.__dict__, "lpd": 9})
     with pytest.raises(api.S3dError):
         api.Simulation(bad)
+    # the kernels index unique nodes in 32 bits: 1000^3 p=4 blocks hold 6.4e10
+    huge = synth.Grid(**{**g.__dict__, "nx": 1000, "ny": 1000, "nz": 1000})
+    with pytest.raises(api.S3dError, match="mesh too large"):
+        api.Simulation(huge)
 
 
 def test_fortran_binding_declares_every_entry_point():
