@@ -159,7 +159,8 @@ struct TileState {
     unsigned lo2[3];    // stress compensation, kahan_sigma 2 (packed BF16 pairs)
 };
 
-// Element-major block fields (the state block, element force, lam/mu/tau):
+// Element-major block fields (the state block, lam/mu/tau, the displacement
+// gradient; the element force has its own two-part layout, force_index):
 // component c of element e occupies the NP entries at (e * nc + c) * NP.
 // The state block's parts are addressed through pointers at their first
 // components (sig, sig_lo, mem) with nc = state_components(KS, VM).
