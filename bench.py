@@ -62,6 +62,8 @@ MODES = {
                                      bf16_static=False),
     "fp32+kahan_v+kahan_sigma_bf16": dict(precision="fp32", kahan_v=True, kahan_sigma=2,
                                           bf16_static=False),
+    "fp32+kahan_v_bf16+kahan_sigma_bf16": dict(precision="fp32", kahan_v=2, kahan_sigma=2,
+                                               bf16_static=False),
     "fp32+kahan_v+bf16_static": dict(precision="fp32", kahan_v=True, kahan_sigma=0,
                                      bf16_static=True),
 }
@@ -74,6 +76,8 @@ MODE_TEXT = {
     "fp32+kahan_v+kahan_sigma": "FP32 with compensated velocity and stress integration",
     "fp32+kahan_v+kahan_sigma_bf16": "FP32 with compensated velocity integration and "
                                      "BF16-compensated stress integration",
+    "fp32+kahan_v_bf16+kahan_sigma_bf16": "FP32 with BF16-compensated velocity and stress "
+                                          "integration",
     "fp32+kahan_v+bf16_static": "FP32 with compensated velocity integration and BF16 "
                                 "moduli / 1/M",
 }
@@ -84,7 +88,7 @@ def mode_sizes(mode):
     """(state, velocity compensation, stress compensation, static) bytes."""
     m = MODES[mode]
     s = 8 if m["precision"] == "fp64" else 4
-    return (s, s if m["kahan_v"] else 0, {0: 0, 1: 4, 2: 2}[m["kahan_sigma"]],
+    return (s, {0: 0, 1: s, 2: 2}[int(m["kahan_v"])], {0: 0, 1: 4, 2: 2}[m["kahan_sigma"]],
             2 if m["bf16_static"] else s)
 
 
@@ -315,7 +319,8 @@ def precision_study(args):
     ref_tr = ref_snap = None
     for name in variants:
         t0 = time.time()
-        sim, recs = api.build_case(case, slab=args.slab, **MODES[name])
+        sim, recs = api.build_case(case, slab=args.slab,
+                                   memory_vars=6 if name == "fp64" else args.memory_vars, **MODES[name])
         chunk = 500
         done = 0
         while done < g.nt:
@@ -344,7 +349,8 @@ def precision_study(args):
         log("[study] %s: median %.3e max %.3e snapshot %s" % (name, np.median(e), e.max(), sn))
     print(json.dumps({"study": "precision", "workload": "layered visco-elastic %dx%dx%d p=4"
                       % (g.nx, g.ny, g.nz), "steps": g.nt, "dt": g.dt,
-                      "reference": "FP64 GPU path", "results": res}), flush=True)
+                      "reference": "FP64 GPU path (six memory words)",
+                      "memory_vars": args.memory_vars, "results": res}), flush=True)
 
 
 def batch_bench(args):

@@ -42,7 +42,9 @@ typedef struct s3d_grid {
 enum { S3D_FP64 = 0, S3D_FP32 = 1 };
 typedef struct s3d_options {
     int precision;         /* S3D_FP64 or S3D_FP32                        */
-    int kahan_v;           /* FP32: compensated velocity integration      */
+    int kahan_v;           /* FP32: compensated velocity integration,
+                              0 off, 1 FP32 compensation words, 2 BF16
+                              compensation words                          */
     int kahan_sigma;       /* FP32: compensated stress integration,
                               0 off, 1 FP32 compensation words,
                               2 BF16 compensation words                   */

@@ -171,6 +171,7 @@ struct s3d_ctx {
     int ns = 6;    // components of the state block
     long nel, nn, nblk, nuniq;
     bool fp32 = false, visco = false, aniso = false, inited = false;
+    int kv = 0;                // velocity compensation words: 0 none, 1 FP32, 2 BF16
     int dev = 0;
     hipStream_t stream = nullptr;
     double *stage[7] = {};
@@ -303,7 +304,7 @@ int alloc_state(s3d_ctx *c)
     const size_t rs = rsize(c);
     for (int q = 0; q < 3; ++q) {
         if (dalloc(c, &c->v[q], rs * c->nuniq)) return 1;
-        if (c->fp32 && c->o.kahan_v && dalloc(c, &c->lo[q], rs * c->nuniq)) return 1;
+        if (c->kv && dalloc(c, &c->lo[q], (c->kv == 2 ? sizeof(bf16) : rs) * c->nuniq)) return 1;
     }
     // stress, its compensation and the memory variables: one element-major
     // block of ns words (FP32: 4-byte words, the BF16 compensation in pairs;
@@ -346,6 +347,12 @@ int upload_tables(s3d_ctx *c, const std::vector<double> &sin1)
 size_t rsize(const s3d_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double); }
 
 // the element kernel's variant (elem_body's LL) for the layout and store order
+// the velocity compensation arrays as FP32 or BF16 words (null when the
+// simulation keeps the other kind or none)
+template <typename R>
+R *lo32(const s3d_ctx *c, int q) { return c->kv == 1 ? (R *)c->lo[q] : nullptr; }
+bf16 *lo16(const s3d_ctx *c, int q) { return c->kv == 2 ? (bf16 *)c->lo[q] : nullptr; }
+
 int kernel_variant(const s3d_ctx *c) { return c->early_stores ? 2 : c->lds_layout; }
 
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
@@ -423,7 +430,8 @@ NodeArgs<R> node_args(s3d_ctx *c)
     a.nx = c->g.nx; a.ny = c->g.ny; a.nz = c->g.nz; a.nel = (int)c->nel;
     a.nblk = c->nblk;
     a.vx = (decltype(a.vx))((R *)c->v[0]); a.vy = (decltype(a.vy))((R *)c->v[1]); a.vz = (decltype(a.vz))((R *)c->v[2]);
-    a.lx = (decltype(a.lx))((R *)c->lo[0]); a.ly = (decltype(a.ly))((R *)c->lo[1]); a.lz = (decltype(a.lz))((R *)c->lo[2]);
+    a.lx = (decltype(a.lx))(lo32<R>(c, 0)); a.ly = (decltype(a.ly))(lo32<R>(c, 1)); a.lz = (decltype(a.lz))(lo32<R>(c, 2));
+    a.hx = (decltype(a.hx))(lo16(c, 0)); a.hy = (decltype(a.hy))(lo16(c, 1)); a.hz = (decltype(a.hz))(lo16(c, 2));
     a.force = (decltype(a.force))((const R *)c->force);
     if (c->o.bf16_static)
         a.dt_inv_mm16 = (decltype(a.dt_inv_mm16))((const bf16 *)c->dt_inv_mm);
@@ -441,7 +449,8 @@ RecArgs<R> rec_args(s3d_ctx *c)
     RecArgs<R> a;
     a.nx = c->g.nx; a.ny = c->g.ny; a.nel = (int)c->nel; a.nt = c->g.nt;
     a.vx = (decltype(a.vx))((const R *)c->v[0]); a.vy = (decltype(a.vy))((const R *)c->v[1]); a.vz = (decltype(a.vz))((const R *)c->v[2]);
-    a.lx = (decltype(a.lx))((const R *)c->lo[0]); a.ly = (decltype(a.ly))((const R *)c->lo[1]); a.lz = (decltype(a.lz))((const R *)c->lo[2]);
+    a.lx = (decltype(a.lx))(lo32<R>(c, 0)); a.ly = (decltype(a.ly))(lo32<R>(c, 1)); a.lz = (decltype(a.lz))(lo32<R>(c, 2));
+    a.hx = (decltype(a.hx))(lo16(c, 0)); a.hy = (decltype(a.hy))(lo16(c, 1)); a.hz = (decltype(a.hz))(lo16(c, 2));
     a.sig = (decltype(a.sig))((const R *)c->sig);
     a.ns = c->ns;
     a.rec_e = (decltype(a.rec_e))(c->d_rec_e); a.rec_nch = (decltype(a.rec_nch))(c->d_rec_nch); a.rec_ch = (decltype(a.rec_ch))(c->d_rec_ch);
@@ -553,7 +562,8 @@ CurlArgs<R> curl_args(s3d_ctx *c)
     const int P1 = P + 1;
     a.nx = c->g.nx; a.ny = c->g.ny; a.nrot = c->nrot;
     a.vx = (decltype(a.vx))((const R *)c->v[0]); a.vy = (decltype(a.vy))((const R *)c->v[1]); a.vz = (decltype(a.vz))((const R *)c->v[2]);
-    a.lx = (decltype(a.lx))((const R *)c->lo[0]); a.ly = (decltype(a.ly))((const R *)c->lo[1]); a.lz = (decltype(a.lz))((const R *)c->lo[2]);
+    a.lx = (decltype(a.lx))(lo32<R>(c, 0)); a.ly = (decltype(a.ly))(lo32<R>(c, 1)); a.lz = (decltype(a.lz))(lo32<R>(c, 2));
+    a.hx = (decltype(a.hx))(lo16(c, 0)); a.hy = (decltype(a.hy))(lo16(c, 1)); a.hz = (decltype(a.hz))(lo16(c, 2));
     a.tth = (decltype(a.tth))((const ThTab<double> *)c->tth);
     a.tr = (decltype(a.tr))((const RTab<double> *)c->tr);
     a.rot_e = (decltype(a.rot_e))(c->d_rot_e);
@@ -584,10 +594,12 @@ int launch_step(s3d_ctx *c)
     NodeArgs<R> na = node_args<P, R>(c);
     const unsigned gn = nblocks(c->nuniq, 256);
     ev_begin(c, 1);
-    if (c->lo[0])
-        hipLaunchKernelGGL((node_kernel<P, R, true>), dim3(gn), dim3(256), 0, c->stream, na);
+    if (c->kv == 1)
+        hipLaunchKernelGGL((node_kernel<P, R, 1>), dim3(gn), dim3(256), 0, c->stream, na);
+    else if (c->kv == 2)
+        hipLaunchKernelGGL((node_kernel<P, R, 2>), dim3(gn), dim3(256), 0, c->stream, na);
     else
-        hipLaunchKernelGGL((node_kernel<P, R, false>), dim3(gn), dim3(256), 0, c->stream, na);
+        hipLaunchKernelGGL((node_kernel<P, R, 0>), dim3(gn), dim3(256), 0, c->stream, na);
     ev_end(c, 1);
     HIPCHK(hipGetLastError());
     if (c->nrec_dev > 0) {
@@ -727,7 +739,9 @@ struct ExpandRun {
     static int run(s3d_ctx *c, const void *src, const void *lo, double *dst)
     {
         hipLaunchKernelGGL((expand_unique<P, R>), dim3(nblocks(c->nn, 256)), dim3(256), 0, c->stream,
-                           (const R *)src, (const R *)lo, dst, c->g.nx, c->g.ny, c->g.nz);
+                           (const R *)src, (const R *)(c->kv == 1 ? lo : nullptr),
+                           (const bf16 *)(c->kv == 2 ? lo : nullptr),
+                           dst, c->g.nx, c->g.ny, c->g.nz);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -772,8 +786,9 @@ struct GradRun {
         const long nn = c->nn;
         hipLaunchKernelGGL((grad_rho_kernel<P, R>), dim3(nblocks(nn, 256)), dim3(256), 0, c->stream,
                            (double *)c->grad_rho, f, f + nn, f + 2 * nn, (const R *)c->v[0],
-                           (const R *)c->v[1], (const R *)c->v[2], (const R *)c->lo[0],
-                           (const R *)c->lo[1], (const R *)c->lo[2], c->g.nx, c->g.ny, c->g.nz,
+                           (const R *)c->v[1], (const R *)c->v[2], (const R *)lo32<R>(c, 0),
+                           (const R *)lo32<R>(c, 1), (const R *)lo32<R>(c, 2), (const bf16 *)lo16(c, 0),
+                           (const bf16 *)lo16(c, 1), (const bf16 *)lo16(c, 2), c->g.nx, c->g.ny, c->g.nz,
                            inc_dt, c->jac);
         HIPCHK(hipGetLastError());
         return 0;
@@ -785,7 +800,8 @@ struct GatherRun {
     static int run(s3d_ctx *c, const double *src, void *dst, void *lo)
     {
         hipLaunchKernelGGL((gather_unique<P, R>), dim3(nblocks(c->nuniq, 256)), dim3(256), 0, c->stream,
-                           src, (R *)dst, (R *)lo, c->g.nx, c->g.ny, c->g.nz, c->nblk);
+                           src, (R *)dst, (R *)(c->kv == 1 ? lo : nullptr), (bf16 *)(c->kv == 2 ? lo : nullptr),
+                           c->g.nx, c->g.ny, c->g.nz, c->nblk);
         HIPCHK(hipGetLastError());
         return 0;
     }
@@ -958,7 +974,8 @@ struct BatchGroup {
     int ll = 0;                 // element kernel variant (elem_body's LL)
     int ks = 0;                 // stress compensation mode
     int vm = 0;                 // memory words per mechanism (0 elastic, 6, 5)
-    bool fp32 = false, visco = false, kahan = false;
+    int kv = 0;                 // velocity compensation words (0, 1 FP32, 2 BF16)
+    bool fp32 = false, visco = false;
     std::vector<s3d_ctx *> sims;
     // element and receiver arguments in two sets of sims.size() entries: set
     // `phase` serves the batch's even (0) or odd (1) steps since it was
@@ -1092,11 +1109,14 @@ int batch_launch_group(const BatchGroup &G, hipStream_t st)
 template <int P, typename R>
 int batch_launch_node(const BatchGroup &G, hipStream_t st)
 {
-    if (G.kahan)
-        hipLaunchKernelGGL((node_kernel_batch<P, R, true>), dim3(G.nwg), dim3(256), 0, st,
+    if (G.kv == 1)
+        hipLaunchKernelGGL((node_kernel_batch<P, R, 1>), dim3(G.nwg), dim3(256), 0, st,
+                           (const NodeArgs<R> *)G.d_nargs, (const int2 *)G.d_nown);
+    else if (G.kv == 2)
+        hipLaunchKernelGGL((node_kernel_batch<P, R, 2>), dim3(G.nwg), dim3(256), 0, st,
                            (const NodeArgs<R> *)G.d_nargs, (const int2 *)G.d_nown);
     else
-        hipLaunchKernelGGL((node_kernel_batch<P, R, false>), dim3(G.nwg), dim3(256), 0, st,
+        hipLaunchKernelGGL((node_kernel_batch<P, R, 0>), dim3(G.nwg), dim3(256), 0, st,
                            (const NodeArgs<R> *)G.d_nargs, (const int2 *)G.d_nown);
     HIPCHK(hipGetLastError());
     if (G.rwg > 0) {
@@ -1191,7 +1211,7 @@ int batch_build(s3d_batch *b)
         for (auto &G : b->groups)
             if (G.p == c->p && G.fp32 == c->fp32 && G.visco == c->visco &&
                 G.vm == (c->visco ? c->nm : 0) &&
-                G.kahan == (c->lo[0] != nullptr) && G.ks == c->o.kahan_sigma) {
+                G.kv == c->kv && G.ks == c->o.kahan_sigma) {
                 G.sims.push_back(c);
                 placed = true;
                 break;
@@ -1202,7 +1222,7 @@ int batch_build(s3d_batch *b)
             G.fp32 = c->fp32;
             G.visco = c->visco;
             G.vm = c->visco ? c->nm : 0;
-            G.kahan = c->lo[0] != nullptr;
+            G.kv = c->kv;
             G.ks = c->o.kahan_sigma;
             G.sims.push_back(c);
             b->groups.push_back(G);
@@ -1299,8 +1319,9 @@ int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out)
     c->o = opt ? *opt : d;
     c->fp32 = c->o.precision == S3D_FP32;
     if (c->o.kahan_sigma < 0 || c->o.kahan_sigma > 2) { delete c; return fail("kahan_sigma must be 0, 1 or 2"); }
+    if (c->o.kahan_v < 0 || c->o.kahan_v > 2) { delete c; return fail("kahan_v must be 0, 1 or 2"); }
     if (!c->fp32) { c->o.kahan_v = 0; c->o.kahan_sigma = 0; c->o.bf16_static = 0; }
-    c->o.kahan_v = c->o.kahan_v != 0;
+    c->kv = c->o.kahan_v;
     c->o.bf16_static = c->o.bf16_static != 0;
     if (c->o.state_buffers < 0 || c->o.state_buffers > 2) {
         delete c;
@@ -1395,7 +1416,7 @@ int s3d_create_shared(s3d_ctx *base, s3d_ctx **out)
     c->nm = m->nm;
     c->p = m->p; c->P1 = m->P1; c->NP = m->NP; c->PB = m->PB;
     c->nel = m->nel; c->nn = m->nn; c->nblk = m->nblk; c->nuniq = m->nuniq;
-    c->fp32 = m->fp32; c->visco = m->visco; c->aniso = m->aniso;
+    c->fp32 = m->fp32; c->visco = m->visco; c->aniso = m->aniso; c->kv = m->kv;
     c->q_finite = m->q_finite; c->abc_nonzero = m->abc_nonzero;
     c->dev = m->dev;
     std::copy(m->knots, m->knots + MAXP1, c->knots);

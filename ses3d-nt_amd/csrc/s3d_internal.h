@@ -149,7 +149,8 @@ struct NodeArgs {
     int nx, ny, nz, nel;
     long nblk;                 // (nx+1)(ny+1)(nz+1)
     gptr<R> vx, vy, vz;
-    gptr<R> lx, ly, lz;        // compensation terms or null
+    gptr<R> lx, ly, lz;        // compensation terms in FP32 (kahan_v 1) or null
+    gptr<bf16> hx, hy, hz;     // compensation terms in BF16 (kahan_v 2) or null
     gptr<const R> force;       // element force (force_index)
     gptr<const R> dt_inv_mm;   // unique, dt / M
     gptr<const bf16> dt_inv_mm16;  // the same in BF16 when dt_inv_mm is null
@@ -157,6 +158,15 @@ struct NodeArgs {
     double alpha;
     int taper_on;
 };
+
+// Compensation term of a compensated velocity component at unique node u,
+// from FP32 words (l), BF16 words (h) or none (both null): the component's
+// value is v - lo.
+template <typename R>
+__device__ __forceinline__ double vel_lo(gptr<const R> l, gptr<const bf16> h, long u)
+{
+    return l ? (double)l[u] : h ? (double)(float)h[u] : 0.0;
+}
 
 // taper = exp(-0.1 * prof^2) after corner normalisation
 // (reference src/geometric_paras.f90:303-316). 0.1 is the reference's

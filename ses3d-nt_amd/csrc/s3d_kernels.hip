@@ -26,10 +26,21 @@
 
 namespace s3d {
 
+// vel_lo for plain pointers (the set-up and field kernels)
+template <typename R>
+__device__ __forceinline__ double lo_at(const R *l, const bf16 *h, long u)
+{
+    return l ? (double)l[u] : h ? (double)(float)h[u] : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // node kernel: assembly, 1/M, velocity integration, taper
 // ---------------------------------------------------------------------------
-template <int P, typename R, bool KV>
+// KV: compensated velocity integration, 0 off, 1 FP32 compensation words,
+// 2 BF16 compensation words (the term carries the rounding error of the
+// sum, below half an FP32 unit of it, so 8 bits of it keep the sum within
+// a small fraction of that unit).
+template <int P, typename R, int KV>
 __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
 {
     constexpr int P1 = P + 1, PB = P * P * P;
@@ -100,7 +111,7 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
         dmp = (R)taper_damp(a.px[ex * P1 + i], a.py[ey * P1 + j], a.pz[ez * P1 + k], a.alpha);
     }
     R *v[3] = {a.vx, a.vy, a.vz};
-    if constexpr (KV) {
+    if constexpr (KV == 1) {
         R *l[3] = {a.lx, a.ly, a.lz};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -108,6 +119,15 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
             kahan_add(s, cc, acc[c] * im);
             v[c][u] = s - s * dmp;
             l[c][u] = cc - cc * dmp;
+        }
+    } else if constexpr (KV == 2) {
+        bf16 *h[3] = {a.hx, a.hy, a.hz};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R s = v[c][u], cc = (R)(float)h[c][u];
+            kahan_add(s, cc, acc[c] * im);
+            v[c][u] = s - s * dmp;
+            h[c][u] = (bf16)(float)(cc - cc * dmp);
         }
     } else {
 #pragma unroll
@@ -118,14 +138,14 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
     }
 }
 
-template <int P, typename R, bool KV>
+template <int P, typename R, int KV>
 __global__ void __launch_bounds__(256)
 node_kernel(const NodeArgs<R> a)
 {
     node_body<P, R, KV>(a, (long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-template <int P, typename R, bool KV>
+template <int P, typename R, int KV>
 __global__ void __launch_bounds__(256)
 node_kernel_batch(const NodeArgs<R> *__restrict__ args, const int2 *__restrict__ own)
 {
@@ -153,6 +173,7 @@ struct RecArgs {
     int nx, ny, nel, nt;
     int ns;     // components of the state block
     gptr<const R> vx, vy, vz, lx, ly, lz, sig;
+    gptr<const bf16> hx, hy, hz;  // BF16 velocity compensation (lx.. null) or null
     gptr<const int> rec_rot;      // curl slot per receiver, -1 none (null: no curl receivers)
     gptr<const double> rot_buf;   // per curl slot [10][NP], see curl_strain_kernel
     gptr<const int> rec_e, rec_nch, rec_off, rec_ch;  // channels of receiver r: rec_ch[rec_off[r] + q]
@@ -195,11 +216,11 @@ __device__ __forceinline__ void record_body(const RecArgs<R> &a, int r)
         const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
         const double w = a.rec_vm[(long)r * NP + n];
         // compensated velocity: value = hi - lo
-        const double vxn = (double)a.vx[u] - (a.lx ? (double)a.lx[u] : 0.0);
-        const double vyn = (double)a.vy[u] - (a.ly ? (double)a.ly[u] : 0.0);
+        const double vxn = (double)a.vx[u] - vel_lo(a.lx, a.hx, u);
+        const double vyn = (double)a.vy[u] - vel_lo(a.ly, a.hy, u);
         acc[0] += w * vxn;
         acc[1] += w * vyn;
-        acc[2] += w * ((double)a.vz[u] - (a.lz ? (double)a.lz[u] : 0.0));
+        acc[2] += w * ((double)a.vz[u] - vel_lo(a.lz, a.hz, u));
         const double tr = (double)a.sig[em_index(e, a.ns, XX, NP, n)] + (double)a.sig[em_index(e, a.ns, YY, NP, n)] +
                           (double)a.sig[em_index(e, a.ns, ZZ, NP, n)];
         acc[3] += -((double)(1.0f / 3.0f) * tr) * w;
@@ -265,6 +286,7 @@ template <typename R>
 struct CurlArgs {
     int nx, ny, nrot;
     gptr<const R> vx, vy, vz, lx, ly, lz;
+    gptr<const bf16> hx, hy, hz;   // BF16 velocity compensation (lx.. null) or null
     gptr<const ThTab<double>> tth;
     gptr<const RTab<double>> tr;
     gptr<const int> rot_e;       // element of each curl slot
@@ -273,9 +295,9 @@ struct CurlArgs {
 };
 
 template <typename R>
-__device__ __forceinline__ double curl_vel(gptr<const R> v, gptr<const R> l, long u)
+__device__ __forceinline__ double curl_vel(gptr<const R> v, gptr<const R> l, gptr<const bf16> h, long u)
 {
-    return (double)v[u] - (l ? (double)l[u] : 0.0);
+    return (double)v[u] - vel_lo(l, h, u);
 }
 
 // The strain-rate parts the curl formulas use, at node (i, j, k) of element
@@ -291,23 +313,23 @@ __device__ inline void curl_parts(const CurlArgs<R> &a, int ex, int ey, int ez, 
     for (int q = 0; q < P1; ++q) {
         const double d1 = a.Dx[i * P1 + q];
         const long u = unique_index<P>(ex, ey, ez, q, j, k, a.nx, a.ny);
-        exy = exy + curl_vel(a.vy, a.ly, u) * d1;
-        exz = exz + curl_vel(a.vz, a.lz, u) * d1;
+        exy = exy + curl_vel(a.vy, a.ly, a.hy, u) * d1;
+        exz = exz + curl_vel(a.vz, a.lz, a.hz, u) * d1;
     }
     for (int q = 0; q < P1; ++q) {
         const double d2 = a.Dy[j * P1 + q];
         const long u = unique_index<P>(ex, ey, ez, i, q, k, a.nx, a.ny);
-        eyx = eyx + curl_vel(a.vx, a.lx, u) * d2;
-        eyz = eyz + curl_vel(a.vz, a.lz, u) * d2;
+        eyx = eyx + curl_vel(a.vx, a.lx, a.hx, u) * d2;
+        eyz = eyz + curl_vel(a.vz, a.lz, a.hz, u) * d2;
     }
     for (int q = 0; q < P1; ++q) {
         const double d3 = a.Dz[k * P1 + q];
         const long u = unique_index<P>(ex, ey, ez, i, j, q, a.nx, a.ny);
-        ezx = ezx + curl_vel(a.vx, a.lx, u) * d3;
-        ezy = ezy + curl_vel(a.vy, a.ly, u) * d3;
+        ezx = ezx + curl_vel(a.vx, a.lx, a.hx, u) * d3;
+        ezy = ezy + curl_vel(a.vy, a.ly, a.hy, u) * d3;
     }
     const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
-    const double vxn = curl_vel(a.vx, a.lx, u), vyn = curl_vel(a.vy, a.ly, u);
+    const double vxn = curl_vel(a.vx, a.lx, a.hx, u), vyn = curl_vel(a.vy, a.ly, a.hy, u);
     const double r = a.tr[ez * P1 + k].r, st = a.tth[ex * P1 + i].sin_t, ct = a.tth[ex * P1 + i].cot;
     const double rs = r * st;
     ezx = -ezx;
@@ -384,7 +406,7 @@ __global__ void __launch_bounds__(256) curl_field_kernel(const CurlArgs<R> a, in
 #pragma unroll
     for (int c = 0; c < 6; ++c) part[c] = parts[(e * 6 + c) * NP + n];
     const long u = unique_index<P>(ex, ey, ez, i, j, k, a.nx, a.ny);
-    const double vxn = curl_vel(a.vx, a.lx, u), vyn = curl_vel(a.vy, a.ly, u);
+    const double vxn = curl_vel(a.vx, a.lx, a.hx, u), vyn = curl_vel(a.vy, a.ly, a.hy, u);
     const double r = a.tr[ez * P1 + k].r, st = a.tth[ex * P1 + i].sin_t, ct = a.tth[ex * P1 + i].cot;
     const long o = ex + (long)a.nx * (ey + (long)a.ny * (ez + (long)nz * (i + (long)P1 * (j + (long)P1 * k))));
     out[o] = curl_formula(comp, part, vxn, vyn, r, r * st, ct, st);
@@ -484,7 +506,8 @@ __global__ void lo_pair_clear(unsigned *p, int half, long nel, int NP, int nc)
 template <int P, typename R>
 __global__ void grad_rho_kernel(double *g, const double *fx, const double *fy, const double *fz,
                                 const R *vx, const R *vy, const R *vz, const R *lx, const R *ly,
-                                const R *lz, int nx, int ny, int nz, double inc_dt, double jac)
+                                const R *lz, const bf16 *hx, const bf16 *hy, const bf16 *hz,
+                                int nx, int ny, int nz, double inc_dt, double jac)
 {
     constexpr int P1 = P + 1, NP = P1 * P1 * P1;
     const long nel = (long)nx * ny * nz;
@@ -495,15 +518,15 @@ __global__ void grad_rho_kernel(double *g, const double *fx, const double *fy, c
     int i = (int)(n % P1), j = (int)((n / P1) % P1), k = (int)(n / (P1 * P1));
     int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((long)nx * ny));
     long u = unique_index<P>(ex, ey, ez, i, j, k, nx, ny);
-    const double ax = (double)vx[u] - (lx ? (double)lx[u] : 0.0);
-    const double ay = (double)vy[u] - (ly ? (double)ly[u] : 0.0);
-    const double az = (double)vz[u] - (lz ? (double)lz[u] : 0.0);
+    const double ax = (double)vx[u] - lo_at(lx, hx, u);
+    const double ay = (double)vy[u] - lo_at(ly, hy, u);
+    const double az = (double)vz[u] - lo_at(lz, hz, u);
     g[t] = g[t] + inc_dt * ((fx[t] * ax + fy[t] * ay) + fz[t] * az) / jac;
 }
 
 // unique-node field -> reference layout (every copy)
 template <int P, typename R>
-__global__ void expand_unique(const R *src, const R *lo, double *dst, int nx, int ny, int nz)
+__global__ void expand_unique(const R *src, const R *lo, const bf16 *lo16, double *dst, int nx, int ny, int nz)
 {
     constexpr int P1 = P + 1, NP = P1 * P1 * P1;
     const long nel = (long)nx * ny * nz;
@@ -514,14 +537,12 @@ __global__ void expand_unique(const R *src, const R *lo, double *dst, int nx, in
     int i = (int)(n % P1), j = (int)((n / P1) % P1), k = (int)(n / (P1 * P1));
     int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((long)nx * ny));
     long u = unique_index<P>(ex, ey, ez, i, j, k, nx, ny);
-    double v = (double)src[u];
-    if (lo) v -= (double)lo[u];
-    dst[t] = v;
+    dst[t] = (double)src[u] - lo_at(lo, lo16, u);
 }
 
 // reference layout -> unique-node field, taking the owner element's copy
 template <int P, typename R>
-__global__ void gather_unique(const double *src, R *dst, R *lo, int nx, int ny, int nz, long nblk)
+__global__ void gather_unique(const double *src, R *dst, R *lo, bf16 *lo16, int nx, int ny, int nz, long nblk)
 {
     constexpr int P1 = P + 1, PB = P * P * P;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -541,6 +562,7 @@ __global__ void gather_unique(const double *src, R *dst, R *lo, int nx, int ny, 
     double v = src[(long)n * nel + e];
     dst[t] = (R)v;
     if (lo) lo[t] = (R)(v - (double)(R)v) * R(-1);
+    if (lo16) lo16[t] = (bf16)(float)(-(v - (double)(R)v));
 }
 
 // FP64 -> storage precision (BF16 through the nearest float)

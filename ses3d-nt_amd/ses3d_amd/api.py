@@ -158,8 +158,8 @@ class Simulation:
     def __init__(self, grid, precision="fp32", kahan_v=True, kahan_sigma=False,
                  device=0, xcd_remap=True, bf16_static=False, state_buffers=0,
                  memory_vars=0):
-        """kahan_sigma: False/0 off, True/1 FP32 compensation words,
-        2 or "bf16" BF16 compensation words. bf16_static: store the moduli,
+        """kahan_v, kahan_sigma: False/0 off, True/1 FP32 compensation
+        words, 2 or "bf16" BF16 compensation words (velocity and stress). bf16_static: store the moduli,
         tau and dt/M in BF16 (FP32 only). state_buffers: 2 (or 0, the
         default) read the state from one block and write it to another, 1
         update one block in place (half the state memory). memory_vars: 6
@@ -170,7 +170,8 @@ class Simulation:
                     grid.rad_max, grid.taper_width, grid.taper_slope, grid.dt,
                     grid.nt)
         ks = 2 if kahan_sigma == "bf16" else int(kahan_sigma)
-        o = S3dOptions(FP32 if precision == "fp32" else FP64, int(kahan_v),
+        kv = 2 if kahan_v == "bf16" else int(kahan_v)
+        o = S3dOptions(FP32 if precision == "fp32" else FP64, kv,
                        ks, int(device), int(xcd_remap), int(bf16_static), int(state_buffers),
                        int(memory_vars))
         h = C.c_void_p()

@@ -184,6 +184,7 @@ def test_precision_modes_against_fp64():
     for name, kw in [("kv", dict(kahan_v=True)),
                      ("kv+ks", dict(kahan_v=True, kahan_sigma=True)),
                      ("kv+ks_bf16", dict(kahan_v=True, kahan_sigma="bf16")),
+                     ("kv_bf16+ks_bf16", dict(kahan_v="bf16", kahan_sigma="bf16")),
                      ("kv+bf16_static", dict(kahan_v=True, bf16_static=True))]:
         sim, _ = api.build_case(case, precision="fp32", **kw)
         sim.step(400)
@@ -191,9 +192,10 @@ def test_precision_modes_against_fp64():
         err = np.linalg.norm(t - t64, axis=2)[keep] / norms[keep]
         out[name] = (float(np.median(err)), float(err.max()))
     print(out)
-    for name in ("kv", "kv+ks", "kv+ks_bf16"):
+    for name in ("kv", "kv+ks", "kv+ks_bf16", "kv_bf16+ks_bf16"):
         assert out[name][1] < 1e-5, (name, out[name])
     assert out["kv+ks_bf16"][0] < 2 * out["kv+ks"][0] + 1e-8
+    assert out["kv_bf16+ks_bf16"][0] < 2 * out["kv+ks_bf16"][0] + 1e-8
     assert 1e-5 < out["kv+bf16_static"][0] < 5e-2
 
 
