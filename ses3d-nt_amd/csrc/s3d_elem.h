@@ -646,7 +646,8 @@ __device__ __forceinline__ void elem_body(const ElemArgs<R> &a, int blk)
 // still in L2; in plane order they came back from memory a plane of tiles
 // later. A permutation of the tiles (identity where a tile would straddle
 // two planes, and for the last plane of an odd count): the arithmetic is
-// unchanged. Measured 0.6-1.2 % faster element kernel at 192x192x96.
+// unchanged. Fewer velocity re-reads from memory; the time gain at
+// 192x192x96 is within run-to-run noise (at most about 0.5 %).
 template <int EPB>
 __device__ inline int plane_pair_tile(int t, int nx, int ny, int nz)
 {

@@ -48,7 +48,7 @@ __device__ __forceinline__ void node_body(const NodeArgs<R> &a, long tid)
     if (tid >= a.nblk * PB) return;
     // 32-bit index arithmetic (s3d_create keeps unique nodes below 2^32 and
     // elements below 2^31): 64-bit divisions by the run-time block counts
-    // cost the kernel about half of its VALU instructions
+    // took about 40 % of the kernel's VALU instructions
     const unsigned t = (unsigned)tid;
     const unsigned b = t / PB;
     const unsigned o = t - b * PB;
