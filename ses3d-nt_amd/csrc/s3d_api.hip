@@ -1921,7 +1921,7 @@ int s3d_get_trace(s3d_ctx *c, int rec, int ch, double *out)
 
 // Component of the memory-variable part of the state block that holds field
 // S3D_MXX + 6 * mechanism + comp (comp < 5 in the five-word form).
-int mem_word(const s3d_ctx *c, int field)
+static int mem_word(const s3d_ctx *c, int field)
 {
     const int f = field - S3D_MXX;
     return c->nm == 5 ? 5 * (f / 6) + f % 6 : f;
