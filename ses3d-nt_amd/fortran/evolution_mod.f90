@@ -179,8 +179,8 @@ CONTAINS
                      cf%lon_max, cf%rad_min, cf%rad_max, cf%taper_width, cf%taper_slope, &
                      cf%dt, cf%nt)
         o%precision = MERGE(S3D_FP32, S3D_FP64, cf%precision == 'fp32')
-        o%kahan_v = MERGE(1, 0, cf%kahan_v)
         o%kahan_sigma = MERGE(MERGE(2, 1, cf%kahan_sigma_bf16), 0, cf%kahan_sigma)
+        o%kahan_v = MERGE(MERGE(2, 1, cf%kahan_v_bf16), 0, cf%kahan_v)
         o%bf16_static = MERGE(1, 0, cf%bf16_static)
         o%device = cf%device
         o%state_buffers = cf%state_buffers

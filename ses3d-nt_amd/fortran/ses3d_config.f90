@@ -7,6 +7,7 @@
 !! Groups: &grid, &model, &time, &general (exactly one each), &source_mt,
 !! &source_sf, &source_sac, &receiver (any number, read in file order), and
 !! the optional &gpu group of the GPU library: precision ('fp32' | 'fp64'), kahan_v,
+!! kahan_v_bf16 (velocity compensation words in BF16, off by default),
 !! kahan_sigma, kahan_sigma_bf16 (stress compensation words in BF16),
 !! bf16_static (moduli, tau and dt/M stored in BF16), device, tile (element
 !! tile of the FP32 step: -1 autotune, 0 default, 1, 2 or 4 elements per
@@ -70,7 +71,7 @@ MODULE configuration_mod
         CHARACTER(LEN=fnl) :: log_file_dir
         ! &gpu
         CHARACTER(LEN=8) :: precision
-        LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
+        LOGICAL :: kahan_v, kahan_v_bf16, kahan_sigma, kahan_sigma_bf16, bf16_static
         INTEGER :: device, tile, state_buffers, memory_vars
         TYPE(source_entry), ALLOCATABLE :: sources(:)
         TYPE(receiver_entry), ALLOCATABLE :: receivers(:)
@@ -131,9 +132,9 @@ CONTAINS
         NAMELIST /general/ log_file_dir, event_name, workflow
         ! &gpu
         CHARACTER(LEN=8) :: precision
-        LOGICAL :: kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static
+        LOGICAL :: kahan_v, kahan_v_bf16, kahan_sigma, kahan_sigma_bf16, bf16_static
         INTEGER :: device, tile, state_buffers, memory_vars
-        NAMELIST /gpu/ precision, kahan_v, kahan_sigma, kahan_sigma_bf16, bf16_static, device, tile, &
+        NAMELIST /gpu/ precision, kahan_v, kahan_v_bf16, kahan_sigma, kahan_sigma_bf16, bf16_static, device, tile, &
                        state_buffers, memory_vars
         INTEGER :: lun, ios, n
         TYPE(configuration_typ) :: cf
@@ -220,6 +221,7 @@ CONTAINS
         ! ---- &gpu (optional) ----
         precision = 'fp32'; kahan_v = .TRUE.; kahan_sigma = .TRUE.; device = 0
         kahan_sigma_bf16 = .TRUE.; bf16_static = .FALSE.; tile = -1; state_buffers = 2
+        kahan_v_bf16 = .FALSE.
         memory_vars = 5
         n = count_groups(lun, 'gpu')
         IF (n > 1) CALL abort('Found ' // i2c(n) // ' &gpu-groups in config-file!')
@@ -231,6 +233,7 @@ CONTAINS
             CALL abort("precision must be 'fp32' or 'fp64'")
         cf%precision = precision; cf%kahan_v = kahan_v; cf%kahan_sigma = kahan_sigma
         cf%kahan_sigma_bf16 = kahan_sigma_bf16; cf%bf16_static = bf16_static
+        cf%kahan_v_bf16 = kahan_v_bf16
         cf%device = device
         IF (ALL(tile /= [-1, 0, 1, 2, 4])) CALL abort('tile must be -1 (autotune), 0, 1, 2 or 4')
         cf%tile = tile

@@ -351,6 +351,15 @@ def test_state_buffers_option_checked(tmp_path):
             assert r.returncode != 0 and "state_buffers must be" in r.stderr
 
 
+def test_bf16_velocity_compensation_option_parses(tmp_path):
+    case = synth.homogeneous(nt=60, nx=6, ny=6, nz=5, n_rec=2)
+    cfg = write_config(case, str(tmp_path))
+    base = open(cfg).read()
+    open(cfg, "w").write(base.replace("&gpu\n", "&gpu\n kahan_v_bf16 = .TRUE.\n", 1))
+    r = check(cfg)
+    assert r.returncode == 0, r.stderr
+
+
 def test_memory_vars_option_checked(tmp_path):
     case = synth.homogeneous(nt=60, nx=6, ny=6, nz=5, n_rec=2)
     cfg = write_config(case, str(tmp_path))
