@@ -209,3 +209,25 @@ def test_batch_with_both_forms_matches_solo_runs():
         _same_traces(s, m, len(recs), "member %d" % q)
         for f in (api.VX, api.SXX, api.MXX + 7):
             assert np.array_equal(s.get(f), m.get(f)), (q, f)
+
+
+@pytest.mark.parametrize("precision,kw,tol", [("fp32", {"bf16_static": True}, 1e-5),
+                                              ("fp64", {}, 1e-11)])
+def test_other_kernels_with_five_words(precision, kw, tol):
+    """The BF16-static FP32 kernel and the FP64 kernel at every tile with
+    five memory words follow the six-word runs of the same kind."""
+    case = _case(nt=60)
+    six, recs = api.build_case(case, precision=precision, memory_vars=6, **kw)
+    six.step(60)
+    tiles = (0,) if kw else (0, 1, 4)
+    for tile in tiles:
+        five, _ = api.build_case(case, precision=precision, memory_vars=5, **kw)
+        five.set_tile(tile)
+        five.step(60)
+        for r in range(len(recs)):
+            for ch in range(3):
+                ref = six.trace(r, ch)
+                if np.linalg.norm(ref) > 0:
+                    assert rel_l2(five.trace(r, ch), ref) < tol, (tile, r, ch)
+        five.close()
+    six.close()
