@@ -124,7 +124,10 @@ int s3d_device_count(int *count);
 
 /* Allocate the device state of one simulation.
  * Replaces allocate_geometric_parameters, allocate_model_parameters and
- * allocate_elastic_variables (reference src/ses3d-nt.F90:128-146). */
+ * allocate_elastic_variables (reference src/ses3d-nt.F90:128-146). Meshes
+ * with more than 2^31 - 1 elements or 2^32 - 1 unique nodes are refused
+ * (the kernels index them in 32 bits; such a mesh would not fit on one GPU
+ * anyway). */
 int s3d_create(const s3d_grid *grid, const s3d_options *opt, s3d_ctx **out);
 
 /* Release everything the context owns. Fails while simulations created with
