@@ -497,6 +497,8 @@ def main():
     ap.add_argument("--nx", type=int, default=192)
     ap.add_argument("--ny", type=int, default=192)
     ap.add_argument("--nz", type=int, default=96)
+    ap.add_argument("--lpd", type=int, default=4, choices=range(2, 8),
+                    help="GLL order of the timed mesh (the workload is quoted at 4)")
     ap.add_argument("--slab", type=int, default=4)
     ap.add_argument("--mode", default=DEFAULT_MODE, choices=list(MODES),
                     help="precision mode of the timed simulation")
@@ -563,7 +565,7 @@ def main():
     nt = args.warmup + args.steps
     misfit_steps = 0 if (args.no_fp64 or args.mode == "fp64") else args.misfit_steps
     case = synth.layered(seed=1, nt=max(nt, misfit_steps), nx=args.nx, ny=args.ny, nz=args.nz,
-                         n_rec=256)
+                         lpd=args.lpd, n_rec=256)
     case = replace(case, receivers=ring_receivers(case.sources[0]) + case.receivers)
     g = case.grid
     N = g.nodes
@@ -593,9 +595,9 @@ def main():
     s_state = mode_sizes(mode)[0]
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": None,
-            "kernel": "elem_kernel<4,%s,visco>, %d elements per workgroup, %s LDS layout, %s "
+            "kernel": "elem_kernel<%d,%s,visco>, %d elements per workgroup, %s LDS layout, %s "
                       "workgroup order, state stored %s the divergence"
-                      % (rtype, tile, "node-interleaved" if sims[mode].layout else "planar",
+                      % (g.lpd, rtype, tile, "node-interleaved" if sims[mode].layout else "planar",
                          "XCD-aware" if sims[mode].remap else "plain",
                          "before" if sims[mode].early else "after"),
             "algorithmic_bytes_per_launch": eb,
@@ -609,7 +611,8 @@ def main():
     # HBM traffic per launch measured by rocprofv3 (FETCH_SIZE, WRITE_SIZE,
     # calibrated on a plain copy kernel) for this workload, when recorded
     tf = os.path.join(ROOT, "profiles", "traffic_layered_192x192x96_p4_%s.json" % mode)
-    tj = json.load(open(tf)) if (g.nx, g.ny, g.nz) == (192, 192, 96) and os.path.exists(tf) else None
+    recorded = (g.nx, g.ny, g.nz, g.lpd) == (192, 192, 96, 4)
+    tj = json.load(open(tf)) if recorded and os.path.exists(tf) else None
     if tj is not None and tj.get("memory_vars", 6) == nm:
         roof["traffic"] = tj["traffic_bytes_per_launch"]["elem_kernel"]
         roof["traffic_kind"] = ("FETCH_SIZE + WRITE_SIZE of the element kernel (memory-side "
@@ -625,9 +628,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64" if mode == "fp64" else "f32",
         "data": "synthetic (seeded layered visco-elastic model, GRF vs perturbation)",
         "config": {"workload": "layered visco-elastic section 40x40 deg x 1400 km, "
-                               "%dx%dx%d elements, GLL order 4, 3 SLS mechanisms%s, %s, "
+                               "%dx%dx%d elements, GLL order %d, 3 SLS mechanisms%s, %s, "
                                "one source, 256 surface receivers and 32 on rings around "
-                               "the source" % (g.nx, g.ny, g.nz,
+                               "the source" % (g.nx, g.ny, g.nz, g.lpd,
                                                " (memory variables held as their deviatoric "
                                                "part, 5 words per mechanism)" if nm == 5 else "",
                                                MODE_TEXT[mode]),
@@ -641,9 +644,9 @@ def main():
                           "elem_ms": elem_ms, "node_ms": ms[1] / max(cnt[1], 1),
                           "record_ms": ms[2] / max(cnt[2], 1)},
     }
-    if mode != "fp64":
-        # the element kernel's arithmetic against the vector peaks: far from
-        # both, the kernel is bound by HBM
+    if mode != "fp64" and g.lpd == 4:
+        # the element kernel's arithmetic against the vector peaks (counted
+        # for order 4): far from both, the kernel is bound by HBM
         f64 = ELEM_FP64_FLOP * N / (elem_ms * 1e-3)
         f32 = ELEM_FP32_FLOP[nm] * N / (elem_ms * 1e-3)
         out["flop_rate"] = {"kernel": "element kernel", "unit": "TFLOP/s",
